@@ -1,0 +1,151 @@
+/*
+ * pgw.h - C ABI of the MI355X coherent F-statistic search core.
+ *
+ * This library is a drop-in for the sky-position / spindown loop of the
+ * polgraw-allsky network search (`gwsearch`): everything `job_core()` does for
+ * one sky point (amplitude modulation, barycentric resampling, spindown
+ * demodulation, zero-padded F-statistic FFT, FStat normalisation, threshold and
+ * candidate emission) runs as HIP kernels on one GPU.  The host keeps the
+ * reference's call order main -> search -> job_core; only the body of
+ * job_core moves to the device.
+ *
+ * All signatures use plain C types.  Every function returns an int status
+ * (PGW_OK = 0, negative on error) and never calls exit().  Units and
+ * conventions are the reference's: frequencies in rad/sample, spindowns in
+ * rad/sample^2, DetSSB in samples (light travel time / dt), forward FFT sign
+ * e^{-2 pi i jk/n}, unnormalised.
+ *
+ * Threading: one host thread per handle; distinct handles are independent.
+ */
+#ifndef PGW_H
+#define PGW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGW_NPAR 5              /* record fields, search/network/src-cpu/settings.h:7 (NPAR) */
+#define PGW_NAV 4096            /* FStat block length, search/network/src-cpu/auxi.h:13 (NAVFSTAT) */
+#define PGW_NAVB 4096           /* band-edge constant, search/network/src-cpu/settings.h:12 (NAV) */
+#define PGW_MAX_DETECTORS 8     /* search/network/src-cpu/struct.h:7 */
+#define PGW_MAXL 2048           /* max known lines, search/network/src-cpu/struct.h:13 */
+
+/* status codes */
+#define PGW_OK 0
+#define PGW_ERR_ARG (-1)        /* invalid argument or settings */
+#define PGW_ERR_CAPACITY (-2)   /* caller's candidate buffer too small; *sgnlc holds the count needed */
+#define PGW_ERR_DEVICE (-3)     /* a HIP runtime call failed */
+#define PGW_ERR_NOMEM (-4)      /* host or device allocation failed */
+#define PGW_ERR_UNSUPPORTED (-5)/* size outside what the kernels are built for */
+#define PGW_ERR_NODEVICE (-6)   /* no HIP device visible */
+
+/* arithmetic of the modulated series and the FFTs; phases are FP64 in both */
+#define PGW_PREC_FP64 0
+#define PGW_PREC_FP32 1
+
+/* Search settings: the fields of Search_settings (search/network/src-cpu/struct.h:103-140)
+ * that the search core reads.  Fill with pgw_search_settings(). */
+typedef struct pgw_settings {
+  double fpo;          /* band start frequency [Hz] */
+  double dt;           /* sampling time [s] */
+  double B;            /* bandwidth 0.5/dt */
+  double oms;          /* 2 pi fpo dt */
+  double omr;          /* Earth rotation C_OMEGA_R dt */
+  double Smin, Smax;   /* spindown bounds */
+  double sepsm, cepsm; /* sin/cos of the ecliptic obliquity (from DetSSB.bin) */
+  int nfft;            /* 2^ceil(log2 N) */
+  int N;               /* samples per segment */
+  int nfftf;           /* fftpad * nfft = F-statistic FFT length L */
+  int nmax, nmin;      /* F-statistic bin range [nmin, nmax) */
+  int nd;              /* degrees of freedom (2) */
+  int interpftpad;     /* resampling factor (2) */
+  int fftpad;          /* zero padding (from grid.bin) */
+  int Ninterp;         /* interpftpad * nfft */
+  int nifo;            /* detectors */
+  double M[16];        /* grid-generating matrix, column-major as in grid.bin */
+  int numlines_band;   /* known lines to veto */
+  double lines[PGW_MAXL][2];
+} pgw_settings;
+
+/* One detector's data: the fields of Detector_settings/Signals
+ * (search/network/src-cpu/struct.h:36-55,154-171) that job_core reads. */
+typedef struct pgw_detector {
+  const double *xDat;    /* N samples, exact zeros mark gaps */
+  const double *DetSSB;  /* 3N doubles: detector position w.r.t. the SSB, in samples */
+  double sig2;           /* crf0 * var(xDat), see pgw_sig2() */
+  double sphir, cphir;   /* sin/cos of the sidereal phase phir at t=0 */
+  double amod[9];        /* c1..c9 of the amplitude modulation, see pgw_rogcvir() */
+} pgw_detector;
+
+/* Search_range, search/network/src-cpu/struct.h:70-73 */
+typedef struct pgw_range {
+  int pmr[2], mr[2], nr[2], spndr[2];
+  int pst, mst, nst, sst;
+} pgw_range;
+
+/* The Command_line_opts fields read by job_core (search/network/src-cpu/struct.h:15-32). */
+typedef struct pgw_opts {
+  double trl;        /* F threshold (default 20) */
+  int white_flag;    /* --whitenoise: skip FStat normalisation */
+  int s0_flag;       /* --nospindown */
+  int use_range;     /* nonzero when -range or -addsig was given: spindowns [sst, spndr[1]],
+                        otherwise the range follows from Smin/Smax (jobcore.c:424-439) */
+} pgw_opts;
+
+typedef struct pgw_handle pgw_handle;
+
+/* search_settings() + the size part of plan_fftw()
+ * (search/network/src-cpu/settings.c:16-81, init.c:835-844) for a segment of N samples.
+ * N replaces round(nod*C_SIDDAY/dt) so that solar-day segment lengths can be used. */
+int pgw_search_settings(double fpo, double dt, int N, int fftpad, pgw_settings *sett);
+
+/* rogcvir(), search/network/src-cpu/settings.c:253-275: amplitude modulation
+ * coefficients c1..c9 from latitude ephi and orientation egam [rad]. */
+int pgw_rogcvir(double ephi, double egam, double amod[9]);
+
+/* crf0*var(xDat) as in init_arrays(), search/network/src-cpu/init.c:359-370. */
+double pgw_sig2(const double *xDat, int N);
+
+/* Allocate device memory, upload the detector data and settings once.
+ * Replaces the allocation part of init_arrays()/plan_fftw() (search/network/src-cpu/init.c:326-457,810-862).
+ * precision: PGW_PREC_FP64 (parity path) or PGW_PREC_FP32 (FP32 series and FFTs, FP64 phases). */
+int pgw_init(pgw_handle **h, const pgw_settings *sett, const pgw_detector *ifo, int nifo, int precision);
+
+/* One sky point, all its spindowns: the drop-in for job_core()
+ * (search/network/src-cpu/jobcore.h:22-35, body jobcore.c:189-785).
+ * Candidates are appended to sgnlv (PGW_NPAR doubles each) at index *sgnlc, in the
+ * reference's (ss, bin) order; *FNum is incremented once per template.
+ * Returns PGW_ERR_CAPACITY (and leaves *sgnlc at the count needed) if sgnlv_cap records
+ * would be exceeded.  A sky point outside the grid disk returns PGW_OK with no work. */
+int pgw_job_core(pgw_handle *h, int pm, int mm, int nn, pgw_range *s_range, const pgw_opts *opts,
+                 int *sgnlc, double *sgnlv, int sgnlv_cap, int *FNum);
+
+/* Many sky points in one call: sky[3*i..3*i+2] = (pm, mm, nn).  Equivalent to calling
+ * pgw_job_core for each in order (the spotlight driver's loop,
+ * search/spotlight/src/jobcore.c:79-122), with the per-sky-point host syncs batched. */
+int pgw_job_batch(pgw_handle *h, const int *sky, int nsky, pgw_range *s_range, const pgw_opts *opts,
+                  int *sgnlc, double *sgnlv, int sgnlv_cap, int *FNum);
+
+/* Debug tap: the normalised F-statistic F[nmin..nmax) of one template (pm, mm, nn, ss),
+ * as job_core holds it just before thresholding (jobcore.c:635-651).  F has nmax-nmin entries. */
+int pgw_debug_fstat(pgw_handle *h, int pm, int mm, int nn, int ss, const pgw_opts *opts, double *F);
+
+/* Debug tap: the resampled, noise-weighted series xDatma/xDatmb of detector `det`
+ * for sky point (pm, mm, nn) (jobcore.c:367-393); each array holds N complex values
+ * as interleaved (re, im) doubles. */
+int pgw_debug_resampled(pgw_handle *h, int pm, int mm, int nn, int det, double *xDatma, double *xDatmb);
+
+/* Elapsed device time (ms) of the last pgw_job_core / pgw_job_batch call, from HIP events. */
+double pgw_last_gpu_ms(const pgw_handle *h);
+
+/* Release everything owned by the handle.  Safe on NULL. */
+void pgw_destroy(pgw_handle *h);
+
+/* Human-readable text for a status code. */
+const char *pgw_strerror(int status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PGW_H */

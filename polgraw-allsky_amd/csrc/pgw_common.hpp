@@ -1,0 +1,205 @@
+// pgw_common.hpp - device building blocks shared by the search-core kernels:
+// complex arithmetic, in-register small DFTs (radix 2/4/8/16) and one Stockham
+// autosort stage over data staged in LDS.
+//
+// FFT layout convention (used by every two-pass transform in this library):
+//   L = L1 * L2,  input index n = n1 + L1*n2,  output index k = L2*k1 + k2.
+//   column pass: for each n1, an L2-point DFT over n2, times W_L^{n1 k2},
+//                stored as Z[k2][n1] (row-major, rows of length L1);
+//   row pass:    for each k2, an L1-point DFT over n1 of row Z[k2][.],
+//                giving X[L2*k1 + k2] at Z[k2][k1] ("T-layout").
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pgw {
+
+template <typename T> struct vec2;
+template <> struct vec2<double> { using type = double2; };
+template <> struct vec2<float> { using type = float2; };
+template <typename T> using cplx = typename vec2<T>::type;
+
+template <typename T2> __device__ __forceinline__ T2 mk(decltype(T2::x) a, decltype(T2::x) b) {
+  T2 r; r.x = a; r.y = b; return r;
+}
+template <typename T2> __device__ __forceinline__ T2 cadd(T2 a, T2 b) { return mk<T2>(a.x + b.x, a.y + b.y); }
+template <typename T2> __device__ __forceinline__ T2 csub(T2 a, T2 b) { return mk<T2>(a.x - b.x, a.y - b.y); }
+template <typename T2> __device__ __forceinline__ T2 cmul(T2 a, T2 b) {
+  return mk<T2>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T2> __device__ __forceinline__ T2 conjg(T2 a) { return mk<T2>(a.x, -a.y); }
+// twiddle from a forward-sign table entry: DIR = -1 forward (as stored), +1 backward (conjugate)
+template <int DIR, typename T2> __device__ __forceinline__ T2 tw_dir(T2 w) { return DIR < 0 ? w : conjg(w); }
+// multiply by W_4 = exp(DIR * i pi/2): -i (forward) or +i (backward)
+template <int DIR, typename T2> __device__ __forceinline__ T2 mul_w4(T2 d) {
+  return DIR < 0 ? mk<T2>(d.y, -d.x) : mk<T2>(-d.y, d.x);
+}
+
+// ---------------------------------------------------------------- small DFTs
+// In-place R-point DFT of v[0..R-1], X[q] = sum_r v[r] exp(DIR 2 pi i r q / R).
+template <int DIR, typename T2> __device__ __forceinline__ void dft2(T2* v) {
+  T2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <int DIR, typename T2> __device__ __forceinline__ void dft4(T2& v0, T2& v1, T2& v2, T2& v3) {
+  T2 t0 = cadd(v0, v2), t1 = csub(v0, v2), t2 = cadd(v1, v3), t3 = mul_w4<DIR>(csub(v1, v3));
+  v0 = cadd(t0, t2);
+  v2 = csub(t0, t2);
+  v1 = cadd(t1, t3);
+  v3 = csub(t1, t3);
+}
+
+template <int DIR, typename T2> __device__ __forceinline__ void dft4(T2* v) { dft4<DIR>(v[0], v[1], v[2], v[3]); }
+
+// multiply by exp(DIR 2 pi i K / 16), K = 0..7
+template <int DIR, int K, typename T2> __device__ __forceinline__ T2 rot16(T2 a) {
+  using T = decltype(T2::x);
+  constexpr double cs[8] = {1.0, 0.92387953251128675613, 0.70710678118654752440,
+                            0.38268343236508977173, 0.0, -0.38268343236508977173,
+                            -0.70710678118654752440, -0.92387953251128675613};
+  constexpr double sn[8] = {0.0, 0.38268343236508977173, 0.70710678118654752440,
+                            0.92387953251128675613, 1.0, 0.92387953251128675613,
+                            0.70710678118654752440, 0.38268343236508977173};
+  constexpr T c = (T)cs[K], s = (T)(DIR * sn[K]);
+  return mk<T2>(a.x * c - a.y * s, a.x * s + a.y * c);
+}
+
+template <int DIR, typename T2> __device__ __forceinline__ void dft8(T2* v) {
+  T2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  T2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<DIR>(e0, e1, e2, e3);
+  dft4<DIR>(o0, o1, o2, o3);
+  o1 = rot16<DIR, 2>(o1);
+  o2 = mul_w4<DIR>(o2);
+  o3 = rot16<DIR, 6>(o3);
+  v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+}
+
+template <int DIR, typename T2> __device__ __forceinline__ void dft16(T2* v) {
+  T2 e[8], o[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+  dft8<DIR>(e);
+  dft8<DIR>(o);
+  o[1] = rot16<DIR, 1>(o[1]); o[2] = rot16<DIR, 2>(o[2]); o[3] = rot16<DIR, 3>(o[3]);
+  o[4] = mul_w4<DIR>(o[4]);   o[5] = rot16<DIR, 5>(o[5]); o[6] = rot16<DIR, 6>(o[6]);
+  o[7] = rot16<DIR, 7>(o[7]);
+#pragma unroll
+  for (int k = 0; k < 8; k++) { v[k] = cadd(e[k], o[k]); v[k + 8] = csub(e[k], o[k]); }
+}
+
+template <int R, int DIR, typename T2> __device__ __forceinline__ void dftR(T2* v) {
+  if constexpr (R == 2) dft2<DIR>(v);
+  else if constexpr (R == 4) dft4<DIR>(v);
+  else if constexpr (R == 8) dft8<DIR>(v);
+  else if constexpr (R == 16) dft16<DIR>(v);
+  else static_assert(R == 2, "unsupported radix");
+}
+
+// ------------------------------------------------------------ Stockham stage
+// One radix-R Stockham autosort stage of C interleaved P-point transforms held in
+// LDS as s[e*C + c] (e = element index 0..P-1, c = transform 0..C-1), in place.
+// Ns = product of the radices of the stages already done.  tw[m] = W_P^m (forward sign).
+// All NT threads of the block must call it (two barriers inside).
+template <typename T2, int P, int C, int R, int NT, int DIR>
+__device__ __forceinline__ void stockham_stage(T2* s, int Ns, const T2* __restrict__ tw) {
+  constexpr int NBF = (P / R) * C;               // butterflies in the stage
+  constexpr int BPT = (NBF + NT - 1) / NT;       // per thread
+  T2 v[BPT][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < BPT; q++) {
+    const int b = tid + q * NT;
+    if (NBF % NT == 0 || b < NBF) {
+      const int c = b % C, j = b / C;
+#pragma unroll
+      for (int r = 0; r < R; r++) v[q][r] = s[(j + r * (P / R)) * C + c];
+      const int k = j & (Ns - 1);
+      if (Ns > 1 && k) {
+        const int step = (P / (Ns * R)) * k;
+#pragma unroll
+        for (int r = 1; r < R; r++) v[q][r] = cmul(v[q][r], tw_dir<DIR>(tw[r * step]));
+      }
+      dftR<R, DIR>(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < BPT; q++) {
+    const int b = tid + q * NT;
+    if (NBF % NT == 0 || b < NBF) {
+      const int c = b % C, j = b / C;
+      const int k = j & (Ns - 1);
+      const int idx = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) s[(idx + r * Ns) * C + c] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Radix plans for the supported transform sizes (product of radices = P).
+template <int P> struct RadixPlan;
+template <> struct RadixPlan<16> { static constexpr int n = 1; static constexpr int r[4] = {16, 1, 1, 1}; };
+template <> struct RadixPlan<32> { static constexpr int n = 2; static constexpr int r[4] = {8, 4, 1, 1}; };
+template <> struct RadixPlan<64> { static constexpr int n = 2; static constexpr int r[4] = {8, 8, 1, 1}; };
+template <> struct RadixPlan<128> { static constexpr int n = 2; static constexpr int r[4] = {16, 8, 1, 1}; };
+template <> struct RadixPlan<256> { static constexpr int n = 2; static constexpr int r[4] = {16, 16, 1, 1}; };
+template <> struct RadixPlan<512> { static constexpr int n = 3; static constexpr int r[4] = {8, 8, 8, 1}; };
+template <> struct RadixPlan<1024> { static constexpr int n = 3; static constexpr int r[4] = {16, 16, 4, 1}; };
+template <> struct RadixPlan<2048> { static constexpr int n = 3; static constexpr int r[4] = {16, 16, 8, 1}; };
+template <> struct RadixPlan<4096> { static constexpr int n = 3; static constexpr int r[4] = {16, 16, 16, 1}; };
+
+// Full P-point transform of C interleaved columns in LDS (natural order in and out).
+template <typename T2, int P, int C, int NT, int DIR>
+__device__ __forceinline__ void lds_fft(T2* s, const T2* __restrict__ tw) {
+  using RP = RadixPlan<P>;
+  int Ns = 1;
+  stockham_stage<T2, P, C, RP::r[0], NT, DIR>(s, Ns, tw);
+  Ns *= RP::r[0];
+  if constexpr (RP::n > 1) {
+    stockham_stage<T2, P, C, RP::r[1], NT, DIR>(s, Ns, tw);
+    Ns *= RP::r[1];
+  }
+  if constexpr (RP::n > 2) {
+    stockham_stage<T2, P, C, RP::r[2], NT, DIR>(s, Ns, tw);
+    Ns *= RP::r[2];
+  }
+  if constexpr (RP::n > 3) {
+    stockham_stage<T2, P, C, RP::r[3], NT, DIR>(s, Ns, tw);
+  }
+}
+
+// W_L^m from the two-level table: hi[m >> 10] * lo[m & 1023] (forward sign).
+template <typename T2> __device__ __forceinline__ T2 tw_big(const T2* __restrict__ hi, const T2* __restrict__ lo, uint32_t m) {
+  return cmul(hi[m >> 10], lo[m & 1023u]);
+}
+
+// Wave-level (64 lanes) reductions in a fixed butterfly order: deterministic.
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+  return x;
+}
+__device__ __forceinline__ float wave_sum(float x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+__device__ __forceinline__ float wave_max(float x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = fmaxf(x, __shfl_xor(x, off, 64));
+  return x;
+}
+
+}  // namespace pgw

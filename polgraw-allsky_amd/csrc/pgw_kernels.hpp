@@ -1,0 +1,548 @@
+// pgw_kernels.hpp - HIP kernels of the coherent F-statistic search core (gfx950).
+//
+// Sky-point stage (once per sky point, amortised over its spindowns):
+//   k_sky_prologue    modvir + shft + first demodulation    (jobcore.c:270-313, settings.c:281-325)
+//   k_fft_col/k_fft_row + loaders   resampling FFTs          (jobcore.c:319-362)
+//   k_untranspose     T-layout -> natural order, times interpftpad/Ninterp
+//   k_spline_u/k_spline_y2/k_spline_eval  natural cubic spline (auxi.c:55-122), /sig2 (jobcore.c:386-389)
+//   k_aabb            noise-weighted sums aa, bb              (jobcore.c:375-393)
+// Template stage (per spindown, the hot loop):
+//   k_spin_col        phase modulation fused into the column pass of the F-statistic FFT
+//                     (jobcore.c:456-627)
+//   k_spin_row        row pass + F = |xa|^2/aa + |xb|^2/bb + FStat block partials (jobcore.c:629-651)
+//   k_threshold       FStat means, normalisation, run-max threshold, candidates (jobcore.c:659-695)
+#pragma once
+#include "pgw_common.hpp"
+
+namespace pgw {
+
+constexpr int NT = 256;          // threads per block everywhere
+constexpr int PGW_NAV_DEV = 4096; // FStat block length (NAVFSTAT, auxi.h:13)
+constexpr int SPL_K = 64;        // spline chunk per thread
+constexpr int SPL_W = 64;        // spline warm-up (|coefficient| ~ 0.268: 0.268^64 ~ 1e-37)
+constexpr int SPL_NTAB = 64;     // spline recursion coefficients tabulated below this index
+
+struct SkyDet {
+  double c[9];
+  double cosalfr, sinalfr, c2d, c2sd, sindel, cosdel;
+  double nS0, nS1, nS2, shft1, sig2;
+};
+
+struct SkyParams {
+  SkyDet det[8];
+  double het0, oms;
+  int N, nfft, nifo;
+};
+
+// Expressions evaluated in the reference's operation order without fused multiply-add
+// (HIP compiles with -ffp-contract=fast-honor-pragmas, so the pragma holds after inlining).
+__device__ __forceinline__ double dot3_nc(double a0, double b0, double a1, double b1, double a2, double b2) {
+#pragma clang fp contract(off)
+  return a0 * b0 + a1 * b1 + a2 * b2;
+}
+__device__ __forceinline__ double lin2_nc(double a0, double b0, double a1, double b1) {
+#pragma clang fp contract(off)
+  return a0 * b0 + a1 * b1;
+}
+
+// ------------------------------------------------------------- sky stage --
+
+// grid (ceil(nfft/NT), nifo).  R[2d][i] = xDat aa e^{-i phase}, R[2d+1][i] = xDat bb e^{-i phase}
+// for i < N, zero for N <= i < nfft; shft, shftf per detector; block partial sums of aa^2, bb^2.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_sky_prologue(SkyParams p, const double* __restrict__ xDat,
+                                                     const double* __restrict__ DetSSB,
+                                                     const double* __restrict__ cosmodf,
+                                                     const double* __restrict__ sinmodf, cplx<T>* __restrict__ R,
+                                                     double* __restrict__ shft, double* __restrict__ shftf,
+                                                     double* __restrict__ paa, double* __restrict__ pbb) {
+  using T2 = cplx<T>;
+  const int d = blockIdx.y;
+  const int i = blockIdx.x * NT + threadIdx.x;
+  const SkyDet& q = p.det[d];
+  double a2 = 0.0, b2 = 0.0;
+  T2 va = mk<T2>(0, 0), vb = mk<T2>(0, 0);
+  if (i < p.N) {
+    const double cm = cosmodf[i], sm = sinmodf[i];
+    // modvir, settings.c:309-322
+    const double c = q.cosalfr * cm + q.sinalfr * sm;
+    const double s = q.sinalfr * cm - q.cosalfr * sm;
+    const double c2s = 2. * (c * c), cs = c * s;
+    const double aa = q.c[0] * (2. - q.c2d) * c2s + q.c[1] * (2. - q.c2d) * 2. * cs + q.c[2] * q.c2sd * c +
+                      q.c[3] * q.c2sd * s - q.c[0] * (2. - q.c2d) + q.c[4] * q.c2d;
+    const double bb = q.c[5] * q.sindel * c2s + q.c[6] * q.sindel * 2. * cs + q.c[7] * q.cosdel * c +
+                      q.c[8] * q.cosdel * s - q.c[5] * q.sindel;
+    const double* D = DetSSB + (size_t)d * 3 * p.N + 3 * (size_t)i;
+    // shft = n.DetSSB and the phase in the reference's operation order, no contraction:
+    // shftf[0] must come out exactly 0 (the spline returns 0 for x < 0)
+    const double sh = dot3_nc(q.nS0, D[0], q.nS1, D[1], q.nS2, D[2]);
+    shft[(size_t)d * p.N + i] = sh;
+    shftf[(size_t)d * p.N + i] = sh - q.shft1;
+    double sp, cp;
+    sincos(lin2_nc(p.het0, (double)i, p.oms, sh), &sp, &cp);
+    const double x = xDat[(size_t)d * p.N + i];
+    const double xa = x * aa, xb = x * bb;
+    va = mk<T2>((T)(xa * cp), (T)(xa * -sp));
+    vb = mk<T2>((T)(xb * cp), (T)(xb * -sp));
+    a2 = aa * aa;
+    b2 = bb * bb;
+  }
+  if (i < p.nfft) {
+    R[(size_t)(2 * d) * p.nfft + i] = va;
+    R[(size_t)(2 * d + 1) * p.nfft + i] = vb;
+  }
+  __shared__ double red[2][NT / 64];
+  a2 = wave_sum(a2);
+  b2 = wave_sum(b2);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) { red[0][w] = a2; red[1][w] = b2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0.0, sb = 0.0;
+    for (int k = 0; k < NT / 64; k++) { sa += red[0][k]; sb += red[1][k]; }
+    paa[(size_t)d * gridDim.x + blockIdx.x] = sa;
+    pbb[(size_t)d * gridDim.x + blockIdx.x] = sb;
+  }
+}
+
+// one block: aa = sum_d (sum_i aa_d^2)/sig2_d, bb likewise (jobcore.c:375-393)
+__global__ __launch_bounds__(NT) void k_aabb(SkyParams p, const double* __restrict__ paa,
+                                             const double* __restrict__ pbb, int nb, double* __restrict__ aabb) {
+  __shared__ double red[2][NT];
+  double aa = 0.0, bb = 0.0;
+  for (int d = 0; d < p.nifo; d++) {
+    double sa = 0.0, sb = 0.0;
+    for (int k = threadIdx.x; k < nb; k += NT) { sa += paa[(size_t)d * nb + k]; sb += pbb[(size_t)d * nb + k]; }
+    red[0][threadIdx.x] = sa;
+    red[1][threadIdx.x] = sb;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) { red[0][threadIdx.x] += red[0][threadIdx.x + s]; red[1][threadIdx.x] += red[1][threadIdx.x + s]; }
+      __syncthreads();
+    }
+    aa += red[0][0] / p.det[d].sig2;
+    bb += red[1][0] / p.det[d].sig2;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { aabb[0] = aa; aabb[1] = bb; }
+}
+
+// ----------------------------------------------------- generic 2-pass FFT --
+
+// Plain loader: x[b][n] for n < len, zero beyond.
+template <typename T2> struct LoadPlain {
+  const T2* src;
+  int len, stride;
+  __device__ T2 operator()(int b, int n) const {
+    return n < len ? src[(size_t)b * stride + n] : mk<T2>(0, 0);
+  }
+};
+
+// Zero-insert loader for the inverse resampling FFT (jobcore.c:334-352): reads the forward
+// spectrum X (length nf, T-layout with row length L1f and L2f rows) as
+//   Y[n] = X[n] (n < nyqst), 0 (nyqst <= n < nyqst + ni - nf), X[n - (ni - nf)] (beyond).
+template <typename T2> struct LoadZeroInsert {
+  const T2* src;
+  int nf, ni, nyqst, L1f, L2f;
+  __device__ T2 operator()(int b, int n) const {
+    int k;
+    if (n < nyqst) k = n;
+    else if (n < nyqst + ni - nf) return mk<T2>(0, 0);
+    else k = n - (ni - nf);
+    return src[(size_t)b * nf + (size_t)(k % L2f) * L1f + k / L2f];
+  }
+};
+
+// Column pass: grid (L1/C, nbatch).  Z[b][k2][n1] = W_L^{n1 k2} sum_n2 x[b][n1 + L1 n2] W_{P2}^{n2 k2}.
+template <typename T, int P2, int C, int DIR, typename Loader>
+__global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T>* __restrict__ Z, int L1,
+                                                const cplx<T>* __restrict__ twc, const cplx<T>* __restrict__ hi,
+                                                const cplx<T>* __restrict__ lo, int twshift) {
+  using T2 = cplx<T>;
+  __shared__ T2 s[P2 * C];
+  const int b = blockIdx.y;
+  const int n1_0 = blockIdx.x * C;
+  const size_t L = (size_t)L1 * P2;
+#pragma unroll
+  for (int q = 0; q < (P2 * C) / NT; q++) {
+    const int e = threadIdx.x + q * NT;
+    const int n2 = e / C, c = e % C;
+    s[e] = ld(b, n1_0 + c + L1 * n2);
+  }
+  __syncthreads();
+  lds_fft<T2, P2, C, NT, DIR>(s, twc);
+#pragma unroll
+  for (int q = 0; q < (P2 * C) / NT; q++) {
+    const int e = threadIdx.x + q * NT;
+    const int k2 = e / C, c = e % C;
+    const uint32_t m = (uint32_t)(n1_0 + c) * (uint32_t)k2;
+    const T2 w = tw_dir<DIR>(tw_big(hi, lo, m << twshift));
+    Z[(size_t)b * L + (size_t)k2 * L1 + n1_0 + c] = cmul(s[e], w);
+  }
+}
+
+// Row pass, in place: grid (L2, nbatch); row k2 of Z[b] becomes X[L2 k1 + k2] at Z[b][k2][k1].
+template <typename T, int P1, int DIR>
+__global__ __launch_bounds__(NT) void k_fft_row(cplx<T>* __restrict__ Z, int L2, const cplx<T>* __restrict__ twr) {
+  using T2 = cplx<T>;
+  __shared__ T2 s[P1];
+  T2* row = Z + (size_t)blockIdx.y * P1 * L2 + (size_t)blockIdx.x * P1;
+#pragma unroll
+  for (int q = 0; q < P1 / NT; q++) s[threadIdx.x + q * NT] = row[threadIdx.x + q * NT];
+  __syncthreads();
+  lds_fft<T2, P1, 1, NT, DIR>(s, twr);
+#pragma unroll
+  for (int q = 0; q < P1 / NT; q++) row[threadIdx.x + q * NT] = s[threadIdx.x + q * NT];
+}
+
+// T-layout (rows of L1, L2 rows) -> natural order y[b][j] = scale * X[j], j < count.
+// grid (ceil(count/ (32*L2)) ... ) handled as tiles of 32 x 32 over the [L2][L1] matrix.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_untranspose(const cplx<T>* __restrict__ Z, cplx<T>* __restrict__ y, int L1, int L2,
+                                                    int count, T scale) {
+  using T2 = cplx<T>;
+  __shared__ T2 tile[32][33];
+  const int b = blockIdx.z;
+  const int k1_0 = blockIdx.x * 32, k2_0 = blockIdx.y * 32;
+  const size_t L = (size_t)L1 * L2;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int k2 = k2_0 + r, k1 = k1_0 + tx;
+    if (k2 < L2 && k1 < L1) tile[r][tx] = Z[(size_t)b * L + (size_t)k2 * L1 + k1];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int k1 = k1_0 + r, k2 = k2_0 + tx;
+    const long j = (long)L2 * k1 + k2;
+    if (k2 < L2 && k1 < L1 && j < count) {
+      const T2 v = tile[tx][r];
+      y[(size_t)b * L + j] = mk<T2>(v.x * scale, v.y * scale);
+    }
+  }
+}
+
+// ----------------------------------------------------------------- spline --
+// Natural cubic spline on n unit-spaced knots (auxi.c:55-81), chunked: each thread owns SPL_K
+// outputs and starts its recursion SPL_W samples early from a zero state (exact when the
+// chunk is within SPL_W of the true start).  invp_i, c_i = -0.5 invp_i tabulated for i < SPL_NTAB.
+struct SplineTab {
+  double invp[SPL_NTAB];
+  double c[SPL_NTAB];
+  double invp_inf, c_inf;
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_spline_u(const cplx<T>* __restrict__ y, size_t ystride, int n,
+                                                 const SplineTab* __restrict__ tab, double2* __restrict__ u,
+                                                 size_t ustride) {
+  const int b = blockIdx.y;
+  const int a = (blockIdx.x * NT + threadIdx.x) * SPL_K;
+  if (a >= n - 1) return;
+  const int e = min(a + SPL_K, n - 1);
+  const cplx<T>* Y = y + (size_t)b * ystride;
+  double2* U = u + (size_t)b * ustride;
+  int i = (a - SPL_W <= 1) ? 1 : a - SPL_W;
+  if (a == 0) U[0] = make_double2(0.0, 0.0);
+  double ur = 0.0, ui = 0.0;
+  double ym_r = (double)Y[i - 1].x, ym_i = (double)Y[i - 1].y;
+  double y0_r = (double)Y[i].x, y0_i = (double)Y[i].y;
+  for (; i < e; i++) {
+    const double yp_r = (double)Y[i + 1].x, yp_i = (double)Y[i + 1].y;
+    const double invp = i < SPL_NTAB ? tab->invp[i] : tab->invp_inf;
+    const double dr = ym_r - 2. * y0_r + yp_r, di = ym_i - 2. * y0_i + yp_i;
+    ur = (-.5 * ur + 3. * dr) * invp;
+    ui = (-.5 * ui + 3. * di) * invp;
+    if (i >= a) U[i] = make_double2(ur, ui);
+    ym_r = y0_r; ym_i = y0_i; y0_r = yp_r; y0_i = yp_i;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_spline_y2(const double2* __restrict__ u, size_t stride, int n,
+                                                  const SplineTab* __restrict__ tab, double2* __restrict__ y2) {
+  const int b = blockIdx.y;
+  const int a = (blockIdx.x * NT + threadIdx.x) * SPL_K;
+  if (a >= n) return;
+  const int e = min(a + SPL_K, n);  // outputs [a, e)
+  const double2* U = u + (size_t)b * stride;
+  double2* Y2 = y2 + (size_t)b * stride;
+  if (e == n) { Y2[n - 1] = make_double2(0.0, 0.0); Y2[n] = make_double2(0.0, 0.0); }
+  int k = (e + SPL_W >= n - 1) ? n - 2 : e + SPL_W - 1;
+  double zr = 0.0, zi = 0.0;  // y2[k+1]
+  for (; k >= a; k--) {
+    const double c = k < SPL_NTAB ? tab->c[k] : tab->c_inf;
+    const double2 uk = U[k];
+    zr = c * zr + uk.x;
+    zi = c * zi + uk.y;
+    if (k < e) Y2[k] = make_double2(zr, zi);
+  }
+}
+
+// out[b][i] = splint(y, y2, n, interpftpad (i - shftf[i])) / sig2   (auxi.c:83-120, jobcore.c:386-389)
+// batch b = 2d + {0: a, 1: b}; output stored as xm[d][{a,b}][i].
+template <typename T, typename TO>
+__global__ __launch_bounds__(NT) void k_spline_eval(const cplx<T>* __restrict__ y, size_t ystride,
+                                                    const double2* __restrict__ y2, size_t y2stride,
+                                                    const double* __restrict__ shftf, int N, int interpftpad,
+                                                    SkyParams p, cplx<TO>* __restrict__ xm) {
+  const int b = blockIdx.y, d = b >> 1;
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= N) return;
+  const int n = interpftpad * N;
+  const double x = interpftpad * (i - shftf[(size_t)d * N + i]);
+  double vr = 0.0, vi = 0.0;
+  if (!(x < 0 || x > n - 1)) {
+    const int klo = (int)floor(x), khi = klo + 1;
+    const double A = khi - x, B = x - klo;
+    const cplx<T>* Y = y + (size_t)b * ystride;
+    const double2* Y2 = y2 + (size_t)b * y2stride;
+    const double ca = A * A * A - A, cb = B * B * B - B;
+    const double2 ylo = make_double2((double)Y[klo].x, (double)Y[klo].y);
+    // khi == n only at x == n-1 exactly, where its weight B is 0
+    const double2 yhi = khi < n ? make_double2((double)Y[khi].x, (double)Y[khi].y) : make_double2(0.0, 0.0);
+    const double2 zlo = Y2[klo], zhi = khi < n ? Y2[khi] : make_double2(0.0, 0.0);
+    vr = A * ylo.x + B * yhi.x + (ca * zlo.x + cb * zhi.x) / 6.0;
+    vi = A * ylo.y + B * yhi.y + (ca * zlo.y + cb * zhi.y) / 6.0;
+  }
+  const double sig2 = p.det[d].sig2;
+  xm[(size_t)b * N + i] = mk<cplx<TO>>((TO)(vr / sig2), (TO)(vi / sig2));
+}
+
+// ---------------------------------------------------------- template stage --
+
+struct SpinParams {
+  double M0, M4, M5, M9, M13;
+  int nn, mm, ss0, s0_flag;
+  int N, L1, nifo, twshift;
+};
+
+__device__ __forceinline__ void spin_params(const SpinParams& p, int ss, double& het1, double& sgnlt1) {
+  // jobcore.c:456,466-467 evaluated without contraction so the values equal the host's
+#pragma clang fp contract(off)
+  sgnlt1 = p.s0_flag ? 0.0 : (double)ss * p.M5 + (double)p.nn * p.M9 + (double)p.mm * p.M13;
+  het1 = fmod((double)ss * p.M4, p.M0);
+  if (het1 < 0) het1 += p.M0;
+}
+
+template <typename T> struct PhaseRot;
+template <> struct PhaseRot<double> {
+  // exp(-i phase) in FP64
+  __device__ static __forceinline__ double2 eval(double ph) {
+    double s, c;
+    sincos(ph, &s, &c);
+    return make_double2(c, -s);
+  }
+};
+template <> struct PhaseRot<float> {
+  // FP64 phase reduced modulo 2 pi in FP64, then FP32 sin/cos of the reduced angle
+  __device__ static __forceinline__ float2 eval(double ph) {
+    const double k = rint(ph * 0.15915494309189533577);
+    const double r = fma(-k, 6.283185307179586232, ph);
+    const double rr = fma(-k, 2.4492935982947064e-16, r);
+    float s, c;
+    sincosf((float)rr, &s, &c);
+    return make_float2(c, -s);
+  }
+};
+
+// Column pass of the F-statistic FFT with the spindown demodulation fused into its load
+// (jobcore.c:473-627): for template t (ss = ss0 + t), x_a[n] = sum_d xm[d][a][n] e^{-i phi_d(n)},
+// phi_d(n) = het1 n + sgnlt1 (n^2 + 2 n shft_d[n]), zero for n >= N; same for x_b.
+// grid (L1/C, ntemplates).  Output Z[t][{a,b}][k2][n1].
+template <typename T, int P2, int C>
+__global__ __launch_bounds__(NT) void k_spin_col(SpinParams p, const cplx<T>* __restrict__ xm,
+                                                 const double* __restrict__ shft, cplx<T>* __restrict__ Z,
+                                                 const cplx<T>* __restrict__ twc, const cplx<T>* __restrict__ hi,
+                                                 const cplx<T>* __restrict__ lo) {
+  using T2 = cplx<T>;
+  __shared__ T2 sa[P2 * C];
+  __shared__ T2 sb[P2 * C];
+  const int t = blockIdx.y;
+  const int n1_0 = blockIdx.x * C;
+  const int L1 = p.L1, N = p.N;
+  const size_t L = (size_t)L1 * P2;
+  double het1, sgnlt1;
+  spin_params(p, p.ss0 + t, het1, sgnlt1);
+#pragma unroll
+  for (int q = 0; q < (P2 * C) / NT; q++) {
+    const int e = threadIdx.x + q * NT;
+    const int n2 = e / C, c = e % C;
+    const int n = n1_0 + c + L1 * n2;
+    T2 xa = mk<T2>(0, 0), xb = mk<T2>(0, 0);
+    if (n < N) {
+      const double dn = (double)n;
+      for (int d = 0; d < p.nifo; d++) {
+        const double sh = shft[(size_t)d * N + n];
+        // _tmp1 = i^2 + 2 i shft, phase = het1 i + sgnlt1 _tmp1 (jobcore.c:294,530), uncontracted
+        const double tmp1 = lin2_nc(dn, dn, (double)(2 * n), sh);
+        const T2 ex = PhaseRot<T>::eval(lin2_nc(het1, dn, sgnlt1, tmp1));
+        const T2 va = xm[((size_t)(2 * d)) * N + n];
+        const T2 vb = xm[((size_t)(2 * d + 1)) * N + n];
+        xa = cadd(xa, cmul(va, ex));
+        xb = cadd(xb, cmul(vb, ex));
+      }
+    }
+    sa[e] = xa;
+    sb[e] = xb;
+  }
+  __syncthreads();
+  lds_fft<T2, P2, C, NT, -1>(sa, twc);
+  lds_fft<T2, P2, C, NT, -1>(sb, twc);
+  T2* Za = Z + (size_t)t * 2 * L;
+  T2* Zb = Za + L;
+#pragma unroll
+  for (int q = 0; q < (P2 * C) / NT; q++) {
+    const int e = threadIdx.x + q * NT;
+    const int k2 = e / C, c = e % C;
+    const uint32_t m = (uint32_t)(n1_0 + c) * (uint32_t)k2;
+    const T2 w = tw_big(hi, lo, m << p.twshift);
+    const size_t o = (size_t)k2 * L1 + n1_0 + c;
+    Za[o] = cmul(sa[e], w);
+    Zb[o] = cmul(sb[e], w);
+  }
+}
+
+struct RowParams {
+  int L2, nmin, nmax, nblk, nparts;
+};
+
+// Row pass + F-statistic + FStat partials: grid (L2, ntemplates).
+// F(k) for k = L2 k1 + k2 in [nmin, nmax) is stored at Ft[t][k2][k1]; psum/pmax[t][part][k2]
+// hold the row's sum and max of F over each FStat block (part < nblk) and the raw tail.
+template <typename T, int P1>
+__global__ __launch_bounds__(NT) void k_spin_row(RowParams p, const cplx<T>* __restrict__ Z,
+                                                 const double* __restrict__ aabb, const cplx<T>* __restrict__ twr,
+                                                 T* __restrict__ Ft, T* __restrict__ psum, T* __restrict__ pmax) {
+  using T2 = cplx<T>;
+  constexpr int Q = P1 / NT;
+  __shared__ T2 s[P1];
+  const int k2 = blockIdx.x, t = blockIdx.y;
+  const size_t L = (size_t)P1 * p.L2;
+  const T2* rowa = Z + (size_t)t * 2 * L + (size_t)k2 * P1;
+  const T2* rowb = rowa + L;
+  const T aa = (T)aabb[0], bb = (T)aabb[1];
+  T F[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) s[threadIdx.x + q * NT] = rowa[threadIdx.x + q * NT];
+  __syncthreads();
+  lds_fft<T2, P1, 1, NT, -1>(s, twr);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const T2 v = s[threadIdx.x + q * NT];
+    F[q] = (v.x * v.x + v.y * v.y) / aa;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < Q; q++) s[threadIdx.x + q * NT] = rowb[threadIdx.x + q * NT];
+  __syncthreads();
+  lds_fft<T2, P1, 1, NT, -1>(s, twr);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const T2 v = s[threadIdx.x + q * NT];
+    F[q] = F[q] + (v.x * v.x + v.y * v.y) / bb;
+  }
+  __syncthreads();
+  T* Fs = reinterpret_cast<T*>(s);
+  T* Frow = Ft + (size_t)t * L + (size_t)k2 * P1;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int k1 = threadIdx.x + q * NT;
+    const int k = p.L2 * k1 + k2;
+    const bool ok = (k >= p.nmin && k < p.nmax);
+    if (ok) Frow[k1] = F[q];
+    Fs[k1] = ok ? F[q] : (T)0;
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int part = w; part < p.nparts; part += NT / 64) {
+    const int lok = p.nmin + PGW_NAV_DEV * part;
+    const int hik = (part < p.nblk) ? lok + PGW_NAV_DEV : p.nmax;
+    const int jlo = max(0, (lok - k2 + p.L2 - 1) / p.L2);
+    const int jhi = min((hik - k2 + p.L2 - 1) / p.L2, P1);
+    T sum = 0, mx = 0;
+    for (int j = jlo + lane; j < jhi; j += 64) {
+      const T f = Fs[j];
+      sum += f;
+      mx = f > mx ? f : mx;
+    }
+    sum = wave_sum(sum);
+    mx = wave_max(mx);
+    if (lane == 0) {
+      psum[((size_t)t * p.nparts + part) * p.L2 + k2] = sum;
+      pmax[((size_t)t * p.nparts + part) * p.L2 + k2] = mx;
+    }
+  }
+}
+
+struct Cand {
+  int ss, ii;
+  double F;
+};
+
+struct ThrParams {
+  int L1, L2, nmin, nmax, nblk, nparts, white, ss0, cap;
+  double trl;
+};
+
+// FStat normalisation + run-max threshold (auxi.c:214-240, jobcore.c:659-695): one block per template.
+// Only parts whose max/mu exceeds trl are scanned; a run is emitted by the thread owning its first bin.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_threshold(ThrParams p, const T* __restrict__ Ft, const T* __restrict__ psum,
+                                                  const T* __restrict__ pmax, double* __restrict__ mu_out,
+                                                  Cand* __restrict__ cand, int* __restrict__ ncand) {
+  constexpr int MAXP = 256;
+  __shared__ double mu[MAXP];
+  __shared__ int flag[MAXP];
+  const int t = blockIdx.x;
+  const size_t L = (size_t)p.L1 * p.L2;
+  const T* F = Ft + (size_t)t * L;
+  for (int part = threadIdx.x; part < p.nparts; part += NT) {
+    double sum = 0.0, mx = 0.0;
+    const T* ps = psum + ((size_t)t * p.nparts + part) * p.L2;
+    const T* pm = pmax + ((size_t)t * p.nparts + part) * p.L2;
+    for (int k2 = 0; k2 < p.L2; k2++) {
+      sum += (double)ps[k2];
+      mx = fmax(mx, (double)pm[k2]);
+    }
+    double m = 1.0;
+    if (part < p.nblk && !p.white) m = sum / (2. * PGW_NAV_DEV);
+    mu[part] = m;
+    flag[part] = (mx / m > p.trl);
+    mu_out[(size_t)t * p.nparts + part] = m;
+  }
+  __syncthreads();
+  for (int part = 0; part < p.nparts; part++) {
+    if (!flag[part]) continue;  // uniform: flag in LDS
+    const int lok = p.nmin + PGW_NAV_DEV * part;
+    const int hik = (part < p.nblk) ? lok + PGW_NAV_DEV : p.nmax;
+    for (int k = lok + threadIdx.x; k < hik; k += NT) {
+      const double fn = (double)F[(size_t)(k % p.L2) * p.L1 + k / p.L2] / mu[part];
+      if (!(fn > p.trl)) continue;
+      bool start = (k == p.nmin);
+      if (!start) {
+        const int kp = k - 1;
+        const int pp = min((kp - p.nmin) / PGW_NAV_DEV, p.nparts - 1);
+        const double fp = (double)F[(size_t)(kp % p.L2) * p.L1 + kp / p.L2] / mu[pp];
+        start = !(fp > p.trl);
+      }
+      if (!start) continue;
+      int ii = k;
+      double Fc = fn;
+      for (int kk = k + 1; kk < p.nmax; kk++) {
+        const int pk = min((kk - p.nmin) / PGW_NAV_DEV, p.nparts - 1);
+        const double v = (double)F[(size_t)(kk % p.L2) * p.L1 + kk / p.L2] / mu[pk];
+        if (!(v > p.trl)) break;
+        if (v >= Fc) { ii = kk; Fc = v; }
+      }
+      const int idx = atomicAdd(ncand, 1);
+      if (idx < p.cap) {
+        Cand c;
+        c.ss = p.ss0 + t;
+        c.ii = ii;
+        c.F = Fc;
+        cand[idx] = c;
+      }
+    }
+  }
+}
+
+}  // namespace pgw

@@ -1,0 +1,121 @@
+"""The HIP search core (libpgw_hip.so) through its C ABI (include/pgw.h).
+
+Raises at load time if the library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from . import abi
+
+LIB_PATH = Path(__file__).resolve().parent / "libpgw_hip.so"
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lib = None
+
+
+class PGWError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        msg = lib().pgw_strerror(status).decode() if _lib is not None else str(status)
+        super().__init__(f"{what}: {msg} ({status})")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(str(LIB_PATH))
+        L.pgw_search_settings.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(abi.Settings)]
+        L.pgw_rogcvir.argtypes = [C.c_double, C.c_double, _dp]
+        L.pgw_sig2.restype = C.c_double
+        L.pgw_sig2.argtypes = [_dp, C.c_int]
+        L.pgw_init.argtypes = [C.POINTER(C.c_void_p), C.POINTER(abi.Settings), C.POINTER(abi.Detector),
+                               C.c_int, C.c_int]
+        L.pgw_job_core.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Range),
+                                   C.POINTER(abi.Opts), _ip, _dp, C.c_int, _ip]
+        L.pgw_job_batch.argtypes = [C.c_void_p, _ip, C.c_int, C.POINTER(abi.Range), C.POINTER(abi.Opts),
+                                    _ip, _dp, C.c_int, _ip]
+        L.pgw_debug_fstat.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts), _dp]
+        L.pgw_debug_resampled.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.pgw_last_gpu_ms.restype = C.c_double
+        L.pgw_last_gpu_ms.argtypes = [C.c_void_p]
+        L.pgw_destroy.argtypes = [C.c_void_p]
+        L.pgw_strerror.restype = C.c_char_p
+        L.pgw_strerror.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_job_core", "pgw_job_batch",
+            "pgw_debug_fstat", "pgw_debug_resampled", "pgw_last_gpu_ms", "pgw_destroy", "pgw_strerror"]
+
+
+class Handle:
+    """One segment resident on the GPU (pgw_init ... pgw_destroy)."""
+
+    def __init__(self, seg, precision: int = abi.PREC_FP64, lines=None):
+        self.seg = seg
+        self.precision = precision
+        self.sett = abi.make_settings(seg, lines)
+        self.dets = abi.make_detectors(seg)
+        self.h = C.c_void_p()
+        st = lib().pgw_init(C.byref(self.h), C.byref(self.sett), self.dets, len(seg.dets), precision)
+        if st != 0:
+            raise PGWError(st, "pgw_init")
+
+    def close(self):
+        if self.h:
+            lib().pgw_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def job_core(self, pm, mm, nn, rng, opts, cap=1 << 16):
+        sgnlv = np.zeros(abi.NPAR * cap)
+        sgnlc, fnum = C.c_int(0), C.c_int(0)
+        st = lib().pgw_job_core(self.h, pm, mm, nn, C.byref(rng), C.byref(opts), C.byref(sgnlc),
+                                sgnlv.ctypes.data_as(_dp), cap, C.byref(fnum))
+        if st != 0:
+            raise PGWError(st, "pgw_job_core")
+        return sgnlv[:abi.NPAR * sgnlc.value].reshape(-1, abi.NPAR), fnum.value
+
+    def job_batch(self, sky, rng, opts, cap=1 << 18):
+        sky = np.ascontiguousarray(np.asarray(sky, dtype=np.int32).reshape(-1, 3))
+        sgnlv = np.zeros(abi.NPAR * cap)
+        sgnlc, fnum = C.c_int(0), C.c_int(0)
+        st = lib().pgw_job_batch(self.h, sky.ctypes.data_as(_ip), len(sky), C.byref(rng), C.byref(opts),
+                                 C.byref(sgnlc), sgnlv.ctypes.data_as(_dp), cap, C.byref(fnum))
+        if st != 0:
+            raise PGWError(st, "pgw_job_batch")
+        return sgnlv[:abi.NPAR * sgnlc.value].reshape(-1, abi.NPAR), fnum.value
+
+    def fstat(self, pm, mm, nn, ss, opts):
+        F = np.zeros(self.sett.nmax - self.sett.nmin)
+        st = lib().pgw_debug_fstat(self.h, pm, mm, nn, ss, C.byref(opts), F.ctypes.data_as(_dp))
+        if st < 0:
+            raise PGWError(st, "pgw_debug_fstat")
+        return F if st == 0 else None
+
+    def resampled(self, pm, mm, nn, det):
+        N = self.sett.N
+        a = np.zeros(2 * N)
+        b = np.zeros(2 * N)
+        st = lib().pgw_debug_resampled(self.h, pm, mm, nn, det, a.ctypes.data_as(_dp), b.ctypes.data_as(_dp))
+        if st < 0:
+            raise PGWError(st, "pgw_debug_resampled")
+        if st != 0:
+            return None
+        return a.view(np.complex128), b.view(np.complex128)
+
+    @property
+    def last_gpu_ms(self) -> float:
+        return lib().pgw_last_gpu_ms(self.h)

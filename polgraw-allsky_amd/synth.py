@@ -1,0 +1,263 @@
+"""Seeded synthetic inputs for the coherent F-statistic search core.
+
+Everything a segment needs, generated in memory instead of read from the
+reference's files (``xdatc_*.bin``, ``DetSSB.bin``, ``grid.bin``):
+
+* white Gaussian noise per detector (numpy PCG64 generator, seeded);
+* a circular-orbit + sidereal-rotation ephemeris in the ``DetSSB`` convention
+  (detector position w.r.t. the solar-system barycentre in units of c*dt,
+  equatorial frame; gen2day/get_barycenter.c:51-84, gen2day/ExtractBand.c:336-342);
+* a grid-generating matrix ``M`` with the zero pattern of the reference's
+  grid files (M1=M2=M3=M6=M7=M11=0), M0 = 2 pi / (fftpad*nfft), the other
+  entries scaled from testdata/test-gpu-cpu/001/grid.bin to the segment length;
+* an injected signal following add_signal() (search/network/src-cpu/init.c:562-642)
+  with amplitudes from sigen.c:348-357.
+
+This is input generation, not part of the search path.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+RAD_TO_DEG = 180.0 / math.pi
+C_OMEGA_R = 7.2921151467064e-5          # settings.h:31
+C_EPSMA = 84381.448 / 3600.0 / RAD_TO_DEG  # settings.h:20
+C_SPEED_OF_LIGHT = 299792.458           # km/s
+C_AU = 1.49597870691e8                  # km
+C_EARTH_R = 6378.140                    # km
+C_YEARSEC = 365.25 * 86400.0
+
+# Detector sites: latitude, longitude, orientation gamma [rad]
+# (search/network/src-cpu/settings.c:174-232; physical constants of the sites)
+SITES = {
+    "H1": ((46 + (27 + 18.528 / 60.0) / 60.0) / RAD_TO_DEG,
+           -(119 + (24 + 27.5657 / 60.0) / 60.0) / RAD_TO_DEG,
+           170.9994 / RAD_TO_DEG),
+    "L1": ((30 + (33 + 46.4196 / 60.0) / 60.0) / RAD_TO_DEG,
+           -(90 + (46 + 27.2654 / 60.0) / 60.0) / RAD_TO_DEG,
+           242.7165 / RAD_TO_DEG),
+    "V1": ((43.0 + 37.0 / 60.0 + 53.0880 / 3600.0) / RAD_TO_DEG,
+           (10.0 + 30.0 / 60.0 + 16.1885 / 3600.0) / RAD_TO_DEG,
+           (135.0 - (19.0 + 25.0 / 60.0 + 57.96 / 3600.0)) / RAD_TO_DEG),
+}
+
+# grid.bin of testdata/test-gpu-cpu/001 (N=86164, nfft=2^17, fftpad=1), column-major
+_FIXTURE_N = 86164
+_FIXTURE_M = {
+    0: 4.79368996e-05, 4: 1.06749473e-04, 8: 8.71359861e-03, 12: -1.03495650e-02,
+    5: -1.07995468e-09, 9: 6.07898667e-10, 13: 1.30440156e-09,
+    10: -8.96932636e+01, 14: 8.72851497e+01, 15: 1.11152781e+02,
+}
+
+
+def settings_dict(N: int, dt: float = 2.0, fftpad: int = 1, fpo: float | None = None,
+                  band: int = 666) -> dict:
+    """Sizes from search_settings() (settings.c:16-81) for a segment of N samples."""
+    if fpo is None:
+        fpo = 10.0 + 0.96875 * band * (0.5 / dt)  # init.c:240
+    B = 0.5 / dt
+    nfft = 1 << int(math.ceil(math.log(N) / math.log(2.0)))
+    return dict(
+        N=N, dt=dt, fpo=fpo, B=B, oms=2.0 * math.pi * fpo * dt, omr=C_OMEGA_R * dt,
+        nfft=nfft, fftpad=fftpad, nfftf=fftpad * nfft, interpftpad=2, Ninterp=2 * nfft,
+        nmin=int(fftpad * 4096 * B), nmax=int((nfft // 2 - 4096 * B) * fftpad), nd=2,
+        Smax=2.0 * math.pi * 0.5e-8 * dt * dt, Smin=2.0 * math.pi * 0.5e-9 * dt * dt,
+    )
+
+
+def grid_matrix(N: int, nfft: int, fftpad: int) -> np.ndarray:
+    """Synthetic grid-generating matrix (16 doubles, column-major like grid.bin)."""
+    r = _FIXTURE_N / float(N)
+    M = np.zeros(16)
+    M[0] = 2.0 * math.pi / (fftpad * nfft)
+    for k in (4, 8, 12):
+        M[k] = _FIXTURE_M[k] / _FIXTURE_M[0] * M[0]
+    for k in (5, 9, 13):
+        M[k] = _FIXTURE_M[k] * r * r
+    for k in (10, 14, 15):
+        M[k] = _FIXTURE_M[k] * r
+    return M
+
+
+def rogcvir(ephi: float, egam: float) -> np.ndarray:
+    """Amplitude-modulation coefficients c1..c9 (settings.c:253-275, JKS98 Eqs. 12-13)."""
+    s2g, c2g = math.sin(2 * egam), math.cos(2 * egam)
+    sp, cp = math.sin(ephi), math.cos(ephi)
+    return np.array([
+        0.25 * s2g * (1 + sp * sp), -0.5 * c2g * sp, 0.5 * s2g * math.sin(2 * ephi),
+        -c2g * cp, 0.75 * s2g * cp * cp, c2g * sp,
+        0.5 * s2g * (1.0 + sp * sp), c2g * cp, 0.5 * s2g * math.sin(2 * ephi)])
+
+
+def modvir(sinal, cosal, sindel, cosdel, sphir, cphir, c, cosmodf, sinmodf):
+    """a(t), b(t) of JKS98 (settings.c:281-325), vectorised."""
+    cosalfr = cosal * cphir + sinal * sphir
+    sinalfr = sinal * cphir - cosal * sphir
+    c2d = cosdel * cosdel
+    c2sd = sindel * cosdel
+    cc = cosalfr * cosmodf + sinalfr * sinmodf
+    ss = sinalfr * cosmodf - cosalfr * sinmodf
+    c2s = 2.0 * cc * cc
+    cs = cc * ss
+    aa = (c[0] * (2.0 - c2d) * c2s + c[1] * (2.0 - c2d) * 2.0 * cs + c[2] * c2sd * cc
+          + c[3] * c2sd * ss - c[0] * (2.0 - c2d) + c[4] * c2d)
+    bb = (c[5] * sindel * c2s + c[6] * sindel * 2.0 * cs + c[7] * cosdel * cc
+          + c[8] * cosdel * ss - c[5] * sindel)
+    return aa, bb
+
+
+def ephemeris(N: int, dt: float, site: str, orbit_phase: float, phir: float) -> np.ndarray:
+    """DetSSB (3N doubles, samples): circular orbit in the ecliptic rotated by the
+    obliquity into equatorial coordinates, plus the detector's diurnal motion."""
+    lat = SITES[site][0]
+    t = np.arange(N, dtype=np.float64) * dt
+    r_orb = C_AU / (C_SPEED_OF_LIGHT * dt)
+    r_e = C_EARTH_R / (C_SPEED_OF_LIGHT * dt)
+    lam = orbit_phase + 2.0 * math.pi * t / C_YEARSEC
+    x_ecl, y_ecl = r_orb * np.cos(lam), r_orb * np.sin(lam)
+    ce, se = math.cos(C_EPSMA), math.sin(C_EPSMA)
+    lst = phir + C_OMEGA_R * t
+    out = np.empty((N, 3))
+    out[:, 0] = x_ecl + r_e * math.cos(lat) * np.cos(lst)
+    out[:, 1] = y_ecl * ce + r_e * math.cos(lat) * np.sin(lst)
+    out[:, 2] = y_ecl * se + r_e * math.sin(lat)
+    return out.reshape(-1)
+
+
+def ast2lin(alfa: float, delta: float, epsm: float):
+    """Hemisphere and linear sky coordinates (auxi.c:26-53)."""
+    be0 = math.cos(epsm) * math.sin(alfa) * math.cos(delta) + math.sin(epsm) * math.sin(delta)
+    be1 = math.cos(alfa) * math.cos(delta)
+    d1 = math.asin(be0 * math.sin(epsm) + math.sqrt(max(0.0, 1.0 - be0 * be0 - be1 * be1)) * math.cos(epsm)) - delta
+    pm = 1 if abs(d1) < 10.0 * np.finfo(float).eps else 2
+    return pm, (be0, be1)
+
+
+@dataclass
+class Segment:
+    sett: dict
+    M: np.ndarray
+    dets: list
+    xDat: list
+    DetSSB: list
+    phir: list
+    epsm: float
+    amod: list
+    sig2: list
+    injection: dict = field(default_factory=dict)
+    range: dict = field(default_factory=dict)
+
+
+def _sig2(x: np.ndarray) -> float:
+    """crf0 * var(x) (init.c:359-370, auxi.c:124-139)."""
+    N = len(x)
+    nz = int(np.count_nonzero(x == 0.0))
+    crf0 = N / (N - nz)
+    mean = np.sum(x) / N
+    return crf0 * float(np.sum((x - mean) ** 2) / (N - 1))
+
+
+def make_segment(N: int = 86400, dets=("H1",), sigmas=(1.0,), dt: float = 2.0, fftpad: int = 1,
+                 seed: int = 1, h0: float = 0.1, inject: bool = True, gap_frac: float = 0.0,
+                 nsky: tuple = (16, 16), nspin: int = 512, band: int = 666) -> Segment:
+    """One seeded segment for a network of detectors, with one injection and a
+    search range of nsky sky points x nspin spindowns centred on it."""
+    rng = np.random.default_rng(seed)
+    sett = settings_dict(N, dt=dt, fftpad=fftpad, band=band)
+    M = grid_matrix(N, sett["nfft"], fftpad)
+    orbit_phase = rng.uniform(0, 2 * math.pi)
+    phir0 = rng.uniform(0, 2 * math.pi)
+    t = np.arange(N, dtype=np.float64)
+    cosmodf, sinmodf = np.cos(sett["omr"] * t), np.sin(sett["omr"] * t)
+
+    xs, ssbs, phirs, amods = [], [], [], []
+    for k, d in enumerate(dets):
+        lat, lon, gam = SITES[d]
+        phir = (phir0 + lon) % (2 * math.pi)   # local sidereal angle at t=0
+        x = rng.standard_normal(N) * sigmas[k]
+        if gap_frac > 0:
+            # zeroed gaps at seeded random positions covering gap_frac of the samples
+            ngap = max(1, int(rng.integers(3, 9)))
+            total = int(gap_frac * N)
+            lens = rng.multinomial(total, np.ones(ngap) / ngap)
+            starts = np.sort(rng.choice(N - total, size=ngap, replace=False))
+            off = 0
+            for s0, ln in zip(starts, lens):
+                x[s0 + off:s0 + off + ln] = 0.0
+                off += ln
+        xs.append(x)
+        ssbs.append(ephemeris(N, dt, d, orbit_phase, phir))
+        phirs.append(phir)
+        amods.append(rogcvir(lat, gam))
+
+    inj = {}
+    rng_s = {}
+    if inject:
+        f = rng.uniform(0.1 * math.pi, 0.9 * math.pi)
+        fdot = -1e-8 * rng.uniform(0.0, 1.0)
+        Mmat = M.reshape(4, 4).T
+        oms = sett["oms"]
+        for _ in range(1000):
+            # uniform sky, redrawn until the whole search patch lies inside the grid disk
+            sindel = rng.uniform(-1.0, 1.0)
+            delta = math.asin(sindel)
+            alpha = rng.uniform(0.0, 2 * math.pi)
+            _, be = ast2lin(alpha, delta, C_EPSMA)
+            g = np.linalg.solve(Mmat, np.array([f, fdot, be[0] * (oms + f), be[1] * (oms + f)]))
+            nn0, mm0 = int(round(g[2])), int(round(g[3]))
+            ok = True
+            for dn in (-(nsky[0] // 2), nsky[0] - nsky[0] // 2 - 1):
+                for dm in (-(nsky[1] // 2), nsky[1] - nsky[1] // 2 - 1):
+                    a1 = (nn0 + dn) * M[10] + (mm0 + dm) * M[14]
+                    a2 = (nn0 + dn) * M[11] + (mm0 + dm) * M[15]
+                    ok &= (a1 * a1 + a2 * a2) / (oms * oms) <= 0.95
+            if ok:
+                break
+        ph_o, psik = rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi)
+        hoc = rng.uniform(-1.0, 1.0)
+        hop = (1.0 + hoc * hoc) / 2.0
+        amp = np.array([
+            math.cos(2 * psik) * hop * math.cos(ph_o) - math.sin(2 * psik) * hoc * math.sin(ph_o),
+            math.sin(2 * psik) * hop * math.cos(ph_o) + math.cos(2 * psik) * hoc * math.sin(ph_o),
+            -math.cos(2 * psik) * hop * math.sin(ph_o) - math.sin(2 * psik) * hoc * math.cos(ph_o),
+            -math.sin(2 * psik) * hop * math.sin(ph_o) + math.cos(2 * psik) * hoc * math.cos(ph_o)])
+        cof = sett["oms"] + f
+        sa, ca, sd, cd = math.sin(alpha), math.cos(alpha), math.sin(delta), math.cos(delta)
+        nS = np.array([ca * cd, sa * cd, sd])
+        for k in range(len(dets)):
+            aa, bb = modvir(sa, ca, sd, cd, math.sin(phirs[k]), math.cos(phirs[k]), amods[k],
+                            cosmodf, sinmodf)
+            shift = ssbs[k].reshape(-1, 3) @ nS
+            phase = f * t + fdot * t * t + (cof + 2.0 * fdot * t) * shift
+            sig = (amp[0] * aa * np.cos(phase) + amp[2] * aa * np.sin(phase)
+                   + amp[1] * bb * np.cos(phase) + amp[3] * bb * np.sin(phase))
+            nzm = xs[k] != 0.0
+            xs[k][nzm] += h0 * sig[nzm]
+        pm, be = ast2lin(alpha, delta, C_EPSMA)
+        g = np.linalg.solve(Mmat, np.array([f, fdot, be[0] * cof, be[1] * cof]))
+        inj = dict(f=f, fdot=fdot, delta=delta, alpha=alpha, amp=amp, h0=h0, pm=pm,
+                   grid=np.round(g).astype(int))
+        ss0, nn0, mm0 = inj["grid"][1], inj["grid"][2], inj["grid"][3]
+    else:
+        pm, ss0, nn0, mm0 = 1, 0, 0, 0
+    rng_s = dict(spndr=[int(ss0 - nspin // 2), int(ss0 - nspin // 2 + nspin - 1)],
+                 nr=[int(nn0 - nsky[0] // 2), int(nn0 - nsky[0] // 2 + nsky[0] - 1)],
+                 mr=[int(mm0 - nsky[1] // 2), int(mm0 - nsky[1] // 2 + nsky[1] - 1)],
+                 pmr=[int(pm), int(pm)])
+    sig2 = [_sig2(x) for x in xs]
+    return Segment(sett=sett, M=M, dets=list(dets), xDat=xs, DetSSB=ssbs, phir=phirs,
+                   epsm=C_EPSMA, amod=amods, sig2=sig2, injection=inj, range=rng_s)
+
+
+def sky_list(seg: Segment, max_points: int | None = None):
+    """(pm, mm, nn) in the reference's loop order (jobcore.c:90-98) over seg.range."""
+    out = []
+    r = seg.range
+    for pm in range(r["pmr"][0], r["pmr"][1] + 1):
+        for mm in range(r["mr"][0], r["mr"][1] + 1):
+            for nn in range(r["nr"][0], r["nr"][1] + 1):
+                out.append((pm, mm, nn))
+    return out if max_points is None else out[:max_points]
