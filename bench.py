@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Throughput of the coherent F-statistic search core on one MI355X.
+
+Workload (the network configuration): two detectors (H1-, L1-like; white noise sigma 1 and 1.3,
+one injection h0 = 0.1), N = 86400 samples at dt = 2 s, nfft = 2^17, fftpad = 2 (F-statistic FFT
+length L = 2^18), 256 sky points x 512 spindowns = 131072 templates, threshold F = 20.
+
+A step is one pgw_job_batch call over `--sky-per-step` sky points (all their spindowns); the
+default 16 steps x 16 sky points cover the 256 sky points once.  Inputs are resident in HBM
+before the timed region.  The headline value is the FP32 form (FP64 phases and spline
+coefficients, FP32 series and FFTs); the FP64 path is timed the same way beside it, and the FP32
+candidates and F values are compared with the FP64 ones.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+import pgwload  # noqa: E402
+
+pgwload.load()
+from polgraw_allsky_amd import abi, pgw, synth  # noqa: E402
+
+METRIC = "F-stat templates/s on 1 MI355X (FP32-batched vs FP64 plain); max rel ΔF vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(args):
+    seg = synth.make_segment(N=86400, dets=("H1", "L1"), sigmas=(1.0, 1.3), fftpad=2, seed=args.seed, h0=0.1,
+                             nsky=(16, 16), nspin=512)
+    return seg
+
+
+def run_path(h, sky, rng, opts, steps, warmup, per_step):
+    """Runs warmup + timed steps; returns (templates, seconds, gpu_ms, records)."""
+    nsteps_total = warmup + steps
+    chunks = [sky[(i * per_step) % len(sky):(i * per_step) % len(sky) + per_step] for i in range(nsteps_total)]
+    for c in chunks[:warmup]:
+        h.job_batch(c, rng, opts)
+    recs, fnum, gpu_ms = [], 0, 0.0
+    t0 = time.perf_counter()
+    for c in chunks[warmup:]:
+        r, f = h.job_batch(c, rng, opts)
+        recs.append(r)
+        fnum += f
+        gpu_ms += h.last_gpu_ms
+    dt = time.perf_counter() - t0
+    return fnum, dt, gpu_ms, (np.concatenate(recs) if recs else np.zeros((0, 5)))
+
+
+def cand_compare(r32, r64):
+    """Candidate-set difference (keys = frequency, spindown, sky fields) and F deviation."""
+    k32 = {tuple(r[:4]): r[4] for r in r32}
+    k64 = {tuple(r[:4]): r[4] for r in r64}
+    common = set(k32) & set(k64)
+    F = lambda snr: snr * snr / 2 + 2  # noqa: E731
+    rel = max((abs(F(k32[k]) - F(k64[k])) / F(k64[k]) for k in common), default=0.0)
+    return dict(n_fp64=len(k64), n_fp32=len(k32), only_fp64=len(set(k64) - set(k32)),
+                only_fp32=len(set(k32) - set(k64)), max_rel_dF_candidates=rel)
+
+
+def fstat_compare(h32, h64, seg, opts, ntempl=4):
+    """max |dF| and max relative dF on bins with F >= 1 over a few full F arrays."""
+    pm, mm, nn = seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2])
+    ss0 = int(seg.injection["grid"][1])
+    mabs, mrel = 0.0, 0.0
+    for ss in range(ss0 - ntempl // 2, ss0 - ntempl // 2 + ntempl):
+        a = h32.fstat(pm, mm, nn, ss, opts)
+        b = h64.fstat(pm, mm, nn, ss, opts)
+        d = np.abs(a - b)
+        mabs = max(mabs, float(d.max()))
+        sel = b >= 1.0
+        mrel = max(mrel, float((d[sel] / b[sel]).max()))
+    return mabs, mrel
+
+
+def roofline(h, seg, opts, prec_bytes, nt=64, reps=10):
+    s = seg.sett
+    pm, mm, nn = seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2])
+    ms = h.time_stages(pm, mm, nn, seg.range["spndr"][0], nt, opts, reps=reps)
+    nifo, N, L, p = len(seg.dets), s["N"], s["nfftf"], prec_bytes
+    M = s["nmax"] - s["nmin"]
+    # algorithmic bytes per template and kernel (both stated in DESIGN.md)
+    b_col = nifo * N * (4 * p + 8) + 4 * L * p     # read xDatma/b + shft, write two L-point spectra
+    b_row = 4 * L * p + M * p                      # read the two spectra, write F once
+    k = "col" if ms["col"] >= ms["row"] else "row"
+    b = (b_col if k == "col" else b_row) * nt
+    achieved = b / (ms[k] * 1e-3) / 1e9
+    stage_ms = ms["col"] + ms["row"] + ms["thr"]
+    b_t = (b_col + b_row) * nt
+    return dict(bound="hbm", kernel="k_spin_col" if k == "col" else "k_spin_row", achieved=round(achieved, 1),
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                templates_per_launch=nt, ms_per_launch=ms, bytes_per_template=dict(col=b_col, row=b_row),
+                template_stage_GBps=round(b_t / (stage_ms * 1e-3) / 1e9, 1))
+
+
+def cpu_baseline(seg, rng, opts):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import orc
+    threads = orc.default_threads()
+    o = orc.Search(seg, nthreads=threads)
+    pm, mm, nn = seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2])
+    t0 = time.perf_counter()
+    _, fn = o.job_core(pm, mm, nn, rng, opts)
+    dt = time.perf_counter() - t0
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(value=round(fn / dt, 2), unit="templates/s", cores=threads, kind="port",
+                sample=f"1 sky point x {fn} spindowns of the same workload (FP64 oracle, own Stockham FFT, "
+                       f"OpenMP over spindowns; {dt:.1f} s wall)", cpu_model=model)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--sky-per-step", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sky points shard across GPUs with no collectives; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    seg = workload(args)
+    rng = abi.make_range(seg)
+    opts = abi.make_opts(trl=20.0)
+    sky = synth.sky_list(seg)
+    print(f"workload: N={seg.sett['N']} nfft={seg.sett['nfft']} fftpad={seg.sett['fftpad']} "
+          f"L={seg.sett['nfftf']} sky={len(sky)} spindowns=512", file=sys.stderr, flush=True)
+
+    h32 = pgw.Handle(seg, abi.PREC_FP32)
+    n32, t32, g32, r32 = run_path(h32, sky, rng, opts, args.steps, args.warmup, args.sky_per_step)
+    v32 = n32 / t32
+    print(f"fp32: {n32} templates in {t32:.3f} s -> {v32:.0f} templates/s (gpu {g32:.1f} ms)",
+          file=sys.stderr, flush=True)
+    out_fp64 = None
+    acc = None
+    if not args.no_fp64:
+        h64 = pgw.Handle(seg, abi.PREC_FP64)
+        n64, t64, g64, r64 = run_path(h64, sky, rng, opts, args.steps, args.warmup, args.sky_per_step)
+        v64 = n64 / t64
+        print(f"fp64: {n64} templates in {t64:.3f} s -> {v64:.0f} templates/s (gpu {g64:.1f} ms)",
+              file=sys.stderr, flush=True)
+        out_fp64 = dict(value=round(v64, 1), ms_per_step=round(1e3 * t64 / args.steps, 3),
+                        gpu_ms_per_step=round(g64 / args.steps, 3), templates=n64,
+                        roofline=roofline(h64, seg, opts, 8))
+        acc = cand_compare(r32, r64)
+        mabs, mrel = fstat_compare(h32, h64, seg, opts)
+        acc.update(max_abs_dF=mabs, max_rel_dF_F_ge_1=mrel)
+        h64.close()
+    rl = roofline(h32, seg, opts, 4)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(seg, rng, opts)
+        print(f"cpu: {cpu}", file=sys.stderr, flush=True)
+
+    s = seg.sett
+    line = {
+        "metric": METRIC, "value": round(v32, 1), "unit": "templates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * t32 / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "network: 2 detectors (H1, L1), N=86400, nfft=2^17, fftpad=2, L=2^18, "
+                               "256 sky x 512 spindowns, trl=20",
+                   "N": s["N"], "nfft": s["nfft"], "fftpad": s["fftpad"], "L": s["nfftf"],
+                   "sky_points": len(sky), "spindowns": 512, "sky_per_step": args.sky_per_step},
+        "templates": n32, "gpu_ms_per_step": round(g32 / args.steps, 3),
+        "fp64": out_fp64,
+        "speedup_fp32_vs_fp64": (round(v32 / out_fp64["value"], 3) if out_fp64 else None),
+        "accuracy_fp32_vs_fp64": acc,
+        "roofline": rl,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -20,6 +20,8 @@
 #ifndef PGW_H
 #define PGW_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -134,6 +136,19 @@ int pgw_debug_fstat(pgw_handle *h, int pm, int mm, int nn, int ss, const pgw_opt
  * for sky point (pm, mm, nn) (jobcore.c:367-393); each array holds N complex values
  * as interleaved (re, im) doubles. */
 int pgw_debug_resampled(pgw_handle *h, int pm, int mm, int nn, int det, double *xDatma, double *xDatmb);
+
+/* Profiling tap (no counterpart in the reference): runs the sky-point stage of (pm, mm, nn)
+ * once, then `reps` times the template stage for spindowns [s0, s0+nt) (nt <= 64), with HIP
+ * events around each kernel on the handle's stream.  Average milliseconds per launch:
+ * ms[0] sky-point stage, ms[1] column pass (demodulation + column FFT), ms[2] row pass
+ * (row FFT + F + FStat partials), ms[3] threshold. */
+int pgw_time_stages(pgw_handle *h, int pm, int mm, int nn, int s0, int nt, const pgw_opts *opts,
+                    int reps, double ms[4]);
+
+/* Calibration helper (no counterpart in the reference): a plain device copy kernel of `bytes`
+ * bytes (read + write), run `reps` times; *ms = average milliseconds per launch.  Used to scale
+ * hardware byte counters. */
+int pgw_copy_bench(size_t bytes, int reps, double *ms);
 
 /* Elapsed device time (ms) of the last pgw_job_core / pgw_job_batch call, from HIP events. */
 double pgw_last_gpu_ms(const pgw_handle *h);

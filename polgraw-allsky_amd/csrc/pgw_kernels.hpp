@@ -545,4 +545,9 @@ __global__ __launch_bounds__(NT) void k_threshold(ThrParams p, const T* __restri
   }
 }
 
+// plain copy (grid-stride), used to calibrate hardware byte counters
+__global__ __launch_bounds__(NT) void k_copy(const double4* __restrict__ a, double4* __restrict__ b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) b[i] = a[i];
+}
+
 }  // namespace pgw

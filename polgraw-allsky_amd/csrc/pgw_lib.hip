@@ -198,6 +198,7 @@ struct pgw_handle {
                         int cap, int* FNum) = 0;
   virtual int debug_fstat(int pm, int mm, int nn, int ss, const pgw_opts* o, double* F) = 0;
   virtual int debug_resampled(int pm, int mm, int nn, int det, double* a, double* b) = 0;
+  virtual int time_stages(int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps, double* ms) = 0;
   double last_ms = 0.0;
 };
 
@@ -393,19 +394,23 @@ template <typename T> struct Impl final : pgw_handle {
     if (o->s0_flag) smin = smax;
   }
 
-  // launches the template stage for spindowns [s0, s0+nt)
-  int launch_templates(int mm, int nn, int s0, int nt, const pgw_opts* o) {
+  // launches the template stage for spindowns [s0, s0+nt); ev (optional) = 4 events bracketing the kernels
+  int launch_templates(int mm, int nn, int s0, int nt, const pgw_opts* o, hipEvent_t* ev = nullptr) {
     const double* M = sett.M;
     SpinParams sp;
     sp.M0 = M[0]; sp.M4 = M[4]; sp.M5 = M[5]; sp.M9 = M[9]; sp.M13 = M[13];
     sp.nn = nn; sp.mm = mm; sp.ss0 = s0; sp.s0_flag = o->s0_flag;
     sp.N = sett.N; sp.L1 = ps.L1; sp.nifo = nifo; sp.twshift = ps.twshift;
+    if (ev) HIPCHK(hipEventRecord(ev[0], stream));
     launch_spin_col<T>(ps, sp, xm.p, shft.p, Z.p, nt, hi.p, lo.p, stream);
+    if (ev) HIPCHK(hipEventRecord(ev[1], stream));
     RowParams rp{ps.L2, sett.nmin, sett.nmax, nblk, nparts};
     launch_spin_row<T>(ps, rp, Z.p, aabb.p, Ft.p, psum.p, pmax.p, nt, stream);
+    if (ev) HIPCHK(hipEventRecord(ev[2], stream));
     ThrParams tp{ps.L1, ps.L2, sett.nmin, sett.nmax, nblk, nparts, o->white_flag, s0, kCandCap, o->trl};
     hipLaunchKernelGGL((k_threshold<T>), dim3(nt), dim3(NT), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p, cand.p,
                        ncand.p);
+    if (ev) HIPCHK(hipEventRecord(ev[3], stream));
     return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
   }
 
@@ -512,6 +517,38 @@ template <typename T> struct Impl final : pgw_handle {
       Fout[k - sett.nmin] = (double)hF[(size_t)(k % ps.L2) * ps.L1 + k / ps.L2] / hmu[part];
     }
     return PGW_OK;
+  }
+
+  int time_stages(int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps, double* ms) override {
+    if (nt < 1 || nt > TB || reps < 1) return PGW_ERR_ARG;
+    SkyState sk;
+    int st;
+    hipEvent_t ev[4];
+    for (int k = 0; k < 4; k++) HIPCHK(hipEventCreate(&ev[k]));
+    HIPCHK(hipEventRecord(ev0, stream));
+    const int inside = sky_stage(pm, mm, nn, sk, st);
+    HIPCHK(hipEventRecord(ev1, stream));
+    if (!inside) st = 1;
+    double acc[4] = {0, 0, 0, 0};
+    if (!st) {
+      HIPCHK(hipStreamSynchronize(stream));
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, ev0, ev1));
+      acc[0] = t;
+      for (int r = 0; r < reps && !st; r++) {
+        HIPCHK(hipMemsetAsync(ncand.p, 0, sizeof(int), stream));
+        st = launch_templates(mm, nn, s0, nt, o, ev);
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int k = 0; k < 3; k++) {
+          HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+          acc[k + 1] += t;
+        }
+      }
+    }
+    for (int k = 0; k < 4; k++) (void)hipEventDestroy(ev[k]);
+    ms[0] = acc[0];
+    for (int k = 1; k < 4; k++) ms[k] = acc[k] / reps;
+    return st;
   }
 
   int debug_resampled(int pm, int mm, int nn, int det, double* a, double* b) override {
@@ -643,6 +680,38 @@ int pgw_debug_fstat(pgw_handle* h, int pm, int mm, int nn, int ss, const pgw_opt
 int pgw_debug_resampled(pgw_handle* h, int pm, int mm, int nn, int det, double* xDatma, double* xDatmb) {
   if (!h || !xDatma || !xDatmb) return PGW_ERR_ARG;
   return h->debug_resampled(pm, mm, nn, det, xDatma, xDatmb);
+}
+
+int pgw_time_stages(pgw_handle* h, int pm, int mm, int nn, int s0, int nt, const pgw_opts* opts, int reps,
+                    double ms[4]) {
+  if (!h || !opts || !ms) return PGW_ERR_ARG;
+  return h->time_stages(pm, mm, nn, s0, nt, opts, reps, ms);
+}
+
+int pgw_copy_bench(size_t bytes, int reps, double* ms) {
+  if (!ms || bytes < 16 || reps < 1) return PGW_ERR_ARG;
+  const size_t n = bytes / sizeof(double4);
+  double4 *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  HIPCHK(hipMalloc(&a, n * sizeof(double4)));
+  if (hipMalloc(&b, n * sizeof(double4)) != hipSuccess) { (void)hipFree(a); return PGW_ERR_NOMEM; }
+  HIPCHK(hipMemset(a, 0, n * sizeof(double4)));
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int grid = (int)std::min<size_t>((n + NT - 1) / NT, 1 << 20);
+  hipLaunchKernelGGL(k_copy, dim3(grid), dim3(NT), 0, 0, a, b, n);  // warm-up
+  HIPCHK(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(NT), 0, 0, a, b, n);
+  HIPCHK(hipEventRecord(e1, 0));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return PGW_OK;
 }
 
 double pgw_last_gpu_ms(const pgw_handle* h) { return h ? h->last_ms : 0.0; }

@@ -42,6 +42,9 @@ def lib():
                                     _ip, _dp, C.c_int, _ip]
         L.pgw_debug_fstat.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts), _dp]
         L.pgw_debug_resampled.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.pgw_time_stages.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(abi.Opts), C.c_int, _dp]
+        L.pgw_copy_bench.argtypes = [C.c_size_t, C.c_int, _dp]
         L.pgw_last_gpu_ms.restype = C.c_double
         L.pgw_last_gpu_ms.argtypes = [C.c_void_p]
         L.pgw_destroy.argtypes = [C.c_void_p]
@@ -52,7 +55,17 @@ def lib():
 
 
 EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_job_core", "pgw_job_batch",
-            "pgw_debug_fstat", "pgw_debug_resampled", "pgw_last_gpu_ms", "pgw_destroy", "pgw_strerror"]
+            "pgw_debug_fstat", "pgw_debug_resampled", "pgw_time_stages", "pgw_copy_bench", "pgw_last_gpu_ms",
+            "pgw_destroy", "pgw_strerror"]
+
+
+def copy_bench(nbytes: int, reps: int = 20) -> float:
+    """Average ms of a plain device copy of nbytes (read + write)."""
+    ms = C.c_double()
+    st = lib().pgw_copy_bench(nbytes, reps, C.byref(ms))
+    if st != 0:
+        raise PGWError(st, "pgw_copy_bench")
+    return ms.value
 
 
 class Handle:
@@ -115,6 +128,13 @@ class Handle:
         if st != 0:
             return None
         return a.view(np.complex128), b.view(np.complex128)
+
+    def time_stages(self, pm, mm, nn, s0, nt, opts, reps=5):
+        ms = np.zeros(4)
+        st = lib().pgw_time_stages(self.h, pm, mm, nn, s0, nt, C.byref(opts), reps, ms.ctypes.data_as(_dp))
+        if st != 0:
+            raise PGWError(st, "pgw_time_stages")
+        return dict(sky=ms[0], col=ms[1], row=ms[2], thr=ms[3])
 
     @property
     def last_gpu_ms(self) -> float:
