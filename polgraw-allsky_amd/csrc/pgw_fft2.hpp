@@ -1,0 +1,204 @@
+// pgw_fft2.hpp - register-resident two-pass FFT building blocks (gfx950, 256-thread blocks).
+//
+// Same index convention as pgw_common.hpp: L = L1*L2, n = n1 + L1 n2, k = L2 k1 + k2.
+//
+// Column pass (P2-point transforms over n2, C consecutive columns n1 per block):
+//   P2 = E * G.  Thread t owns column c = t % C and group g = t / C (C = 256/G) and holds
+//   x[g + G r], r < E, in registers.  Step 1: E-point DFT in registers, times W_P2^{g kr}.
+//   One LDS exchange ([g][kr][c], c fastest: conflict-free).  Step 2: G-point DFTs over g,
+//   giving X[kr + E kg] in registers.
+//
+// Row pass (P1 = 16*16*R3 points, NT = P1/16 threads): radix-16, radix-16, radix-R3 Stockham
+//   stages with the data in registers and three LDS exchanges padded by one element per 16;
+//   thread t ends with X[t + NT m], m < 16 (natural order).
+//
+// Stage twiddles come from one table entry per thread and stage, raised to the needed powers by
+// a product tree of depth <= 4 (error a few ulp); arrays are processed together so they share
+// every barrier.
+#pragma once
+#include "pgw_common.hpp"
+
+namespace pgw {
+
+// w^0..w^(R-1) by a product tree (depth <= 4 for R <= 16)
+template <int R, typename T2> __device__ __forceinline__ void tw_powers(T2 w, T2* p) {
+  p[0] = mk<T2>(1, 0);
+  if constexpr (R > 1) p[1] = w;
+#pragma unroll
+  for (int r = 2; r < R; r++) {
+    const int h = 1 << (31 - __builtin_clz(r));  // highest power of two <= r
+    p[r] = (h == r) ? cmul(p[r / 2], p[r / 2]) : cmul(p[h], p[r - h]);
+  }
+}
+
+// v[r] *= w^r for r < R.  FP32: product-tree powers (few ulp).  FP64: running product (one
+// complex register live instead of R; error grows ~R ulp, 2e-15 at R = 16).
+template <int R, int DIR, typename T2> __device__ __forceinline__ void twiddle_run(T2* v, T2 w) {
+  w = tw_dir<DIR>(w);
+  if constexpr (sizeof(decltype(T2::x)) == 8) {
+    T2 pr = w;
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+      v[r] = cmul(v[r], pr);
+      if (r + 1 < R) pr = cmul(pr, w);
+    }
+  } else {
+    T2 p[R];
+    tw_powers<R>(w, p);
+#pragma unroll
+    for (int r = 1; r < R; r++) v[r] = cmul(v[r], p[r]);
+  }
+}
+
+template <int R, int DIR, typename T2> __device__ __forceinline__ void apply_tw(T2* v, T2 w) {
+  T2 p[R];
+  tw_powers<R>(tw_dir<DIR>(w), p);
+#pragma unroll
+  for (int r = 1; r < R; r++) v[r] = cmul(v[r], p[r]);
+}
+
+template <int R, int DIR, typename T2> __device__ __forceinline__ void apply_tw_pow(T2* v, const T2* p) {
+#pragma unroll
+  for (int r = 1; r < R; r++) v[r] = cmul(v[r], p[r]);
+}
+
+// ----------------------------------------------------------- column pass --
+template <int P2, int E> struct ColShape {
+  static constexpr int G = P2 / E;     // groups per column
+  static constexpr int C = 256 / G;    // columns per block
+  static constexpr int Q = E / G;      // kr values per thread in step 2
+  static_assert(G >= 1 && G <= E && E % G == 0, "column split");
+};
+
+// NA arrays x[a][r] (registers) -> X in registers: out[a][q*G + kg] = X[kr + E kg], kr = grp*Q + q.
+// lds must hold NA * P2 * C elements.  twc[m] = W_P2^m (forward sign).  One barrier inside; the
+// caller must barrier before reusing lds.
+template <typename T2, int P2, int E, int NA, int DIR>
+__device__ __forceinline__ void col_fft(T2 (&x)[NA][E], T2* lds, const T2* __restrict__ twc) {
+  using S = ColShape<P2, E>;
+  const int c = threadIdx.x % S::C, g = threadIdx.x / S::C;
+  const T2 wg = twc[g];
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    dftR<E, DIR>(x[a]);
+    twiddle_run<E, DIR>(x[a], wg);  // W_P2^{g kr}
+#pragma unroll
+    for (int kr = 0; kr < E; kr++) lds[(size_t)a * P2 * S::C + (g * E + kr) * S::C + c] = x[a][kr];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+#pragma unroll
+    for (int q = 0; q < S::Q; q++) {
+      const int kr = g * S::Q + q;
+      T2 v[S::G];
+#pragma unroll
+      for (int gg = 0; gg < S::G; gg++) v[gg] = lds[(size_t)a * P2 * S::C + (gg * E + kr) * S::C + c];
+      if constexpr (S::G > 1) dftR<S::G, DIR>(v);
+#pragma unroll
+      for (int kg = 0; kg < S::G; kg++) x[a][q * S::G + kg] = v[kg];
+    }
+  }
+}
+
+// Inter-pass twiddles W_L^{n1 k2} for the thread's E output slots from one table entry u = W_L^{n1}:
+// k2 = kr + E kg with kr = grp*Q + q, so W = u^{grp Q} * u^q * (u^E)^{kg}  (forward sign).
+template <int P2, int E, typename T2>
+__device__ __forceinline__ void col_out_twiddles(T2 u, T2* w) {
+  using S = ColShape<P2, E>;
+  const int g = threadIdx.x / S::C;
+  T2 pu[E + 1];  // u^0..u^E
+  tw_powers<E>(u, pu);
+  pu[E] = cmul(pu[E / 2], pu[E / 2]);
+  // u^{g Q}: binary powering over the small exponent (g < G, Q <= E)
+  T2 base = mk<T2>(1, 0), sq = u;
+  for (int e = g * S::Q; e; e >>= 1) {
+    if (e & 1) base = cmul(base, sq);
+    sq = cmul(sq, sq);
+  }
+  T2 pe[S::G];  // (u^E)^kg
+  tw_powers<S::G>(pu[E], pe);
+#pragma unroll
+  for (int q = 0; q < S::Q; q++) {
+    const T2 bq = cmul(base, pu[q]);
+#pragma unroll
+    for (int kg = 0; kg < S::G; kg++) w[q * S::G + kg] = cmul(bq, pe[kg]);
+  }
+}
+
+// output index k2 of register slot s (= q*G + kg) for the thread
+template <int P2, int E> __device__ __forceinline__ int col_out_k2(int s) {
+  using S = ColShape<P2, E>;
+  const int g = threadIdx.x / S::C;
+  const int q = s / S::G, kg = s % S::G;
+  return (g * S::Q + q) + E * kg;
+}
+
+// -------------------------------------------------------------- row pass --
+__device__ __forceinline__ int rpad(int i) { return i + (i >> 4); }
+
+template <int P1> struct RowShape {
+  static constexpr int NT = P1 / 16;
+  static constexpr int R3 = P1 / 256;
+  static constexpr int PADN = P1 + P1 / 16;  // padded elements per array
+  static_assert(R3 == 4 || R3 == 8 || R3 == 16, "row length 1024, 2048 or 4096");
+};
+
+// x[a][m] = row element (t + NT m) on entry, X[t + NT m] on exit.  lds holds NA * PADN elements.
+// twr[m] = W_P1^m (forward sign).  Barriers inside; ends after a barrier-free read (caller must
+// barrier before reusing lds).
+template <typename T2, int P1, int NA, int DIR>
+__device__ __forceinline__ void row_fft(T2 (&x)[NA][16], T2* lds, const T2* __restrict__ twr) {
+  using S = RowShape<P1>;
+  constexpr int NT = S::NT, R3 = S::R3, PADN = S::PADN;
+  const int t = threadIdx.x;
+  // stage twiddle bases loaded up front so their latency overlaps stage A
+  const T2 wB = twr[(t & 15) * (P1 / 256)];
+  T2 wC[16 / R3];
+#pragma unroll
+  for (int i = 0; i < 16 / R3; i++) wC[i] = twr[t + NT * i];
+  // stage A: radix 16, Ns = 1; outputs at 16 t + r
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    dftR<16, DIR>(x[a]);
+#pragma unroll
+    for (int r = 0; r < 16; r++) lds[a * PADN + rpad(16 * t + r)] = x[a][r];
+  }
+  __syncthreads();
+  // stage B: radix 16, Ns = 16; butterfly j = t reads j + NT r
+  {
+    const int k = t & 15;
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+#pragma unroll
+      for (int r = 0; r < 16; r++) x[a][r] = lds[a * PADN + rpad(t + NT * r)];
+      twiddle_run<16, DIR>(x[a], wB);  // W_256^{k r}
+      dftR<16, DIR>(x[a]);
+    }
+    __syncthreads();
+    const int base = (t - k) * 16 + k;
+#pragma unroll
+    for (int a = 0; a < NA; a++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) lds[a * PADN + rpad(base + 16 * r)] = x[a][r];
+    __syncthreads();
+  }
+  // stage C: radix R3, Ns = 256; butterflies b = t + NT i (i < 16/R3) read b + 256 r, k = b
+#pragma unroll
+  for (int i = 0; i < 16 / R3; i++) {
+    const int b = t + NT * i;
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      T2 v[R3];
+#pragma unroll
+      for (int r = 0; r < R3; r++) v[r] = lds[a * PADN + rpad(b + 256 * r)];
+      twiddle_run<R3, DIR>(v, wC[i]);  // W_P1^{b r}
+      dftR<R3, DIR>(v);
+      // outputs b + 256 r = t + NT (i + (16/R3) r)
+#pragma unroll
+      for (int r = 0; r < R3; r++) x[a][i + (16 / R3) * r] = v[r];
+    }
+  }
+}
+
+}  // namespace pgw

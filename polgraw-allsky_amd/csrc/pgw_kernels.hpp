@@ -333,14 +333,14 @@ template <> struct PhaseRot<double> {
   }
 };
 template <> struct PhaseRot<float> {
-  // FP64 phase reduced modulo 2 pi in FP64, then FP32 sin/cos of the reduced angle
+  // FP64 phase reduced modulo 2 pi in FP64 (two-term Cody-Waite), then the hardware sin/cos of
+  // the reduced angle in revolutions (v_sin_f32 / v_cos_f32 take x and return sin/cos(2 pi x))
   __device__ static __forceinline__ float2 eval(double ph) {
     const double k = rint(ph * 0.15915494309189533577);
     const double r = fma(-k, 6.283185307179586232, ph);
     const double rr = fma(-k, 2.4492935982947064e-16, r);
-    float s, c;
-    sincosf((float)rr, &s, &c);
-    return make_float2(c, -s);
+    const float rev = (float)(rr * 0.15915494309189533577);
+    return make_float2(__builtin_amdgcn_cosf(rev), -__builtin_amdgcn_sinf(rev));
   }
 };
 

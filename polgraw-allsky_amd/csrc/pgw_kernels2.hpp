@@ -1,0 +1,197 @@
+// pgw_kernels2.hpp - template-stage and resampling FFT kernels on the register-resident FFT
+// blocks of pgw_fft2.hpp.
+#pragma once
+#include "pgw_fft2.hpp"
+#include "pgw_kernels.hpp"
+
+namespace pgw {
+
+// column-pass register depth per precision and column length (LDS per array = 256*E elements)
+template <typename T, int P2> struct ColE;
+template <int P2> struct ColE<float, P2> { static constexpr int E = 16; static constexpr bool SEQ = false; };
+template <> struct ColE<double, 16> { static constexpr int E = 8; static constexpr bool SEQ = false; };
+template <> struct ColE<double, 32> { static constexpr int E = 8; static constexpr bool SEQ = false; };
+template <> struct ColE<double, 64> { static constexpr int E = 8; static constexpr bool SEQ = false; };
+template <> struct ColE<double, 128> { static constexpr int E = 16; static constexpr bool SEQ = true; };
+template <> struct ColE<double, 256> { static constexpr int E = 16; static constexpr bool SEQ = true; };
+
+// Fused spindown demodulation + column pass of the F-statistic FFT (jobcore.c:456-630).
+// grid (L1/C, ntemplates), 256 threads.  SEQ: the two spectra share one LDS buffer in turn.  For template t (ss = ss0 + t) and detector d:
+//   x_{a,b}[n] += xm[d][{a,b}][n] exp(-i phi_d(n)),  phi_d(n) = het1 n + sgnlt1 (n^2 + 2 n shft_d[n]),
+// zero for n >= N; the column FFT and the W_L^{n1 k2} twiddle follow; Z[t][{a,b}][k2][n1].
+template <typename T, int P2, bool SEQ>
+__global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, const cplx<T>* __restrict__ xm,
+                                                   const double* __restrict__ shft, cplx<T>* __restrict__ Z,
+                                                   const cplx<T>* __restrict__ twc, const cplx<T>* __restrict__ hi,
+                                                   const cplx<T>* __restrict__ lo) {
+  using T2 = cplx<T>;
+  constexpr int E = ColE<T, P2>::E;
+  using S = ColShape<P2, E>;
+  __shared__ T2 lds[(SEQ ? 1 : 2) * P2 * S::C];
+  const int t = blockIdx.y;
+  const int c = threadIdx.x % S::C, g = threadIdx.x / S::C;
+  const int n1 = blockIdx.x * S::C + c;
+  const int L1 = p.L1, N = p.N;
+  const size_t L = (size_t)L1 * P2;
+  double het1, sgnlt1;
+  spin_params(p, p.ss0 + t, het1, sgnlt1);
+  T2 x[2][E];
+#pragma unroll
+  for (int r = 0; r < E; r++) {
+    const int n = n1 + L1 * (g + S::G * r);
+    T2 xa = mk<T2>(0, 0), xb = mk<T2>(0, 0);
+    if (n < N) {
+      const double dn = (double)n;
+      for (int d = 0; d < p.nifo; d++) {
+        const double sh = shft[(size_t)d * N + n];
+        // _tmp1 = i^2 + 2 i shft, phase = het1 i + sgnlt1 _tmp1 (jobcore.c:294,530), uncontracted
+        const double tmp1 = lin2_nc(dn, dn, (double)(2 * n), sh);
+        const T2 ex = PhaseRot<T>::eval(lin2_nc(het1, dn, sgnlt1, tmp1));
+        xa = cadd(xa, cmul(xm[(size_t)(2 * d) * N + n], ex));
+        xb = cadd(xb, cmul(xm[(size_t)(2 * d + 1) * N + n], ex));
+      }
+    }
+    x[0][r] = xa;
+    x[1][r] = xb;
+  }
+  T2* Za = Z + (size_t)t * 2 * L;
+  T2 w[E];
+  col_out_twiddles<P2, E>(tw_big(hi, lo, (uint32_t)n1 << p.twshift), w);
+  if constexpr (!SEQ) {
+    col_fft<T2, P2, E, 2, -1>(x, lds, twc);
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+      const int k2 = col_out_k2<P2, E>(s);
+      Za[(size_t)k2 * L1 + n1] = cmul(x[0][s], w[s]);
+      Za[L + (size_t)k2 * L1 + n1] = cmul(x[1][s], w[s]);
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      T2 (&xa)[1][E] = *reinterpret_cast<T2(*)[1][E]>(&x[a]);
+      if (a) __syncthreads();
+      col_fft<T2, P2, E, 1, -1>(xa, lds, twc);
+#pragma unroll
+      for (int s = 0; s < E; s++) {
+        const int k2 = col_out_k2<P2, E>(s);
+        Za[a * L + (size_t)k2 * L1 + n1] = cmul(xa[0][s], w[s]);
+      }
+    }
+  }
+}
+
+// Row pass + F-statistic + FStat partials (jobcore.c:629-651, auxi.c:214-240): grid (L2, ntemplates),
+// P1/16 threads.  F(k), k = L2 k1 + k2 in [nmin, nmax), at Ft[t][k2][k1]; psum/pmax[t][part][k2].
+// NA = 2 transforms both spectra together; NA = 1 one after the other through the same LDS.
+template <typename T, int P1, int NA>
+__global__ __launch_bounds__(P1 / 16) void k_spin_row2(RowParams p, const cplx<T>* __restrict__ Z,
+                                                      const double* __restrict__ aabb,
+                                                      const cplx<T>* __restrict__ twr, T* __restrict__ Ft,
+                                                      T* __restrict__ psum, T* __restrict__ pmax) {
+  using T2 = cplx<T>;
+  using S = RowShape<P1>;
+  constexpr int NT = S::NT;
+  __shared__ T2 lds[NA * S::PADN];
+  const int k2 = blockIdx.x, t = blockIdx.y, tid = threadIdx.x;
+  const size_t L = (size_t)P1 * p.L2;
+  const T2* rowa = Z + (size_t)t * 2 * L + (size_t)k2 * P1;
+  const T2* rowb = rowa + L;
+  const T aa = (T)aabb[0], bb = (T)aabb[1];
+  T F[16];
+  if constexpr (NA == 2) {
+    T2 x[2][16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) { x[0][m] = rowa[tid + NT * m]; x[1][m] = rowb[tid + NT * m]; }
+    row_fft<T2, P1, 2, -1>(x, lds, twr);
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) / aa + (x[1][m].x * x[1][m].x + x[1][m].y * x[1][m].y) / bb;
+  } else {
+    T2 x[1][16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) x[0][m] = rowa[tid + NT * m];
+    row_fft<T2, P1, 1, -1>(x, lds, twr);
+#pragma unroll
+    for (int m = 0; m < 16; m++) F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) / aa;
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; m++) x[0][m] = rowb[tid + NT * m];
+    row_fft<T2, P1, 1, -1>(x, lds, twr);
+#pragma unroll
+    for (int m = 0; m < 16; m++) F[m] = F[m] + (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) / bb;
+  }
+  __syncthreads();
+  T* Fs = reinterpret_cast<T*>(lds);
+  T* Frow = Ft + (size_t)t * L + (size_t)k2 * P1;
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k1 = tid + NT * m;
+    const int k = p.L2 * k1 + k2;
+    const bool ok = (k >= p.nmin && k < p.nmax);
+    if (ok) Frow[k1] = F[m];
+    Fs[k1] = ok ? F[m] : (T)0;
+  }
+  __syncthreads();
+  const int w = tid >> 6, lane = tid & 63;
+  for (int part = w; part < p.nparts; part += NT / 64) {
+    const int lok = p.nmin + PGW_NAV_DEV * part;
+    const int hik = (part < p.nblk) ? lok + PGW_NAV_DEV : p.nmax;
+    const int jlo = max(0, (lok - k2 + p.L2 - 1) / p.L2);
+    const int jhi = min((hik - k2 + p.L2 - 1) / p.L2, P1);
+    T sum = 0, mx = 0;
+    for (int j = jlo + lane; j < jhi; j += 64) {
+      const T f = Fs[j];
+      sum += f;
+      mx = f > mx ? f : mx;
+    }
+    sum = wave_sum(sum);
+    mx = wave_max(mx);
+    if (lane == 0) {
+      psum[((size_t)t * p.nparts + part) * p.L2 + k2] = sum;
+      pmax[((size_t)t * p.nparts + part) * p.L2 + k2] = mx;
+    }
+  }
+}
+
+// Generic column pass (one array per block): grid (L1/C, nbatch).
+template <typename T, int P2, int DIR, typename Loader>
+__global__ __launch_bounds__(256) void k_fft_col2(Loader ld, cplx<T>* __restrict__ Z, int L1,
+                                                  const cplx<T>* __restrict__ twc, const cplx<T>* __restrict__ hi,
+                                                  const cplx<T>* __restrict__ lo, int twshift) {
+  using T2 = cplx<T>;
+  constexpr int E = ColE<T, P2>::E;
+  using S = ColShape<P2, E>;
+  __shared__ T2 lds[P2 * S::C];
+  const int b = blockIdx.y;
+  const int c = threadIdx.x % S::C, g = threadIdx.x / S::C;
+  const int n1 = blockIdx.x * S::C + c;
+  const size_t L = (size_t)L1 * P2;
+  T2 x[1][E];
+#pragma unroll
+  for (int r = 0; r < E; r++) x[0][r] = ld(b, n1 + L1 * (g + S::G * r));
+  T2 w[E];
+  col_out_twiddles<P2, E>(tw_dir<DIR>(tw_big(hi, lo, (uint32_t)n1 << twshift)), w);
+  col_fft<T2, P2, E, 1, DIR>(x, lds, twc);
+#pragma unroll
+  for (int s = 0; s < E; s++) {
+    const int k2 = col_out_k2<P2, E>(s);
+    Z[(size_t)b * L + (size_t)k2 * L1 + n1] = cmul(x[0][s], w[s]);
+  }
+}
+
+// Generic row pass, in place: grid (L2, nbatch), P1/16 threads.
+template <typename T, int P1, int DIR>
+__global__ __launch_bounds__(P1 / 16) void k_fft_row2(cplx<T>* __restrict__ Z, int L2, const cplx<T>* __restrict__ twr) {
+  using T2 = cplx<T>;
+  using S = RowShape<P1>;
+  __shared__ T2 lds[S::PADN];
+  T2* row = Z + (size_t)blockIdx.y * P1 * L2 + (size_t)blockIdx.x * P1;
+  T2 x[1][16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) x[0][m] = row[threadIdx.x + S::NT * m];
+  row_fft<T2, P1, 1, DIR>(x, lds, twr);
+#pragma unroll
+  for (int m = 0; m < 16; m++) row[threadIdx.x + S::NT * m] = x[0][m];
+}
+
+}  // namespace pgw

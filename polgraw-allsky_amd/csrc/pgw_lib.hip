@@ -15,7 +15,7 @@
 #include <vector>
 
 #include "../../include/pgw.h"
-#include "pgw_kernels.hpp"
+#include "pgw_kernels2.hpp"
 
 using namespace pgw;
 
@@ -38,11 +38,9 @@ int ilog2(int v) {
   return l;
 }
 
-// L = L1 * L2 split (row length L1 <= 4096, column length L2 >= 16)
+// L = L1 * L2 split (row length L1 in {1024, 2048, 4096}, column length L2 in 16..256)
 bool split_len(int L, int& L1, int& L2) {
   switch (L) {
-    case 1 << 12: L1 = 256; L2 = 16; return true;
-    case 1 << 13: L1 = 512; L2 = 16; return true;
     case 1 << 14: L1 = 1024; L2 = 16; return true;
     case 1 << 15: L1 = 1024; L2 = 32; return true;
     case 1 << 16: L1 = 2048; L2 = 32; return true;
@@ -54,10 +52,7 @@ bool split_len(int L, int& L1, int& L2) {
   }
 }
 
-// column tile: 32 KB of LDS per array
-template <typename T> constexpr int col_tile(int P2) {
-  return std::min(64, (32768 / (int)sizeof(cplx<T>)) / P2);
-}
+template <typename T, int P2> constexpr int col_tile() { return ColShape<P2, ColE<T, P2>::E>::C; }
 
 template <typename T> struct DevBuf {
   T* p = nullptr;
@@ -109,12 +104,10 @@ template <typename T, int DIR, typename Loader>
 void launch_col(const Plan<T>& pl, Loader ld, cplx<T>* Z, int nbatch, const cplx<T>* hi, const cplx<T>* lo,
                 hipStream_t st) {
 #define PGW_COL_CASE(P2)                                                                       \
-  case P2: {                                                                                   \
-    constexpr int C = col_tile<T>(P2);                                                         \
-    dim3 g(pl.L1 / C, nbatch);                                                                 \
-    hipLaunchKernelGGL((k_fft_col<T, P2, C, DIR, Loader>), g, dim3(NT), 0, st, ld, Z, pl.L1,   \
-                       pl.twc.p, hi, lo, pl.twshift);                                          \
-  } break;
+  case P2:                                                                                     \
+    hipLaunchKernelGGL((k_fft_col2<T, P2, DIR, Loader>), dim3(pl.L1 / col_tile<T, P2>(), nbatch), dim3(256), 0, \
+                       st, ld, Z, pl.L1, pl.twc.p, hi, lo, pl.twshift);                        \
+    break;
   switch (pl.L2) {
     PGW_COL_CASE(16)
     PGW_COL_CASE(32)
@@ -127,13 +120,11 @@ void launch_col(const Plan<T>& pl, Loader ld, cplx<T>* Z, int nbatch, const cplx
 
 template <typename T, int DIR>
 void launch_row(const Plan<T>& pl, cplx<T>* Z, int nbatch, hipStream_t st) {
-#define PGW_ROW_CASE(P1)                                                                           \
-  case P1:                                                                                         \
-    hipLaunchKernelGGL((k_fft_row<T, P1, DIR>), dim3(pl.L2, nbatch), dim3(NT), 0, st, Z, pl.L2, pl.twr.p); \
+#define PGW_ROW_CASE(P1)                                                                                    \
+  case P1:                                                                                                  \
+    hipLaunchKernelGGL((k_fft_row2<T, P1, DIR>), dim3(pl.L2, nbatch), dim3(P1 / 16), 0, st, Z, pl.L2, pl.twr.p); \
     break;
   switch (pl.L1) {
-    PGW_ROW_CASE(256)
-    PGW_ROW_CASE(512)
     PGW_ROW_CASE(1024)
     PGW_ROW_CASE(2048)
     PGW_ROW_CASE(4096)
@@ -143,13 +134,16 @@ void launch_row(const Plan<T>& pl, cplx<T>* Z, int nbatch, hipStream_t st) {
 
 template <typename T>
 void launch_spin_col(const Plan<T>& pl, const SpinParams& sp, const cplx<T>* xm, const double* shft, cplx<T>* Z,
-                     int nt, const cplx<T>* hi, const cplx<T>* lo, hipStream_t st) {
-#define PGW_SCOL_CASE(P2)                                                                     \
-  case P2: {                                                                                  \
-    constexpr int C = col_tile<T>(P2);                                                        \
-    hipLaunchKernelGGL((k_spin_col<T, P2, C>), dim3(pl.L1 / C, nt), dim3(NT), 0, st, sp, xm, shft, Z, \
-                       pl.twc.p, hi, lo);                                                     \
-  } break;
+                     int nt, const cplx<T>* hi, const cplx<T>* lo, bool seq, hipStream_t st) {
+#define PGW_SCOL_CASE(P2)                                                                                \
+  case P2:                                                                                               \
+    if (seq || ColE<T, P2>::SEQ)                                                                         \
+      hipLaunchKernelGGL((k_spin_col2<T, P2, true>), dim3(pl.L1 / col_tile<T, P2>(), nt), dim3(256), 0, st, sp, xm, \
+                         shft, Z, pl.twc.p, hi, lo);                                                     \
+    else                                                                                                 \
+      hipLaunchKernelGGL((k_spin_col2<T, P2, false>), dim3(pl.L1 / col_tile<T, P2>(), nt), dim3(256), 0, st, sp, xm, \
+                         shft, Z, pl.twc.p, hi, lo);                                                     \
+    break;
   switch (pl.L2) {
     PGW_SCOL_CASE(16)
     PGW_SCOL_CASE(32)
@@ -162,14 +156,17 @@ void launch_spin_col(const Plan<T>& pl, const SpinParams& sp, const cplx<T>* xm,
 
 template <typename T>
 void launch_spin_row(const Plan<T>& pl, const RowParams& rp, const cplx<T>* Z, const double* aabb, T* Ft, T* psum,
-                     T* pmax, int nt, hipStream_t st) {
+                     T* pmax, int nt, int na, hipStream_t st) {
 #define PGW_SROW_CASE(P1)                                                                                 \
   case P1:                                                                                                \
-    hipLaunchKernelGGL((k_spin_row<T, P1>), dim3(pl.L2, nt), dim3(NT), 0, st, rp, Z, aabb, pl.twr.p, Ft, psum, pmax); \
+    if (na == 2)                                                                                          \
+      hipLaunchKernelGGL((k_spin_row2<T, P1, 2>), dim3(pl.L2, nt), dim3(P1 / 16), 0, st, rp, Z, aabb, pl.twr.p, Ft, \
+                         psum, pmax);                                                                     \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_spin_row2<T, P1, 1>), dim3(pl.L2, nt), dim3(P1 / 16), 0, st, rp, Z, aabb, pl.twr.p, Ft, \
+                         psum, pmax);                                                                     \
     break;
   switch (pl.L1) {
-    PGW_SROW_CASE(256)
-    PGW_SROW_CASE(512)
     PGW_SROW_CASE(1024)
     PGW_SROW_CASE(2048)
     PGW_SROW_CASE(4096)
@@ -225,6 +222,8 @@ template <typename T> struct Impl final : pgw_handle {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int TB = 64;          // templates per batch
+  int row_na = 1;        // row pass: 2 = both spectra through one LDS exchange, 1 = one after the other
+  bool col_seq = false;  // column pass: spectra share one LDS buffer in turn
   int nblk = 0, nparts = 0, npro = 0;
   int nspl = 0;         // spline knots interpftpad*N
   std::vector<Cand> hcand;
@@ -257,6 +256,13 @@ template <typename T> struct Impl final : pgw_handle {
     nparts = nblk + (M % PGW_NAV ? 1 : 0);
     if (nparts > 256) return PGW_ERR_UNSUPPORTED;
     nspl = sett.interpftpad * N;
+    // kernel variants; PGW_TUNE="row_na=2,col_seq=0,tb=32" overrides (used when measuring variants)
+    if (const char* tune = getenv("PGW_TUNE")) {
+      int v;
+      if (const char* q = strstr(tune, "row_na=")) if (sscanf(q, "row_na=%d", &v) == 1 && (v == 1 || v == 2)) row_na = v;
+      if (const char* q = strstr(tune, "col_seq=")) if (sscanf(q, "col_seq=%d", &v) == 1) col_seq = v != 0;
+      if (const char* q = strstr(tune, "tb=")) if (sscanf(q, "tb=%d", &v) == 1 && v >= 1 && v <= 256) TB = v;
+    }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
@@ -402,10 +408,10 @@ template <typename T> struct Impl final : pgw_handle {
     sp.nn = nn; sp.mm = mm; sp.ss0 = s0; sp.s0_flag = o->s0_flag;
     sp.N = sett.N; sp.L1 = ps.L1; sp.nifo = nifo; sp.twshift = ps.twshift;
     if (ev) HIPCHK(hipEventRecord(ev[0], stream));
-    launch_spin_col<T>(ps, sp, xm.p, shft.p, Z.p, nt, hi.p, lo.p, stream);
+    launch_spin_col<T>(ps, sp, xm.p, shft.p, Z.p, nt, hi.p, lo.p, col_seq, stream);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));
     RowParams rp{ps.L2, sett.nmin, sett.nmax, nblk, nparts};
-    launch_spin_row<T>(ps, rp, Z.p, aabb.p, Ft.p, psum.p, pmax.p, nt, stream);
+    launch_spin_row<T>(ps, rp, Z.p, aabb.p, Ft.p, psum.p, pmax.p, nt, row_na, stream);
     if (ev) HIPCHK(hipEventRecord(ev[2], stream));
     ThrParams tp{ps.L1, ps.L2, sett.nmin, sett.nmax, nblk, nparts, o->white_flag, s0, kCandCap, o->trl};
     hipLaunchKernelGGL((k_threshold<T>), dim3(nt), dim3(NT), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p, cand.p,
@@ -520,7 +526,7 @@ template <typename T> struct Impl final : pgw_handle {
   }
 
   int time_stages(int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps, double* ms) override {
-    if (nt < 1 || nt > TB || reps < 1) return PGW_ERR_ARG;
+    if (nt < 1 || nt > TB || reps < 1) return PGW_ERR_ARG;  // nt <= templates per batch
     SkyState sk;
     int st;
     hipEvent_t ev[4];
