@@ -88,6 +88,7 @@ def roofline(h, seg, opts, prec_bytes, nt=64, reps=10):
     s = seg.sett
     pm, mm, nn = seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2])
     ms = h.time_stages(pm, mm, nn, seg.range["spndr"][0], nt, opts, reps=reps)
+    nt = ms.pop("nt")
     nifo, N, L, p = len(seg.dets), s["N"], s["nfftf"], prec_bytes
     M = s["nmax"] - s["nmin"]
     # algorithmic bytes per template and kernel (both stated in DESIGN.md)

@@ -138,12 +138,13 @@ int pgw_debug_fstat(pgw_handle *h, int pm, int mm, int nn, int ss, const pgw_opt
 int pgw_debug_resampled(pgw_handle *h, int pm, int mm, int nn, int det, double *xDatma, double *xDatmb);
 
 /* Profiling tap (no counterpart in the reference): runs the sky-point stage of (pm, mm, nn)
- * once, then `reps` times the template stage for spindowns [s0, s0+nt) (nt <= 64), with HIP
- * events around each kernel on the handle's stream.  Average milliseconds per launch:
- * ms[0] sky-point stage, ms[1] column pass (demodulation + column FFT), ms[2] row pass
- * (row FFT + F + FStat partials), ms[3] threshold. */
+ * once, then `reps` times the template stage for spindowns [s0, s0+nt) (nt is capped at the
+ * handle's templates per launch), with HIP events around each kernel on the handle's stream.
+ * Average milliseconds per launch: ms[0] sky-point stage, ms[1] column pass (demodulation +
+ * column FFT), ms[2] row pass (row FFT + F + FStat partials), ms[3] threshold; ms[4] = the
+ * number of templates per launch used. */
 int pgw_time_stages(pgw_handle *h, int pm, int mm, int nn, int s0, int nt, const pgw_opts *opts,
-                    int reps, double ms[4]);
+                    int reps, double ms[5]);
 
 /* Calibration helper (no counterpart in the reference): a plain device copy kernel of `bytes`
  * bytes (read + write), run `reps` times; *ms = average milliseconds per launch.  Used to scale

@@ -18,8 +18,8 @@ namespace pgw {
 
 constexpr int NT = 256;          // threads per block everywhere
 constexpr int PGW_NAV_DEV = 4096; // FStat block length (NAVFSTAT, auxi.h:13)
-constexpr int SPL_K = 64;        // spline chunk per thread
-constexpr int SPL_W = 64;        // spline warm-up (|coefficient| ~ 0.268: 0.268^64 ~ 1e-37)
+constexpr int SPL_K = 16;        // spline chunk per thread
+constexpr int SPL_W = 32;        // spline warm-up (|coefficient| ~ 0.268: 0.268^32 ~ 5e-19)
 constexpr int SPL_NTAB = 64;     // spline recursion coefficients tabulated below this index
 
 struct SkyDet {
@@ -483,64 +483,73 @@ struct ThrParams {
   double trl;
 };
 
-// FStat normalisation + run-max threshold (auxi.c:214-240, jobcore.c:659-695): one block per template.
-// Only parts whose max/mu exceeds trl are scanned; a run is emitted by the thread owning its first bin.
+// FStat normalisation + run-max threshold (auxi.c:214-240, jobcore.c:659-695).
+// grid (nparts, ntemplates), one 64-lane wave per (part, template).  The wave reduces its part's
+// L2 row partials in a fixed order (mean mu = sum / (2*4096) for full blocks, 1 for the raw tail
+// or --whitenoise) and exits unless max(F)/mu > trl.  A flagged part computes every part's mu and
+// scans its own bins; each maximal run of F/mu > trl is emitted by the lane owning its first bin
+// (it may continue into later parts), at the last index of the run maximum.
+__device__ __forceinline__ double part_mean(const double sum, int part, const ThrParams& p) {
+  return (part < p.nblk && !p.white) ? sum / (2. * PGW_NAV_DEV) : 1.0;
+}
+
 template <typename T>
-__global__ __launch_bounds__(NT) void k_threshold(ThrParams p, const T* __restrict__ Ft, const T* __restrict__ psum,
+__global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restrict__ Ft, const T* __restrict__ psum,
                                                   const T* __restrict__ pmax, double* __restrict__ mu_out,
                                                   Cand* __restrict__ cand, int* __restrict__ ncand) {
   constexpr int MAXP = 256;
   __shared__ double mu[MAXP];
-  __shared__ int flag[MAXP];
-  const int t = blockIdx.x;
+  const int part = blockIdx.x, t = blockIdx.y, lane = threadIdx.x;
   const size_t L = (size_t)p.L1 * p.L2;
   const T* F = Ft + (size_t)t * L;
-  for (int part = threadIdx.x; part < p.nparts; part += NT) {
-    double sum = 0.0, mx = 0.0;
-    const T* ps = psum + ((size_t)t * p.nparts + part) * p.L2;
-    const T* pm = pmax + ((size_t)t * p.nparts + part) * p.L2;
-    for (int k2 = 0; k2 < p.L2; k2++) {
-      sum += (double)ps[k2];
-      mx = fmax(mx, (double)pm[k2]);
-    }
-    double m = 1.0;
-    if (part < p.nblk && !p.white) m = sum / (2. * PGW_NAV_DEV);
-    mu[part] = m;
-    flag[part] = (mx / m > p.trl);
-    mu_out[(size_t)t * p.nparts + part] = m;
+  const T* ps0 = psum + (size_t)t * p.nparts * p.L2;
+  const T* pm0 = pmax + (size_t)t * p.nparts * p.L2;
+  double sum = 0.0, mx = 0.0;
+  for (int k2 = lane; k2 < p.L2; k2 += 64) {
+    sum += (double)ps0[(size_t)part * p.L2 + k2];
+    mx = fmax(mx, (double)pm0[(size_t)part * p.L2 + k2]);
+  }
+  sum = wave_sum(sum);
+  mx = wave_max(mx);
+  const double m = part_mean(sum, part, p);
+  if (lane == 0) mu_out[(size_t)t * p.nparts + part] = m;
+  if (!(mx / m > p.trl)) return;  // uniform across the wave
+  // flagged: means of all parts (same fixed-order reduction as above)
+  for (int q = 0; q < p.nparts; q++) {
+    double sq = 0.0;
+    for (int k2 = lane; k2 < p.L2; k2 += 64) sq += (double)ps0[(size_t)q * p.L2 + k2];
+    sq = wave_sum(sq);
+    if (lane == 0) mu[q] = part_mean(sq, q, p);
   }
   __syncthreads();
-  for (int part = 0; part < p.nparts; part++) {
-    if (!flag[part]) continue;  // uniform: flag in LDS
-    const int lok = p.nmin + PGW_NAV_DEV * part;
-    const int hik = (part < p.nblk) ? lok + PGW_NAV_DEV : p.nmax;
-    for (int k = lok + threadIdx.x; k < hik; k += NT) {
-      const double fn = (double)F[(size_t)(k % p.L2) * p.L1 + k / p.L2] / mu[part];
-      if (!(fn > p.trl)) continue;
-      bool start = (k == p.nmin);
-      if (!start) {
-        const int kp = k - 1;
-        const int pp = min((kp - p.nmin) / PGW_NAV_DEV, p.nparts - 1);
-        const double fp = (double)F[(size_t)(kp % p.L2) * p.L1 + kp / p.L2] / mu[pp];
-        start = !(fp > p.trl);
-      }
-      if (!start) continue;
-      int ii = k;
-      double Fc = fn;
-      for (int kk = k + 1; kk < p.nmax; kk++) {
-        const int pk = min((kk - p.nmin) / PGW_NAV_DEV, p.nparts - 1);
-        const double v = (double)F[(size_t)(kk % p.L2) * p.L1 + kk / p.L2] / mu[pk];
-        if (!(v > p.trl)) break;
-        if (v >= Fc) { ii = kk; Fc = v; }
-      }
-      const int idx = atomicAdd(ncand, 1);
-      if (idx < p.cap) {
-        Cand c;
-        c.ss = p.ss0 + t;
-        c.ii = ii;
-        c.F = Fc;
-        cand[idx] = c;
-      }
+  const int lok = p.nmin + PGW_NAV_DEV * part;
+  const int hik = (part < p.nblk) ? lok + PGW_NAV_DEV : p.nmax;
+  for (int k = lok + lane; k < hik; k += 64) {
+    const double fn = (double)F[(size_t)(k % p.L2) * p.L1 + k / p.L2] / mu[part];
+    if (!(fn > p.trl)) continue;
+    bool start = (k == p.nmin);
+    if (!start) {
+      const int kp = k - 1;
+      const int pp = min((kp - p.nmin) / PGW_NAV_DEV, p.nparts - 1);
+      const double fp = (double)F[(size_t)(kp % p.L2) * p.L1 + kp / p.L2] / mu[pp];
+      start = !(fp > p.trl);
+    }
+    if (!start) continue;
+    int ii = k;
+    double Fc = fn;
+    for (int kk = k + 1; kk < p.nmax; kk++) {
+      const int pk = min((kk - p.nmin) / PGW_NAV_DEV, p.nparts - 1);
+      const double v = (double)F[(size_t)(kk % p.L2) * p.L1 + kk / p.L2] / mu[pk];
+      if (!(v > p.trl)) break;
+      if (v >= Fc) { ii = kk; Fc = v; }
+    }
+    const int idx = atomicAdd(ncand, 1);
+    if (idx < p.cap) {
+      Cand c;
+      c.ss = p.ss0 + t;
+      c.ii = ii;
+      c.F = Fc;
+      cand[idx] = c;
     }
   }
 }

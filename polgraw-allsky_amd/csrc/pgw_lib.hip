@@ -30,7 +30,8 @@ using namespace pgw;
 
 namespace {
 
-constexpr int kCandCap = 1 << 20;
+constexpr int kCandCap = 1 << 20;   // candidates per sky point (device buffer)
+constexpr int kCandChunk = 4096;    // candidates copied back with the count, without a second sync
 
 int ilog2(int v) {
   int l = 0;
@@ -221,7 +222,10 @@ template <typename T> struct Impl final : pgw_handle {
   DevBuf<int> ncand;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int TB = 64;          // templates per batch
+  hipEvent_t evd[2] = {nullptr, nullptr};  // candidate copy of buffer 0/1 done
+  Cand* hc_pin = nullptr;                  // pinned mirrors: 2 x kCandChunk candidates, 2 counts
+  int* hn_pin = nullptr;
+  int TB = 128;         // templates per launch
   int row_na = 1;        // row pass: 2 = both spectra through one LDS exchange, 1 = one after the other
   bool col_seq = false;  // column pass: spectra share one LDS buffer in turn
   int nblk = 0, nparts = 0, npro = 0;
@@ -238,6 +242,10 @@ template <typename T> struct Impl final : pgw_handle {
     cand.release(); ncand.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (auto& e : evd)
+      if (e) (void)hipEventDestroy(e);
+    if (hc_pin) (void)hipHostFree(hc_pin);
+    if (hn_pin) (void)hipHostFree(hn_pin);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -266,6 +274,11 @@ template <typename T> struct Impl final : pgw_handle {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
+    HIPCHK(hipEventCreateWithFlags(&evd[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&evd[1], hipEventDisableTiming));
+    if (hipHostMalloc((void**)&hc_pin, sizeof(Cand) * 2 * kCandChunk, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&hn_pin, sizeof(int) * 2, hipHostMallocDefault) != hipSuccess)
+      return PGW_ERR_NOMEM;
     int st;
     if ((st = pf.init(sett.nfft)) || (st = pi.init(sett.Ninterp)) || (st = ps.init(sett.nfftf))) return st;
     // W_{2^20}^m two-level tables: hi[h] = W^{1024 h}, lo[l] = W^l
@@ -297,8 +310,8 @@ template <typename T> struct Impl final : pgw_handle {
         (st = u.alloc((size_t)2 * n * (nspl + 1))) || (st = y2.alloc((size_t)2 * n * (nspl + 1))) ||
         (st = xm.alloc((size_t)2 * n * N)) || (st = Z.alloc((size_t)TB * 2 * L)) || (st = Ft.alloc((size_t)TB * L)) ||
         (st = psum.alloc((size_t)TB * nparts * ps.L2)) || (st = pmax.alloc((size_t)TB * nparts * ps.L2)) ||
-        (st = mu.alloc((size_t)TB * nparts)) || (st = spl.alloc(1)) || (st = cand.alloc(kCandCap)) ||
-        (st = ncand.alloc(1)))
+        (st = mu.alloc((size_t)TB * nparts)) || (st = spl.alloc(1)) || (st = cand.alloc(2 * (size_t)kCandCap)) ||
+        (st = ncand.alloc(2)))
       return st;
     HIPCHK(hipMemcpy(xDat.p, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(DetSSB.p, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice));
@@ -400,8 +413,9 @@ template <typename T> struct Impl final : pgw_handle {
     if (o->s0_flag) smin = smax;
   }
 
-  // launches the template stage for spindowns [s0, s0+nt); ev (optional) = 4 events bracketing the kernels
-  int launch_templates(int mm, int nn, int s0, int nt, const pgw_opts* o, hipEvent_t* ev = nullptr) {
+  // launches the template stage for spindowns [s0, s0+nt) into candidate buffer buf;
+  // ev (optional) = 4 events bracketing the kernels
+  int launch_templates(int mm, int nn, int s0, int nt, const pgw_opts* o, int buf = 0, hipEvent_t* ev = nullptr) {
     const double* M = sett.M;
     SpinParams sp;
     sp.M0 = M[0]; sp.M4 = M[4]; sp.M5 = M[5]; sp.M9 = M[9]; sp.M13 = M[13];
@@ -414,8 +428,8 @@ template <typename T> struct Impl final : pgw_handle {
     launch_spin_row<T>(ps, rp, Z.p, aabb.p, Ft.p, psum.p, pmax.p, nt, row_na, stream);
     if (ev) HIPCHK(hipEventRecord(ev[2], stream));
     ThrParams tp{ps.L1, ps.L2, sett.nmin, sett.nmax, nblk, nparts, o->white_flag, s0, kCandCap, o->trl};
-    hipLaunchKernelGGL((k_threshold<T>), dim3(nt), dim3(NT), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p, cand.p,
-                       ncand.p);
+    hipLaunchKernelGGL((k_threshold<T>), dim3(nparts, nt), dim3(64), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p,
+                       cand.p + (size_t)buf * kCandCap, ncand.p + buf);
     if (ev) HIPCHK(hipEventRecord(ev[3], stream));
     return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
   }
@@ -456,53 +470,87 @@ template <typename T> struct Impl final : pgw_handle {
     return PGW_OK;
   }
 
-  int run_sky(int pm, int mm, int nn, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap, int* FNum,
-              float* ms) {
+  struct Pending {
     SkyState sk;
-    int st;
-    HIPCHK(hipEventRecord(ev0, stream));
-    if (!sky_stage(pm, mm, nn, sk, st)) return PGW_OK;
-    if (st) return st;
+    int mm, nn, buf;
+  };
+
+  // Queues all device work of one sky point into candidate buffer buf, followed by the async copy
+  // of the candidate count and the first kCandChunk candidates; returns 0 outside the disk.
+  int launch_sky(int pm, int mm, int nn, pgw_range* r, const pgw_opts* o, int buf, SkyState& sk, int* FNum,
+                 int& status) {
+    if (!sky_stage(pm, mm, nn, sk, status)) return 0;
+    if (status) return 1;
     int smin, smax;
     spindown_range(mm, nn, r, o, smin, smax);
-    HIPCHK(hipMemsetAsync(ncand.p, 0, sizeof(int), stream));
+    status = PGW_ERR_DEVICE;
+    if (hipMemsetAsync(ncand.p + buf, 0, sizeof(int), stream) != hipSuccess) return 1;
     for (int s0 = smin; s0 <= smax; s0 += TB) {
       const int nt = std::min(TB, smax - s0 + 1);
-      if ((st = launch_templates(mm, nn, s0, nt, o))) return st;
+      if ((status = launch_templates(mm, nn, s0, nt, o, buf))) return 1;
     }
-    HIPCHK(hipEventRecord(ev1, stream));
-    int nc = 0;
-    HIPCHK(hipMemcpyAsync(&nc, ncand.p, sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
+    status = PGW_ERR_DEVICE;
+    if (hipMemcpyAsync(hn_pin + buf, ncand.p + buf, sizeof(int), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipMemcpyAsync(hc_pin + (size_t)buf * kCandChunk, cand.p + (size_t)buf * kCandCap, sizeof(Cand) * kCandChunk,
+                       hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipEventRecord(evd[buf], stream) != hipSuccess)
+      return 1;
+    status = PGW_OK;
+    if (smax >= smin) *FNum += smax - smin + 1;
+    return 1;
+  }
+
+  // Waits for a queued sky point's candidates and appends its records.
+  int finish(const Pending& pd, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap) {
+    HIPCHK(hipEventSynchronize(evd[pd.buf]));
+    const int nc = hn_pin[pd.buf];
     if (nc > kCandCap) return PGW_ERR_CAPACITY;
-    if (nc > 0) {
-      HIPCHK(hipMemcpyAsync(hcand.data(), cand.p, sizeof(Cand) * nc, hipMemcpyDeviceToHost, stream));
+    memcpy(hcand.data(), hc_pin + (size_t)pd.buf * kCandChunk, sizeof(Cand) * std::min(nc, kCandChunk));
+    if (nc > kCandChunk) {  // rare: the rest synchronously (buffer pd.buf is not reused before this returns)
+      HIPCHK(hipMemcpyAsync(hcand.data() + kCandChunk, cand.p + (size_t)pd.buf * kCandCap + kCandChunk,
+                            sizeof(Cand) * (nc - kCandChunk), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
     }
+    return emit(pd.sk, pd.mm, pd.nn, o, nc, sgnlc, sgnlv, cap);
+  }
+
+  // Sky points in order; the device works on point i while the host turns point i-1 into records.
+  int run_batch(const int* sky, int nsky, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap,
+                int* FNum) {
+    HIPCHK(hipEventRecord(ev0, stream));
+    Pending prev{};
+    bool have_prev = false;
+    int buf = 0, st = PGW_OK;
+    for (int i = 0; i < nsky && !st; i++) {
+      Pending cur{};
+      cur.mm = sky[3 * i + 1];
+      cur.nn = sky[3 * i + 2];
+      cur.buf = buf;
+      if (!launch_sky(sky[3 * i], cur.mm, cur.nn, r, o, buf, cur.sk, FNum, st)) continue;
+      if (st) break;
+      if (have_prev) st = finish(prev, o, sgnlc, sgnlv, cap);
+      prev = cur;
+      have_prev = true;
+      buf ^= 1;
+    }
+    HIPCHK(hipEventRecord(ev1, stream));
+    if (have_prev && !st) st = finish(prev, o, sgnlc, sgnlv, cap);
+    HIPCHK(hipEventSynchronize(ev1));
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, ev0, ev1));
-    *ms += t;
-    if (smax >= smin) *FNum += smax - smin + 1;
-    return emit(sk, mm, nn, o, nc, sgnlc, sgnlv, cap);
+    last_ms = t;
+    return st;
   }
 
   int job_core(int pm, int mm, int nn, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap,
                int* FNum) override {
-    float ms = 0.f;
-    const int st = run_sky(pm, mm, nn, r, o, sgnlc, sgnlv, cap, FNum, &ms);
-    last_ms = ms;
-    return st;
+    const int sky[3] = {pm, mm, nn};
+    return run_batch(sky, 1, r, o, sgnlc, sgnlv, cap, FNum);
   }
 
   int job_batch(const int* sky, int nsky, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap,
                 int* FNum) override {
-    float ms = 0.f;
-    for (int i = 0; i < nsky; i++) {
-      const int st = run_sky(sky[3 * i], sky[3 * i + 1], sky[3 * i + 2], r, o, sgnlc, sgnlv, cap, FNum, &ms);
-      if (st) { last_ms = ms; return st; }
-    }
-    last_ms = ms;
-    return PGW_OK;
+    return run_batch(sky, nsky, r, o, sgnlc, sgnlv, cap, FNum);
   }
 
   int debug_fstat(int pm, int mm, int nn, int ss, const pgw_opts* o, double* Fout) override {
@@ -526,7 +574,8 @@ template <typename T> struct Impl final : pgw_handle {
   }
 
   int time_stages(int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps, double* ms) override {
-    if (nt < 1 || nt > TB || reps < 1) return PGW_ERR_ARG;  // nt <= templates per batch
+    if (nt < 1 || reps < 1) return PGW_ERR_ARG;
+    nt = std::min(nt, TB);  // at most one batch
     SkyState sk;
     int st;
     hipEvent_t ev[4];
@@ -543,7 +592,7 @@ template <typename T> struct Impl final : pgw_handle {
       acc[0] = t;
       for (int r = 0; r < reps && !st; r++) {
         HIPCHK(hipMemsetAsync(ncand.p, 0, sizeof(int), stream));
-        st = launch_templates(mm, nn, s0, nt, o, ev);
+        st = launch_templates(mm, nn, s0, nt, o, 0, ev);
         HIPCHK(hipStreamSynchronize(stream));
         for (int k = 0; k < 3; k++) {
           HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
@@ -554,6 +603,7 @@ template <typename T> struct Impl final : pgw_handle {
     for (int k = 0; k < 4; k++) (void)hipEventDestroy(ev[k]);
     ms[0] = acc[0];
     for (int k = 1; k < 4; k++) ms[k] = acc[k] / reps;
+    ms[4] = nt;
     return st;
   }
 
@@ -689,7 +739,7 @@ int pgw_debug_resampled(pgw_handle* h, int pm, int mm, int nn, int det, double* 
 }
 
 int pgw_time_stages(pgw_handle* h, int pm, int mm, int nn, int s0, int nt, const pgw_opts* opts, int reps,
-                    double ms[4]) {
+                    double ms[5]) {
   if (!h || !opts || !ms) return PGW_ERR_ARG;
   return h->time_stages(pm, mm, nn, s0, nt, opts, reps, ms);
 }

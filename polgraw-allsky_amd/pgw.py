@@ -130,11 +130,11 @@ class Handle:
         return a.view(np.complex128), b.view(np.complex128)
 
     def time_stages(self, pm, mm, nn, s0, nt, opts, reps=5):
-        ms = np.zeros(4)
+        ms = np.zeros(5)
         st = lib().pgw_time_stages(self.h, pm, mm, nn, s0, nt, C.byref(opts), reps, ms.ctypes.data_as(_dp))
         if st != 0:
             raise PGWError(st, "pgw_time_stages")
-        return dict(sky=ms[0], col=ms[1], row=ms[2], thr=ms[3])
+        return dict(sky=ms[0], col=ms[1], row=ms[2], thr=ms[3], nt=int(ms[4]))
 
     @property
     def last_gpu_ms(self) -> float:
