@@ -8,7 +8,7 @@ namespace pgw {
 
 // column-pass register depth per precision and column length (LDS per array = 256*E elements)
 template <typename T, int P2> struct ColE;
-template <int P2> struct ColE<float, P2> { static constexpr int E = 16; static constexpr bool SEQ = false; };
+template <int P2> struct ColE<float, P2> { static constexpr int E = (P2 <= 64 ? 8 : 16); static constexpr bool SEQ = false; };
 template <> struct ColE<double, 16> { static constexpr int E = 8; static constexpr bool SEQ = false; };
 template <> struct ColE<double, 32> { static constexpr int E = 8; static constexpr bool SEQ = false; };
 template <> struct ColE<double, 64> { static constexpr int E = 8; static constexpr bool SEQ = false; };
@@ -107,15 +107,17 @@ __global__ __launch_bounds__(P1 / 16) void k_spin_row2(RowParams p, const cplx<T
     for (int m = 0; m < 16; m++)
       F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) / aa + (x[1][m].x * x[1][m].x + x[1][m].y * x[1][m].y) / bb;
   } else {
-    T2 x[1][16];
+    T2 x[1][16], y[1][16];
 #pragma unroll
     for (int m = 0; m < 16; m++) x[0][m] = rowa[tid + NT * m];
+#pragma unroll
+    for (int m = 0; m < 16; m++) y[0][m] = rowb[tid + NT * m];  // in flight during the first FFT
     row_fft<T2, P1, 1, -1>(x, lds, twr);
 #pragma unroll
     for (int m = 0; m < 16; m++) F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) / aa;
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < 16; m++) x[0][m] = rowb[tid + NT * m];
+    for (int m = 0; m < 16; m++) x[0][m] = y[0][m];
     row_fft<T2, P1, 1, -1>(x, lds, twr);
 #pragma unroll
     for (int m = 0; m < 16; m++) F[m] = F[m] + (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) / bb;
