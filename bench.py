@@ -84,7 +84,19 @@ def fstat_compare(h32, h64, seg, opts, ntempl=4):
     return mabs, mrel
 
 
-def roofline(h, seg, opts, prec_bytes, nt=64, reps=10):
+def measured_traffic(kernel, prec):
+    """HBM bytes per template of a kernel from the committed rocprofv3 counter summary
+    (profiles/*.md lines 'bench-traffic: {json}'), or None."""
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            if line.startswith("bench-traffic:"):
+                d = json.loads(line.split(":", 1)[1])
+                if d.get("kernel") == kernel and d.get("precision") == prec:
+                    return d["bytes_per_template"]
+    return None
+
+
+def roofline(h, seg, opts, prec_bytes, nt=128, reps=10):
     s = seg.sett
     pm, mm, nn = seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2])
     ms = h.time_stages(pm, mm, nn, seg.range["spndr"][0], nt, opts, reps=reps)
@@ -95,12 +107,15 @@ def roofline(h, seg, opts, prec_bytes, nt=64, reps=10):
     b_col = nifo * N * (4 * p + 8) + 4 * L * p     # read xDatma/b + shft, write two L-point spectra
     b_row = 4 * L * p + M * p                      # read the two spectra, write F once
     k = "col" if ms["col"] >= ms["row"] else "row"
+    kname = "k_spin_col2" if k == "col" else "k_spin_row2"
     b = (b_col if k == "col" else b_row) * nt
     achieved = b / (ms[k] * 1e-3) / 1e9
     stage_ms = ms["col"] + ms["row"] + ms["thr"]
     b_t = (b_col + b_row) * nt
-    return dict(bound="hbm", kernel="k_spin_col" if k == "col" else "k_spin_row", achieved=round(achieved, 1),
-                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+    tr = measured_traffic(kname, "fp32" if prec_bytes == 4 else "fp64")
+    return dict(bound="hbm", kernel=kname, achieved=round(achieved, 1),
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
+                traffic=(int(tr * nt) if tr else None), traffic_unit="bytes per launch",
                 templates_per_launch=nt, ms_per_launch=ms, bytes_per_template=dict(col=b_col, row=b_row),
                 template_stage_GBps=round(b_t / (stage_ms * 1e-3) / 1e9, 1))
 
@@ -136,12 +151,17 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="first run a plain 256 MiB device copy (scale for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(sky points shard across GPUs with no collectives; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calib:
+        ms = pgw.copy_bench(256 << 20, reps=3)
+        print(f"copy 256 MiB: {ms:.3f} ms ({2 * 256 * 1.048576 / ms:.0f} GB/s)", file=sys.stderr, flush=True)
     seg = workload(args)
     rng = abi.make_range(seg)
     opts = abi.make_opts(trl=20.0)
