@@ -124,9 +124,32 @@ int pgw_job_core(pgw_handle *h, int pm, int mm, int nn, pgw_range *s_range, cons
 
 /* Many sky points in one call: sky[3*i..3*i+2] = (pm, mm, nn).  Equivalent to calling
  * pgw_job_core for each in order (the spotlight driver's loop,
- * search/spotlight/src/jobcore.c:79-122), with the per-sky-point host syncs batched. */
+ * search/spotlight/src/jobcore.c:79-122).  Up to 16 consecutive sky points form a group whose
+ * templates share kernel launches; the host builds the records of one group while the device
+ * works on the next. */
 int pgw_job_batch(pgw_handle *h, const int *sky, int nsky, pgw_range *s_range, const pgw_opts *opts,
                   int *sgnlc, double *sgnlv, int sgnlv_cap, int *FNum);
+
+/* Several segments (time frames) of one network in one handle: sett[nseg], ifo[nseg*nifo]
+ * (detector d of segment k at ifo[k*nifo + d]).  Segments may have different N; those with the
+ * same nfft and fftpad share kernel launches.  pgw_init is the nseg = 1 case.  No counterpart
+ * in the reference, which searches one frame per process (search/network/src-cpu/init.c:81-99). */
+int pgw_init_multi(pgw_handle **h, const pgw_settings *sett, const pgw_detector *ifo, int nseg, int nifo,
+                   int precision);
+
+/* Work items over segments: work[4*i..4*i+3] = (segment, pm, mm, nn), processed in order with
+ * pgw_job_core's semantics per item.  rec_work (optional, sgnlv_cap entries) receives the work
+ * index of each record. */
+int pgw_job_batch_multi(pgw_handle *h, const int *work, int nwork, pgw_range *s_range,
+                        const pgw_opts *opts, int *sgnlc, double *sgnlv, int sgnlv_cap, int *rec_work,
+                        int *FNum);
+
+/* Autotuner (no counterpart in the reference): times the template stage of sky point
+ * (pm, mm, nn) of segment `seg` for every kernel variant (templates per launch 64/128/256,
+ * row pass with both spectra together or in turn, column pass LDS sharing) and keeps the
+ * fastest for this handle.  The chosen values are returned (pointers may be NULL). */
+int pgw_autotune(pgw_handle *h, int seg, int pm, int mm, int nn, const pgw_opts *opts, int *tb,
+                 int *row_na, int *col_seq);
 
 /* Debug tap: the normalised F-statistic F[nmin..nmax) of one template (pm, mm, nn, ss),
  * as job_core holds it just before thresholding (jobcore.c:635-651).  F has nmax-nmin entries. */

@@ -82,20 +82,28 @@ def make_settings(seg, lines=None) -> Settings:
 
 def make_detectors(seg):
     """Detector array; keeps references to the numpy buffers alive on the returned object."""
-    n = len(seg.dets)
-    arr = (Detector * n)()
+    return make_detectors_multi([seg])
+
+
+def make_detectors_multi(segs):
+    """Detectors of several segments, segment-major (ifo[k*nifo + d])."""
+    nifo = len(segs[0].dets)
+    arr = (Detector * (nifo * len(segs)))()
     keep = []
-    for k in range(n):
-        x = np.ascontiguousarray(seg.xDat[k], dtype=np.float64)
-        e = np.ascontiguousarray(seg.DetSSB[k], dtype=np.float64)
-        keep += [x, e]
-        arr[k].xDat = dptr(x)
-        arr[k].DetSSB = dptr(e)
-        arr[k].sig2 = float(seg.sig2[k])
-        arr[k].sphir = float(np.sin(seg.phir[k]))
-        arr[k].cphir = float(np.cos(seg.phir[k]))
-        for i in range(9):
-            arr[k].amod[i] = float(seg.amod[k][i])
+    for s, seg in enumerate(segs):
+        assert len(seg.dets) == nifo
+        for k in range(nifo):
+            x = np.ascontiguousarray(seg.xDat[k], dtype=np.float64)
+            e = np.ascontiguousarray(seg.DetSSB[k], dtype=np.float64)
+            keep += [x, e]
+            a = arr[s * nifo + k]
+            a.xDat = dptr(x)
+            a.DetSSB = dptr(e)
+            a.sig2 = float(seg.sig2[k])
+            a.sphir = float(np.sin(seg.phir[k]))
+            a.cphir = float(np.cos(seg.phir[k]))
+            for i in range(9):
+                a.amod[i] = float(seg.amod[k][i])
     arr._keep = keep
     return arr
 

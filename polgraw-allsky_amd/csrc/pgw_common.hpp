@@ -22,6 +22,17 @@ template <typename T> using cplx = typename vec2<T>::type;
 template <typename T2> __device__ __forceinline__ T2 mk(decltype(T2::x) a, decltype(T2::x) b) {
   T2 r; r.x = a; r.y = b; return r;
 }
+// Loads through a pointer that is known to address device (global) memory.  Pointers read from
+// memory (template descriptors) are generic to the compiler, which then emits flat loads; these
+// helpers cast to the global address space so the loads are global_load and wait on vmcnt only.
+template <typename T> using gptr = const __attribute__((address_space(1))) T*;
+__device__ __forceinline__ double ld_global(const double* p, size_t i) { return ((gptr<double>)p)[i]; }
+template <typename T2> __device__ __forceinline__ T2 ld_global(const T2* p, size_t i) {
+  using V = decltype(T2::x) __attribute__((ext_vector_type(2)));
+  const V v = ((gptr<V>)p)[i];
+  return mk<T2>(v.x, v.y);
+}
+
 template <typename T2> __device__ __forceinline__ T2 cadd(T2 a, T2 b) { return mk<T2>(a.x + b.x, a.y + b.y); }
 template <typename T2> __device__ __forceinline__ T2 csub(T2 a, T2 b) { return mk<T2>(a.x - b.x, a.y - b.y); }
 template <typename T2> __device__ __forceinline__ T2 cmul(T2 a, T2 b) {

@@ -153,48 +153,6 @@ template <typename T2> struct LoadZeroInsert {
   }
 };
 
-// Column pass: grid (L1/C, nbatch).  Z[b][k2][n1] = W_L^{n1 k2} sum_n2 x[b][n1 + L1 n2] W_{P2}^{n2 k2}.
-template <typename T, int P2, int C, int DIR, typename Loader>
-__global__ __launch_bounds__(NT) void k_fft_col(Loader ld, cplx<T>* __restrict__ Z, int L1,
-                                                const cplx<T>* __restrict__ twc, const cplx<T>* __restrict__ hi,
-                                                const cplx<T>* __restrict__ lo, int twshift) {
-  using T2 = cplx<T>;
-  __shared__ T2 s[P2 * C];
-  const int b = blockIdx.y;
-  const int n1_0 = blockIdx.x * C;
-  const size_t L = (size_t)L1 * P2;
-#pragma unroll
-  for (int q = 0; q < (P2 * C) / NT; q++) {
-    const int e = threadIdx.x + q * NT;
-    const int n2 = e / C, c = e % C;
-    s[e] = ld(b, n1_0 + c + L1 * n2);
-  }
-  __syncthreads();
-  lds_fft<T2, P2, C, NT, DIR>(s, twc);
-#pragma unroll
-  for (int q = 0; q < (P2 * C) / NT; q++) {
-    const int e = threadIdx.x + q * NT;
-    const int k2 = e / C, c = e % C;
-    const uint32_t m = (uint32_t)(n1_0 + c) * (uint32_t)k2;
-    const T2 w = tw_dir<DIR>(tw_big(hi, lo, m << twshift));
-    Z[(size_t)b * L + (size_t)k2 * L1 + n1_0 + c] = cmul(s[e], w);
-  }
-}
-
-// Row pass, in place: grid (L2, nbatch); row k2 of Z[b] becomes X[L2 k1 + k2] at Z[b][k2][k1].
-template <typename T, int P1, int DIR>
-__global__ __launch_bounds__(NT) void k_fft_row(cplx<T>* __restrict__ Z, int L2, const cplx<T>* __restrict__ twr) {
-  using T2 = cplx<T>;
-  __shared__ T2 s[P1];
-  T2* row = Z + (size_t)blockIdx.y * P1 * L2 + (size_t)blockIdx.x * P1;
-#pragma unroll
-  for (int q = 0; q < P1 / NT; q++) s[threadIdx.x + q * NT] = row[threadIdx.x + q * NT];
-  __syncthreads();
-  lds_fft<T2, P1, 1, NT, DIR>(s, twr);
-#pragma unroll
-  for (int q = 0; q < P1 / NT; q++) row[threadIdx.x + q * NT] = s[threadIdx.x + q * NT];
-}
-
 // T-layout (rows of L1, L2 rows) -> natural order y[b][j] = scale * X[j], j < count.
 // grid (ceil(count/ (32*L2)) ... ) handled as tiles of 32 x 32 over the [L2][L1] matrix.
 template <typename T>
@@ -309,19 +267,29 @@ __global__ __launch_bounds__(NT) void k_spline_eval(const cplx<T>* __restrict__ 
 
 // ---------------------------------------------------------- template stage --
 
-struct SpinParams {
-  double M0, M4, M5, M9, M13;
-  int nn, mm, ss0, s0_flag;
-  int N, L1, nifo, twshift;
+// One template of a launch: the resampled series of its sky point (a sky slot), its shifts and
+// noise-weighted aa/bb, and its frequency offset het1 and spindown sgnlt1 computed on the host
+// with the reference's expressions (jobcore.c:456,466-467).  Templates of one launch may come
+// from different sky points and segments that share the F-statistic FFT length.
+// One template: its sky slot (whose resampled data xm, barycentric shifts shft and aa/bb sums the
+// kernels find at slot * stride from the slot bases), spindown phase coefficients and data length.
+struct TmplDesc {
+  double het1, sgnlt1;
+  int slot, N, nifo, pad;
+};
+static_assert(sizeof(TmplDesc) == 32, "descriptor layout");
+
+// Sky-slot bases: xm cplx<T>[slot][nifo][{a,b}][N], shft double[slot][nifo][N], aabb double[slot][2].
+struct SlotBases {
+  const void* xm;
+  const double* shft;
+  const double* aabb;
+  unsigned long long xm_stride, shft_stride;  // elements per slot
 };
 
-__device__ __forceinline__ void spin_params(const SpinParams& p, int ss, double& het1, double& sgnlt1) {
-  // jobcore.c:456,466-467 evaluated without contraction so the values equal the host's
-#pragma clang fp contract(off)
-  sgnlt1 = p.s0_flag ? 0.0 : (double)ss * p.M5 + (double)p.nn * p.M9 + (double)p.mm * p.M13;
-  het1 = fmod((double)ss * p.M4, p.M0);
-  if (het1 < 0) het1 += p.M0;
-}
+struct SpinParams {
+  int L1, twshift;
+};
 
 template <typename T> struct PhaseRot;
 template <> struct PhaseRot<double> {
@@ -344,142 +312,17 @@ template <> struct PhaseRot<float> {
   }
 };
 
-// Column pass of the F-statistic FFT with the spindown demodulation fused into its load
-// (jobcore.c:473-627): for template t (ss = ss0 + t), x_a[n] = sum_d xm[d][a][n] e^{-i phi_d(n)},
-// phi_d(n) = het1 n + sgnlt1 (n^2 + 2 n shft_d[n]), zero for n >= N; same for x_b.
-// grid (L1/C, ntemplates).  Output Z[t][{a,b}][k2][n1].
-template <typename T, int P2, int C>
-__global__ __launch_bounds__(NT) void k_spin_col(SpinParams p, const cplx<T>* __restrict__ xm,
-                                                 const double* __restrict__ shft, cplx<T>* __restrict__ Z,
-                                                 const cplx<T>* __restrict__ twc, const cplx<T>* __restrict__ hi,
-                                                 const cplx<T>* __restrict__ lo) {
-  using T2 = cplx<T>;
-  __shared__ T2 sa[P2 * C];
-  __shared__ T2 sb[P2 * C];
-  const int t = blockIdx.y;
-  const int n1_0 = blockIdx.x * C;
-  const int L1 = p.L1, N = p.N;
-  const size_t L = (size_t)L1 * P2;
-  double het1, sgnlt1;
-  spin_params(p, p.ss0 + t, het1, sgnlt1);
-#pragma unroll
-  for (int q = 0; q < (P2 * C) / NT; q++) {
-    const int e = threadIdx.x + q * NT;
-    const int n2 = e / C, c = e % C;
-    const int n = n1_0 + c + L1 * n2;
-    T2 xa = mk<T2>(0, 0), xb = mk<T2>(0, 0);
-    if (n < N) {
-      const double dn = (double)n;
-      for (int d = 0; d < p.nifo; d++) {
-        const double sh = shft[(size_t)d * N + n];
-        // _tmp1 = i^2 + 2 i shft, phase = het1 i + sgnlt1 _tmp1 (jobcore.c:294,530), uncontracted
-        const double tmp1 = lin2_nc(dn, dn, (double)(2 * n), sh);
-        const T2 ex = PhaseRot<T>::eval(lin2_nc(het1, dn, sgnlt1, tmp1));
-        const T2 va = xm[((size_t)(2 * d)) * N + n];
-        const T2 vb = xm[((size_t)(2 * d + 1)) * N + n];
-        xa = cadd(xa, cmul(va, ex));
-        xb = cadd(xb, cmul(vb, ex));
-      }
-    }
-    sa[e] = xa;
-    sb[e] = xb;
-  }
-  __syncthreads();
-  lds_fft<T2, P2, C, NT, -1>(sa, twc);
-  lds_fft<T2, P2, C, NT, -1>(sb, twc);
-  T2* Za = Z + (size_t)t * 2 * L;
-  T2* Zb = Za + L;
-#pragma unroll
-  for (int q = 0; q < (P2 * C) / NT; q++) {
-    const int e = threadIdx.x + q * NT;
-    const int k2 = e / C, c = e % C;
-    const uint32_t m = (uint32_t)(n1_0 + c) * (uint32_t)k2;
-    const T2 w = tw_big(hi, lo, m << p.twshift);
-    const size_t o = (size_t)k2 * L1 + n1_0 + c;
-    Za[o] = cmul(sa[e], w);
-    Zb[o] = cmul(sb[e], w);
-  }
-}
-
 struct RowParams {
   int L2, nmin, nmax, nblk, nparts;
 };
 
-// Row pass + F-statistic + FStat partials: grid (L2, ntemplates).
-// F(k) for k = L2 k1 + k2 in [nmin, nmax) is stored at Ft[t][k2][k1]; psum/pmax[t][part][k2]
-// hold the row's sum and max of F over each FStat block (part < nblk) and the raw tail.
-template <typename T, int P1>
-__global__ __launch_bounds__(NT) void k_spin_row(RowParams p, const cplx<T>* __restrict__ Z,
-                                                 const double* __restrict__ aabb, const cplx<T>* __restrict__ twr,
-                                                 T* __restrict__ Ft, T* __restrict__ psum, T* __restrict__ pmax) {
-  using T2 = cplx<T>;
-  constexpr int Q = P1 / NT;
-  __shared__ T2 s[P1];
-  const int k2 = blockIdx.x, t = blockIdx.y;
-  const size_t L = (size_t)P1 * p.L2;
-  const T2* rowa = Z + (size_t)t * 2 * L + (size_t)k2 * P1;
-  const T2* rowb = rowa + L;
-  const T aa = (T)aabb[0], bb = (T)aabb[1];
-  T F[Q];
-#pragma unroll
-  for (int q = 0; q < Q; q++) s[threadIdx.x + q * NT] = rowa[threadIdx.x + q * NT];
-  __syncthreads();
-  lds_fft<T2, P1, 1, NT, -1>(s, twr);
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const T2 v = s[threadIdx.x + q * NT];
-    F[q] = (v.x * v.x + v.y * v.y) / aa;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < Q; q++) s[threadIdx.x + q * NT] = rowb[threadIdx.x + q * NT];
-  __syncthreads();
-  lds_fft<T2, P1, 1, NT, -1>(s, twr);
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const T2 v = s[threadIdx.x + q * NT];
-    F[q] = F[q] + (v.x * v.x + v.y * v.y) / bb;
-  }
-  __syncthreads();
-  T* Fs = reinterpret_cast<T*>(s);
-  T* Frow = Ft + (size_t)t * L + (size_t)k2 * P1;
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const int k1 = threadIdx.x + q * NT;
-    const int k = p.L2 * k1 + k2;
-    const bool ok = (k >= p.nmin && k < p.nmax);
-    if (ok) Frow[k1] = F[q];
-    Fs[k1] = ok ? F[q] : (T)0;
-  }
-  __syncthreads();
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int part = w; part < p.nparts; part += NT / 64) {
-    const int lok = p.nmin + PGW_NAV_DEV * part;
-    const int hik = (part < p.nblk) ? lok + PGW_NAV_DEV : p.nmax;
-    const int jlo = max(0, (lok - k2 + p.L2 - 1) / p.L2);
-    const int jhi = min((hik - k2 + p.L2 - 1) / p.L2, P1);
-    T sum = 0, mx = 0;
-    for (int j = jlo + lane; j < jhi; j += 64) {
-      const T f = Fs[j];
-      sum += f;
-      mx = f > mx ? f : mx;
-    }
-    sum = wave_sum(sum);
-    mx = wave_max(mx);
-    if (lane == 0) {
-      psum[((size_t)t * p.nparts + part) * p.L2 + k2] = sum;
-      pmax[((size_t)t * p.nparts + part) * p.L2 + k2] = mx;
-    }
-  }
-}
-
 struct Cand {
-  int ss, ii;
-  double F;
+  int t, ii;   // template index within the group, frequency bin
+  double F;    // normalised F at the run maximum
 };
 
 struct ThrParams {
-  int L1, L2, nmin, nmax, nblk, nparts, white, ss0, cap;
+  int L1, L2, nmin, nmax, nblk, nparts, white, t0, cap;
   double trl;
 };
 
@@ -546,12 +389,18 @@ __global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restri
     const int idx = atomicAdd(ncand, 1);
     if (idx < p.cap) {
       Cand c;
-      c.ss = p.ss0 + t;
+      c.t = p.t0 + t;
       c.ii = ii;
       c.F = Fc;
       cand[idx] = c;
     }
   }
+}
+
+// zero n ints (used instead of a memset so the launch sequence can be captured in a graph)
+__global__ void k_zero_int(int* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 // plain copy (grid-stride), used to calibrate hardware byte counters

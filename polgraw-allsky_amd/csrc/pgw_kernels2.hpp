@@ -20,21 +20,23 @@ template <> struct ColE<double, 256> { static constexpr int E = 16; static const
 //   x_{a,b}[n] += xm[d][{a,b}][n] exp(-i phi_d(n)),  phi_d(n) = het1 n + sgnlt1 (n^2 + 2 n shft_d[n]),
 // zero for n >= N; the column FFT and the W_L^{n1 k2} twiddle follow; Z[t][{a,b}][k2][n1].
 template <typename T, int P2, bool SEQ>
-__global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, const cplx<T>* __restrict__ xm,
-                                                   const double* __restrict__ shft, cplx<T>* __restrict__ Z,
-                                                   const cplx<T>* __restrict__ twc, const cplx<T>* __restrict__ hi,
-                                                   const cplx<T>* __restrict__ lo) {
+__global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, const TmplDesc* __restrict__ desc,
+                                                   cplx<T>* __restrict__ Z, const cplx<T>* __restrict__ twc,
+                                                   const cplx<T>* __restrict__ hi, const cplx<T>* __restrict__ lo) {
   using T2 = cplx<T>;
   constexpr int E = ColE<T, P2>::E;
   using S = ColShape<P2, E>;
   __shared__ T2 lds[(SEQ ? 1 : 2) * P2 * S::C];
   const int t = blockIdx.y;
+  const TmplDesc& dsc = desc[t];
+  const T2* __restrict__ xm = static_cast<const T2*>(sb.xm) + (size_t)dsc.slot * sb.xm_stride;
+  const double* __restrict__ shft = sb.shft + (size_t)dsc.slot * sb.shft_stride;
+  const double het1 = dsc.het1, sgnlt1 = dsc.sgnlt1;
+  const int N = dsc.N, nifo = dsc.nifo;
   const int c = threadIdx.x % S::C, g = threadIdx.x / S::C;
   const int n1 = blockIdx.x * S::C + c;
-  const int L1 = p.L1, N = p.N;
+  const int L1 = p.L1;
   const size_t L = (size_t)L1 * P2;
-  double het1, sgnlt1;
-  spin_params(p, p.ss0 + t, het1, sgnlt1);
   T2 x[2][E];
 #pragma unroll
   for (int r = 0; r < E; r++) {
@@ -42,13 +44,13 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, const cplx<T>* 
     T2 xa = mk<T2>(0, 0), xb = mk<T2>(0, 0);
     if (n < N) {
       const double dn = (double)n;
-      for (int d = 0; d < p.nifo; d++) {
-        const double sh = shft[(size_t)d * N + n];
+      for (int d = 0; d < nifo; d++) {
+        const double sh = ld_global(shft, (size_t)d * N + n);
         // _tmp1 = i^2 + 2 i shft, phase = het1 i + sgnlt1 _tmp1 (jobcore.c:294,530), uncontracted
         const double tmp1 = lin2_nc(dn, dn, (double)(2 * n), sh);
         const T2 ex = PhaseRot<T>::eval(lin2_nc(het1, dn, sgnlt1, tmp1));
-        xa = cadd(xa, cmul(xm[(size_t)(2 * d) * N + n], ex));
-        xb = cadd(xb, cmul(xm[(size_t)(2 * d + 1) * N + n], ex));
+        xa = cadd(xa, cmul(ld_global(xm, (size_t)(2 * d) * N + n), ex));
+        xb = cadd(xb, cmul(ld_global(xm, (size_t)(2 * d + 1) * N + n), ex));
       }
     }
     x[0][r] = xa;
@@ -84,8 +86,9 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, const cplx<T>* 
 // P1/16 threads.  F(k), k = L2 k1 + k2 in [nmin, nmax), at Ft[t][k2][k1]; psum/pmax[t][part][k2].
 // NA = 2 transforms both spectra together; NA = 1 one after the other through the same LDS.
 template <typename T, int P1, int NA>
-__global__ __launch_bounds__(P1 / 16) void k_spin_row2(RowParams p, const cplx<T>* __restrict__ Z,
-                                                      const double* __restrict__ aabb,
+__global__ __launch_bounds__(P1 / 16) void k_spin_row2(RowParams p, const double* __restrict__ aabb_slots,
+                                                      const TmplDesc* __restrict__ desc,
+                                                      const cplx<T>* __restrict__ Z,
                                                       const cplx<T>* __restrict__ twr, T* __restrict__ Ft,
                                                       T* __restrict__ psum, T* __restrict__ pmax) {
   using T2 = cplx<T>;
@@ -96,7 +99,8 @@ __global__ __launch_bounds__(P1 / 16) void k_spin_row2(RowParams p, const cplx<T
   const size_t L = (size_t)P1 * p.L2;
   const T2* rowa = Z + (size_t)t * 2 * L + (size_t)k2 * P1;
   const T2* rowb = rowa + L;
-  const T aa = (T)aabb[0], bb = (T)aabb[1];
+  const double* aabb = aabb_slots + 2 * desc[t].slot;
+  const T aa = (T)ld_global(aabb, 0), bb = (T)ld_global(aabb, 1);
   T F[16];
   if constexpr (NA == 2) {
     T2 x[2][16];

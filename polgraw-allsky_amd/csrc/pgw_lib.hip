@@ -1,9 +1,15 @@
-// pgw_lib.hip - host side of the MI355X search core: device state, launch sequence of one
-// sky point, and the C ABI declared in include/pgw.h.
+// pgw_lib.hip - host side of the MI355X search core: device state, launch sequences, and the
+// C ABI declared in include/pgw.h.
 //
-// One handle = one segment (detector data resident in HBM) + one HIP stream.  job_core keeps
-// the reference's per-sky-point semantics (search/network/src-cpu/jobcore.c:189-785); the host
-// only does the scalar bookkeeping (grid -> sky coordinates, spindown range, candidate records).
+// A handle holds one or more segments (detector data resident in HBM) and one HIP stream.  Work
+// is a list of (segment, pm, mm, nn) sky points processed in groups: the sky-point stage of each
+// point of a group fills one of kSlots slots (resampled series, shifts, aa/bb), then template
+// launches of up to TB templates run over all spindowns of all points of the group, described by
+// per-template descriptors in device memory.  Points of one group share the F-statistic FFT
+// length and bin range, so segments of uneven N with the same nfft batch together.  The host does
+// only the scalar bookkeeping of job_core (search/network/src-cpu/jobcore.c:189-785): sky
+// coordinates, spindown range, per-template het1/sgnlt1 and the candidate records; it turns the
+// records of group g-1 into output while the device works on group g.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -11,6 +17,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <new>
 #include <vector>
 
@@ -30,8 +38,11 @@ using namespace pgw;
 
 namespace {
 
-constexpr int kCandCap = 1 << 20;   // candidates per sky point (device buffer)
+constexpr int kCandCap = 1 << 20;   // candidates per group (device buffer)
 constexpr int kCandChunk = 4096;    // candidates copied back with the count, without a second sync
+constexpr int kSlots = 16;          // sky points per group
+constexpr int kDescCap = 1 << 16;   // templates per group
+constexpr int kTBMax = 256;         // templates per launch, upper bound
 
 int ilog2(int v) {
   int l = 0;
@@ -59,6 +70,7 @@ template <typename T> struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
   int alloc(size_t count) {
+    release();
     n = count;
     if (hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) { p = nullptr; return PGW_ERR_NOMEM; }
     return PGW_OK;
@@ -67,9 +79,21 @@ template <typename T> struct DevBuf {
     if (p) (void)hipFree(p);
     p = nullptr;
   }
+  ~DevBuf() { release(); }
 };
 
-// Twiddle table W_P^m = exp(-2 pi i m / P), m < P, computed in FP64 on the host.
+template <typename T> struct PinBuf {
+  T* p = nullptr;
+  int alloc(size_t count) {
+    return hipHostMalloc((void**)&p, sizeof(T) * std::max<size_t>(count, 1), hipHostMallocDefault) == hipSuccess
+               ? PGW_OK : PGW_ERR_NOMEM;
+  }
+  ~PinBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+// Twiddle table W_P^m = exp(-2 pi i m / P) (m scaled by 2^stride_exp), computed in FP64 on the host.
 template <typename T> int upload_tw(DevBuf<cplx<T>>& buf, int P, int stride_exp = 0, int count = -1) {
   if (count < 0) count = P;
   std::vector<cplx<T>> h(count);
@@ -97,7 +121,6 @@ template <typename T> struct Plan {
     if (!st) st = upload_tw<T>(twr, L1);
     return st;
   }
-  void release() { twc.release(); twr.release(); }
 };
 
 // ---------------------------------------------------------- launch helpers
@@ -134,16 +157,17 @@ void launch_row(const Plan<T>& pl, cplx<T>* Z, int nbatch, hipStream_t st) {
 }
 
 template <typename T>
-void launch_spin_col(const Plan<T>& pl, const SpinParams& sp, const cplx<T>* xm, const double* shft, cplx<T>* Z,
-                     int nt, const cplx<T>* hi, const cplx<T>* lo, bool seq, hipStream_t st) {
+void launch_spin_col(const Plan<T>& pl, const SlotBases& sb, const TmplDesc* desc, cplx<T>* Z, int nt, const cplx<T>* hi,
+                     const cplx<T>* lo, bool seq, hipStream_t st) {
+  const SpinParams sp{pl.L1, pl.twshift};
 #define PGW_SCOL_CASE(P2)                                                                                \
   case P2:                                                                                               \
     if (seq || ColE<T, P2>::SEQ)                                                                         \
-      hipLaunchKernelGGL((k_spin_col2<T, P2, true>), dim3(pl.L1 / col_tile<T, P2>(), nt), dim3(256), 0, st, sp, xm, \
-                         shft, Z, pl.twc.p, hi, lo);                                                     \
+      hipLaunchKernelGGL((k_spin_col2<T, P2, true>), dim3(pl.L1 / col_tile<T, P2>(), nt), dim3(256), 0, st, sp, sb, desc, \
+                         Z, pl.twc.p, hi, lo);                                                           \
     else                                                                                                 \
-      hipLaunchKernelGGL((k_spin_col2<T, P2, false>), dim3(pl.L1 / col_tile<T, P2>(), nt), dim3(256), 0, st, sp, xm, \
-                         shft, Z, pl.twc.p, hi, lo);                                                     \
+      hipLaunchKernelGGL((k_spin_col2<T, P2, false>), dim3(pl.L1 / col_tile<T, P2>(), nt), dim3(256), 0, st, sp, sb, desc, \
+                         Z, pl.twc.p, hi, lo);                                                           \
     break;
   switch (pl.L2) {
     PGW_SCOL_CASE(16)
@@ -156,15 +180,15 @@ void launch_spin_col(const Plan<T>& pl, const SpinParams& sp, const cplx<T>* xm,
 }
 
 template <typename T>
-void launch_spin_row(const Plan<T>& pl, const RowParams& rp, const cplx<T>* Z, const double* aabb, T* Ft, T* psum,
+void launch_spin_row(const Plan<T>& pl, const RowParams& rp, const double* aabb, const TmplDesc* desc, const cplx<T>* Z, T* Ft, T* psum,
                      T* pmax, int nt, int na, hipStream_t st) {
 #define PGW_SROW_CASE(P1)                                                                                 \
   case P1:                                                                                                \
     if (na == 2)                                                                                          \
-      hipLaunchKernelGGL((k_spin_row2<T, P1, 2>), dim3(pl.L2, nt), dim3(P1 / 16), 0, st, rp, Z, aabb, pl.twr.p, Ft, \
+      hipLaunchKernelGGL((k_spin_row2<T, P1, 2>), dim3(pl.L2, nt), dim3(P1 / 16), 0, st, rp, aabb, desc, Z, pl.twr.p, Ft, \
                          psum, pmax);                                                                     \
     else                                                                                                  \
-      hipLaunchKernelGGL((k_spin_row2<T, P1, 1>), dim3(pl.L2, nt), dim3(P1 / 16), 0, st, rp, Z, aabb, pl.twr.p, Ft, \
+      hipLaunchKernelGGL((k_spin_row2<T, P1, 1>), dim3(pl.L2, nt), dim3(P1 / 16), 0, st, rp, aabb, desc, Z, pl.twr.p, Ft, \
                          psum, pmax);                                                                     \
     break;
   switch (pl.L1) {
@@ -190,13 +214,13 @@ void lin2ast(double be1, double be2, int pm, double sepsm, double cepsm, double*
 // ---------------------------------------------------------------- handle --
 struct pgw_handle {
   virtual ~pgw_handle() {}
-  virtual int job_core(int pm, int mm, int nn, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap,
-                       int* FNum) = 0;
-  virtual int job_batch(const int* sky, int nsky, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv,
-                        int cap, int* FNum) = 0;
+  virtual int run(const int* work, int n, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap,
+                  int* rec_work, int* FNum) = 0;
   virtual int debug_fstat(int pm, int mm, int nn, int ss, const pgw_opts* o, double* F) = 0;
   virtual int debug_resampled(int pm, int mm, int nn, int det, double* a, double* b) = 0;
-  virtual int time_stages(int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps, double* ms) = 0;
+  virtual int time_stages(int seg, int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps,
+                          double* ms) = 0;
+  virtual int autotune(int seg, int pm, int mm, int nn, const pgw_opts* o, int* tb, int* row_na, int* col_seq) = 0;
   double last_ms = 0.0;
 };
 
@@ -206,117 +230,184 @@ struct SkyState {
   double het0, sgnlt2, sgnlt3;
 };
 
-template <typename T> struct Impl final : pgw_handle {
+template <typename T> struct Engine final : pgw_handle {
   using T2 = cplx<T>;
-  pgw_settings sett;
-  int nifo = 0;
-  std::vector<double> amod, sig2, sphir, cphir, ssb0;  // per detector
-  Plan<T> pf, pi, ps;                                    // nfft fwd, Ninterp inv, L = nfftf
-  DevBuf<T2> hi, lo;
-  DevBuf<double> xDat, DetSSB, cosmodf, sinmodf, shft, shftf, paa, pbb, aabb, mu;
-  DevBuf<T2> R, Zf, Zi, Y, xm, Z;
-  DevBuf<double2> u, y2;
-  DevBuf<T> Ft, psum, pmax;
-  DevBuf<SplineTab> spl;
-  DevBuf<Cand> cand;
-  DevBuf<int> ncand;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t evd[2] = {nullptr, nullptr};  // candidate copy of buffer 0/1 done
-  Cand* hc_pin = nullptr;                  // pinned mirrors: 2 x kCandChunk candidates, 2 counts
-  int* hn_pin = nullptr;
-  int TB = 128;         // templates per launch
-  int row_na = 1;        // row pass: 2 = both spectra through one LDS exchange, 1 = one after the other
-  bool col_seq = false;  // column pass: spectra share one LDS buffer in turn
-  int nblk = 0, nparts = 0, npro = 0;
-  int nspl = 0;         // spline knots interpftpad*N
-  std::vector<Cand> hcand;
 
-  ~Impl() override {
-    pf.release(); pi.release(); ps.release();
-    hi.release(); lo.release();
-    xDat.release(); DetSSB.release(); cosmodf.release(); sinmodf.release(); shft.release(); shftf.release();
-    paa.release(); pbb.release(); aabb.release(); mu.release();
-    R.release(); Zf.release(); Zi.release(); Y.release(); xm.release(); Z.release();
-    u.release(); y2.release(); Ft.release(); psum.release(); pmax.release(); spl.release();
-    cand.release(); ncand.release();
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
-    for (auto& e : evd)
+  struct Seg {
+    pgw_settings sett;
+    std::vector<double> amod, sig2, sphir, cphir, ssb0;
+    DevBuf<double> xDat, DetSSB, cosmodf, sinmodf;
+    Plan<T>*pf = nullptr, *pi = nullptr, *ps = nullptr;
+    int nblk = 0, nparts = 0, npro = 0, nspl = 0;
+  };
+
+  // one sky point of a group
+  struct Item {
+    int work, seg, pm, mm, nn, slot, smin, smax, t0;
+    SkyState sk;
+  };
+  struct Group {
+    std::vector<Item> items;
+    int ntempl = 0, buf = 0;
+  };
+
+  std::vector<std::unique_ptr<Seg>> segs;
+  std::map<int, std::unique_ptr<Plan<T>>> plans;
+  int nifo = 0;
+  size_t Nmax = 0, nfftmax = 0, Nimax = 0, Lmax = 0, L2max = 0, partsmax = 0, nsplmax = 0, npromax = 0;
+  DevBuf<T2> hi, lo;
+  // shared sky-stage scratch
+  DevBuf<T2> R, Zf, Zi, Y;
+  DevBuf<double2> u, y2;
+  DevBuf<double> shftf, paa, pbb;
+  DevBuf<SplineTab> spl;
+  // sky slots
+  DevBuf<T2> xm_slots;
+  DevBuf<double> shft_slots, aabb_slots;
+  size_t slot_xm = 0, slot_shft = 0;
+  // template stage
+  DevBuf<T2> Z;
+  DevBuf<T> Ft, psum, pmax;
+  DevBuf<double> mu;
+  DevBuf<TmplDesc> desc;  // 2 x kDescCap
+  PinBuf<TmplDesc> desc_pin;
+  DevBuf<Cand> cand;      // 2 x kCandCap
+  DevBuf<int> ncand;      // 2
+  PinBuf<Cand> hc_pin;    // 2 x kCandChunk
+  PinBuf<int> hn_pin;     // 2
+  std::vector<Cand> hcand;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evd[2] = {nullptr, nullptr};
+  int TB = 128, TBalloc = 0;
+  int row_na = 1;
+  bool col_seq = false;
+
+  ~Engine() override {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (hipEvent_t e : {ev0, ev1, evd[0], evd[1]})
       if (e) (void)hipEventDestroy(e);
-    if (hc_pin) (void)hipHostFree(hc_pin);
-    if (hn_pin) (void)hipHostFree(hn_pin);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  int init(const pgw_settings* s, const pgw_detector* ifo, int n) {
-    sett = *s;
-    nifo = n;
-    sett.nifo = n;
-    const int N = sett.N;
-    if (sett.Ninterp != sett.interpftpad * sett.nfft || sett.nfftf != sett.fftpad * sett.nfft) return PGW_ERR_ARG;
-    if (sett.interpftpad * N > sett.Ninterp || N > sett.nfft || sett.nmin < 1 || sett.nmax > sett.nfftf ||
-        sett.nmax <= sett.nmin)
+  Plan<T>* plan(int L, int& st) {
+    auto it = plans.find(L);
+    if (it != plans.end()) return it->second.get();
+    auto p = std::make_unique<Plan<T>>();
+    st = p->init(L);
+    if (st) return nullptr;
+    Plan<T>* q = p.get();
+    plans[L] = std::move(p);
+    return q;
+  }
+
+  int add_segment(const pgw_settings* s, const pgw_detector* ifo) {
+    auto sg = std::make_unique<Seg>();
+    Seg& g = *sg;
+    g.sett = *s;
+    g.sett.nifo = nifo;
+    const pgw_settings& S = g.sett;
+    const int N = S.N;
+    if (S.Ninterp != S.interpftpad * S.nfft || S.nfftf != S.fftpad * S.nfft) return PGW_ERR_ARG;
+    if (S.interpftpad * N > S.Ninterp || N > S.nfft || S.nmin < 1 || S.nmax > S.nfftf || S.nmax <= S.nmin ||
+        !(S.oms > 0) || S.numlines_band < 0 || S.numlines_band > PGW_MAXL)
       return PGW_ERR_ARG;
-    if (sett.Ninterp - sett.nfft != sett.nfft) return PGW_ERR_UNSUPPORTED;  // zero-insert map assumes interpftpad = 2
-    const int M = sett.nmax - sett.nmin;
-    nblk = M / PGW_NAV;
-    nparts = nblk + (M % PGW_NAV ? 1 : 0);
-    if (nparts > 256) return PGW_ERR_UNSUPPORTED;
-    nspl = sett.interpftpad * N;
-    // kernel variants; PGW_TUNE="row_na=2,col_seq=0,tb=32" overrides (used when measuring variants)
-    if (const char* tune = getenv("PGW_TUNE")) {
-      int v;
-      if (const char* q = strstr(tune, "row_na=")) if (sscanf(q, "row_na=%d", &v) == 1 && (v == 1 || v == 2)) row_na = v;
-      if (const char* q = strstr(tune, "col_seq=")) if (sscanf(q, "col_seq=%d", &v) == 1) col_seq = v != 0;
-      if (const char* q = strstr(tune, "tb=")) if (sscanf(q, "tb=%d", &v) == 1 && v >= 1 && v <= 256) TB = v;
-    }
-    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIPCHK(hipEventCreate(&ev0));
-    HIPCHK(hipEventCreate(&ev1));
-    HIPCHK(hipEventCreateWithFlags(&evd[0], hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&evd[1], hipEventDisableTiming));
-    if (hipHostMalloc((void**)&hc_pin, sizeof(Cand) * 2 * kCandChunk, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&hn_pin, sizeof(int) * 2, hipHostMallocDefault) != hipSuccess)
-      return PGW_ERR_NOMEM;
-    int st;
-    if ((st = pf.init(sett.nfft)) || (st = pi.init(sett.Ninterp)) || (st = ps.init(sett.nfftf))) return st;
-    // W_{2^20}^m two-level tables: hi[h] = W^{1024 h}, lo[l] = W^l
-    if ((st = upload_tw<T>(hi, 1 << 20, 10, 1024)) || (st = upload_tw<T>(lo, 1 << 20, 0, 1024))) return st;
-    amod.resize(9 * n); sig2.resize(n); sphir.resize(n); cphir.resize(n); ssb0.resize(3 * n);
-    std::vector<double> hx((size_t)n * N), hd((size_t)n * 3 * N);
-    for (int d = 0; d < n; d++) {
+    if (S.Ninterp - S.nfft != S.nfft) return PGW_ERR_UNSUPPORTED;  // zero-insert map assumes interpftpad = 2
+    const int M = S.nmax - S.nmin;
+    g.nblk = M / PGW_NAV;
+    g.nparts = g.nblk + (M % PGW_NAV ? 1 : 0);
+    if (g.nparts > 256) return PGW_ERR_UNSUPPORTED;
+    g.nspl = S.interpftpad * N;
+    g.npro = (S.nfft + NT - 1) / NT;
+    int st = PGW_OK;
+    if (!(g.pf = plan(S.nfft, st)) || !(g.pi = plan(S.Ninterp, st)) || !(g.ps = plan(S.nfftf, st)))
+      return st ? st : PGW_ERR_UNSUPPORTED;
+    g.amod.resize(9 * nifo);
+    g.sig2.resize(nifo);
+    g.sphir.resize(nifo);
+    g.cphir.resize(nifo);
+    g.ssb0.resize(3 * nifo);
+    std::vector<double> hx((size_t)nifo * N), hd((size_t)nifo * 3 * N);
+    for (int d = 0; d < nifo; d++) {
       if (!ifo[d].xDat || !ifo[d].DetSSB || !(ifo[d].sig2 > 0)) return PGW_ERR_ARG;
-      memcpy(&amod[9 * d], ifo[d].amod, 9 * sizeof(double));
-      sig2[d] = ifo[d].sig2; sphir[d] = ifo[d].sphir; cphir[d] = ifo[d].cphir;
-      memcpy(&ssb0[3 * d], ifo[d].DetSSB, 3 * sizeof(double));
+      memcpy(&g.amod[9 * d], ifo[d].amod, 9 * sizeof(double));
+      g.sig2[d] = ifo[d].sig2;
+      g.sphir[d] = ifo[d].sphir;
+      g.cphir[d] = ifo[d].cphir;
+      memcpy(&g.ssb0[3 * d], ifo[d].DetSSB, 3 * sizeof(double));
       memcpy(&hx[(size_t)d * N], ifo[d].xDat, sizeof(double) * N);
       memcpy(&hd[(size_t)d * 3 * N], ifo[d].DetSSB, sizeof(double) * 3 * N);
     }
     // Earth-rotation auxiliary arrays, init.c:448-455 (host glibc, same values as the CPU path)
     std::vector<double> hc(N), hs(N);
     for (int i = 0; i < N; i++) {
-      const double omrt = sett.omr * i;
+      const double omrt = S.omr * i;
       hc[i] = cos(omrt);
       hs[i] = sin(omrt);
     }
-    npro = (sett.nfft + NT - 1) / NT;
-    const size_t L = sett.nfftf;
-    if ((st = xDat.alloc((size_t)n * N)) || (st = DetSSB.alloc((size_t)n * 3 * N)) || (st = cosmodf.alloc(N)) ||
-        (st = sinmodf.alloc(N)) || (st = shft.alloc((size_t)n * N)) || (st = shftf.alloc((size_t)n * N)) ||
-        (st = paa.alloc((size_t)n * npro)) || (st = pbb.alloc((size_t)n * npro)) || (st = aabb.alloc(2)) ||
-        (st = R.alloc((size_t)2 * n * sett.nfft)) || (st = Zf.alloc((size_t)2 * n * sett.nfft)) ||
-        (st = Zi.alloc((size_t)2 * n * sett.Ninterp)) || (st = Y.alloc((size_t)2 * n * sett.Ninterp)) ||
-        (st = u.alloc((size_t)2 * n * (nspl + 1))) || (st = y2.alloc((size_t)2 * n * (nspl + 1))) ||
-        (st = xm.alloc((size_t)2 * n * N)) || (st = Z.alloc((size_t)TB * 2 * L)) || (st = Ft.alloc((size_t)TB * L)) ||
-        (st = psum.alloc((size_t)TB * nparts * ps.L2)) || (st = pmax.alloc((size_t)TB * nparts * ps.L2)) ||
-        (st = mu.alloc((size_t)TB * nparts)) || (st = spl.alloc(1)) || (st = cand.alloc(2 * (size_t)kCandCap)) ||
-        (st = ncand.alloc(2)))
+    if ((st = g.xDat.alloc(hx.size())) || (st = g.DetSSB.alloc(hd.size())) || (st = g.cosmodf.alloc(N)) ||
+        (st = g.sinmodf.alloc(N)))
       return st;
-    HIPCHK(hipMemcpy(xDat.p, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(DetSSB.p, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(cosmodf.p, hc.data(), sizeof(double) * N, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(sinmodf.p, hs.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g.xDat.p, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g.DetSSB.p, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g.cosmodf.p, hc.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g.sinmodf.p, hs.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    Nmax = std::max<size_t>(Nmax, N);
+    nfftmax = std::max<size_t>(nfftmax, S.nfft);
+    Nimax = std::max<size_t>(Nimax, S.Ninterp);
+    Lmax = std::max<size_t>(Lmax, S.nfftf);
+    L2max = std::max<size_t>(L2max, g.ps->L2);
+    partsmax = std::max<size_t>(partsmax, g.nparts);
+    nsplmax = std::max<size_t>(nsplmax, g.nspl);
+    npromax = std::max<size_t>(npromax, g.npro);
+    segs.push_back(std::move(sg));
+    return PGW_OK;
+  }
+
+  // template-stage buffers for `tb` templates per launch
+  int ensure_tb(int tb) {
+    if (tb <= TBalloc) return PGW_OK;
+    if (stream) HIPCHK(hipStreamSynchronize(stream));
+    int st;
+    if ((st = Z.alloc((size_t)tb * 2 * Lmax)) || (st = Ft.alloc((size_t)tb * Lmax)) ||
+        (st = psum.alloc((size_t)tb * partsmax * L2max)) || (st = pmax.alloc((size_t)tb * partsmax * L2max)) ||
+        (st = mu.alloc((size_t)tb * partsmax)))
+      return st;
+    TBalloc = tb;
+    return PGW_OK;
+  }
+
+  int init(const pgw_settings* s, const pgw_detector* ifo, int nseg, int ndet) {
+    nifo = ndet;
+    int st;
+    for (int k = 0; k < nseg; k++)
+      if ((st = add_segment(&s[k], ifo + (size_t)k * ndet))) return st;
+    // kernel variants; PGW_TUNE="row_na=2,col_seq=1,tb=64" overrides (used when measuring variants)
+    if (const char* tune = getenv("PGW_TUNE")) {
+      int v;
+      if (const char* q = strstr(tune, "row_na=")) if (sscanf(q, "row_na=%d", &v) == 1 && (v == 1 || v == 2)) row_na = v;
+      if (const char* q = strstr(tune, "col_seq=")) if (sscanf(q, "col_seq=%d", &v) == 1) col_seq = v != 0;
+      if (const char* q = strstr(tune, "tb=")) if (sscanf(q, "tb=%d", &v) == 1 && v >= 1 && v <= kTBMax) TB = v;
+    }
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    HIPCHK(hipEventCreateWithFlags(&evd[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&evd[1], hipEventDisableTiming));
+    // W_{2^20}^m two-level tables: hi[h] = W^{1024 h}, lo[l] = W^l
+    if ((st = upload_tw<T>(hi, 1 << 20, 10, 1024)) || (st = upload_tw<T>(lo, 1 << 20, 0, 1024))) return st;
+    slot_xm = (size_t)2 * nifo * Nmax;
+    slot_shft = (size_t)nifo * Nmax;
+    if ((st = R.alloc((size_t)2 * nifo * nfftmax)) || (st = Zf.alloc((size_t)2 * nifo * nfftmax)) ||
+        (st = Zi.alloc((size_t)2 * nifo * Nimax)) || (st = Y.alloc((size_t)2 * nifo * Nimax)) ||
+        (st = u.alloc((size_t)2 * nifo * (nsplmax + 1))) || (st = y2.alloc((size_t)2 * nifo * (nsplmax + 1))) ||
+        (st = shftf.alloc((size_t)nifo * Nmax)) || (st = paa.alloc((size_t)nifo * npromax)) ||
+        (st = pbb.alloc((size_t)nifo * npromax)) || (st = spl.alloc(1)) ||
+        (st = xm_slots.alloc((size_t)kSlots * slot_xm)) || (st = shft_slots.alloc((size_t)kSlots * slot_shft)) ||
+        (st = aabb_slots.alloc((size_t)kSlots * 2)) || (st = desc.alloc((size_t)2 * kDescCap)) ||
+        (st = desc_pin.alloc((size_t)2 * kDescCap)) || (st = cand.alloc((size_t)2 * kCandCap)) ||
+        (st = ncand.alloc(2)) || (st = hc_pin.alloc((size_t)2 * kCandChunk)) || (st = hn_pin.alloc(2)) ||
+        (st = ensure_tb(TB)))
+      return st;
     // spline recursion coefficients (auxi.c:66-72): invp_i = 2/(c_{i-1}+4), c_i = -invp_i/2, c_0 = 0
     SplineTab tab;
     double c = 0.0;
@@ -336,31 +427,44 @@ template <typename T> struct Impl final : pgw_handle {
     return PGW_OK;
   }
 
-  // Sky-point stage, jobcore.c:243-393.  Returns 1 when the point is inside the disk.
-  int sky_stage(int pm, int mm, int nn, SkyState& ss, int& status) {
+  T2* slot_xm_ptr(int slot) { return xm_slots.p + (size_t)slot * slot_xm; }
+  double* slot_shft_ptr(int slot) { return shft_slots.p + (size_t)slot * slot_shft; }
+  double* slot_aabb_ptr(int slot) { return aabb_slots.p + (size_t)slot * 2; }
+
+  // Sky coordinates of a grid point (jobcore.c:243-264); 0 when outside the disk.
+  int sky_coords(const Seg& g, int pm, int mm, int nn, SkyState& ss, double& sinalt, double& cosalt, double& sindelt,
+                 double& cosdelt) {
 #pragma clang fp contract(off)
-    status = PGW_OK;
-    const double* M = sett.M;
+    const double* M = g.sett.M;
     const double al1 = nn * M[10] + mm * M[14];
     const double al2 = nn * M[11] + mm * M[15];
-    if ((al1 * al1 + al2 * al2) / (sett.oms * sett.oms) > 1.) return 0;
-    double sinalt, cosalt, sindelt, cosdelt;
-    lin2ast(al1 / sett.oms, al2 / sett.oms, pm, sett.sepsm, sett.cepsm, &sinalt, &cosalt, &sindelt, &cosdelt);
+    if ((al1 * al1 + al2 * al2) / (g.sett.oms * g.sett.oms) > 1.) return 0;
+    lin2ast(al1 / g.sett.oms, al2 / g.sett.oms, pm, g.sett.sepsm, g.sett.cepsm, &sinalt, &cosalt, &sindelt, &cosdelt);
     ss.sgnlt2 = asin(sindelt);
     ss.sgnlt3 = fmod(atan2(sinalt, cosalt) + 2. * M_PI, 2. * M_PI);
     ss.het0 = fmod(nn * M[8] + mm * M[12], M[0]);
+    return 1;
+  }
+
+  // Sky-point stage, jobcore.c:270-393, into slot `slot`.  Returns 1 when inside the disk.
+  int sky_stage(const Seg& g, int pm, int mm, int nn, int slot, SkyState& ss, int& status) {
+#pragma clang fp contract(off)
+    status = PGW_OK;
+    double sinalt, cosalt, sindelt, cosdelt;
+    if (!sky_coords(g, pm, mm, nn, ss, sinalt, cosalt, sindelt, cosdelt)) return 0;
+    const pgw_settings& S = g.sett;
     SkyParams p;
     memset(&p, 0, sizeof(p));
     p.het0 = ss.het0;
-    p.oms = sett.oms;
-    p.N = sett.N;
-    p.nfft = sett.nfft;
+    p.oms = S.oms;
+    p.N = S.N;
+    p.nfft = S.nfft;
     p.nifo = nifo;
     for (int d = 0; d < nifo; d++) {
       SkyDet& q = p.det[d];
-      memcpy(q.c, &amod[9 * d], sizeof(q.c));
-      q.cosalfr = cosalt * cphir[d] + sinalt * sphir[d];
-      q.sinalfr = sinalt * cphir[d] - cosalt * sphir[d];
+      memcpy(q.c, &g.amod[9 * d], sizeof(q.c));
+      q.cosalfr = cosalt * g.cphir[d] + sinalt * g.sphir[d];
+      q.sinalfr = sinalt * g.cphir[d] - cosalt * g.sphir[d];
       q.c2d = cosdelt * cosdelt;
       q.c2sd = sindelt * cosdelt;
       q.sindel = sindelt;
@@ -368,173 +472,215 @@ template <typename T> struct Impl final : pgw_handle {
       q.nS0 = cosalt * cosdelt;
       q.nS1 = sinalt * cosdelt;
       q.nS2 = sindelt;
-      q.shft1 = q.nS0 * ssb0[3 * d] + q.nS1 * ssb0[3 * d + 1] + q.nS2 * ssb0[3 * d + 2];
-      q.sig2 = sig2[d];
+      q.shft1 = q.nS0 * g.ssb0[3 * d] + q.nS1 * g.ssb0[3 * d + 1] + q.nS2 * g.ssb0[3 * d + 2];
+      q.sig2 = g.sig2[d];
     }
     const int nb = 2 * nifo;
-    hipLaunchKernelGGL((k_sky_prologue<T>), dim3(npro, nifo), dim3(NT), 0, stream, p, xDat.p, DetSSB.p, cosmodf.p,
-                       sinmodf.p, R.p, shft.p, shftf.p, paa.p, pbb.p);
-    hipLaunchKernelGGL(k_aabb, dim3(1), dim3(NT), 0, stream, p, paa.p, pbb.p, npro, aabb.p);
+    double* shft = slot_shft_ptr(slot);
+    hipLaunchKernelGGL((k_sky_prologue<T>), dim3(g.npro, nifo), dim3(NT), 0, stream, p, g.xDat.p, g.DetSSB.p,
+                       g.cosmodf.p, g.sinmodf.p, R.p, shft, shftf.p, paa.p, pbb.p);
+    hipLaunchKernelGGL(k_aabb, dim3(1), dim3(NT), 0, stream, p, paa.p, pbb.p, g.npro, slot_aabb_ptr(slot));
     // resampling: forward FFT (nfft), zero-insert, backward FFT (Ninterp), scale (jobcore.c:319-362)
-    LoadPlain<T2> lp{R.p, sett.nfft, sett.nfft};
-    launch_col<T, -1>(pf, lp, Zf.p, nb, hi.p, lo.p, stream);
-    launch_row<T, -1>(pf, Zf.p, nb, stream);
-    LoadZeroInsert<T2> lz{Zf.p, sett.nfft, sett.Ninterp, sett.nfft / 2 + 1, pf.L1, pf.L2};
-    launch_col<T, +1>(pi, lz, Zi.p, nb, hi.p, lo.p, stream);
-    launch_row<T, +1>(pi, Zi.p, nb, stream);
-    const int count = std::min(sett.Ninterp, nspl);
-    const int rows_k1 = std::min(pi.L1, (count + pi.L2 - 1) / pi.L2);
-    dim3 gt((rows_k1 + 31) / 32, (pi.L2 + 31) / 32, nb);
-    const T ft = (T)((double)sett.interpftpad / sett.Ninterp);
-    hipLaunchKernelGGL((k_untranspose<T>), gt, dim3(NT), 0, stream, Zi.p, Y.p, pi.L1, pi.L2, count, ft);
+    LoadPlain<T2> lp{R.p, S.nfft, S.nfft};
+    launch_col<T, -1>(*g.pf, lp, Zf.p, nb, hi.p, lo.p, stream);
+    launch_row<T, -1>(*g.pf, Zf.p, nb, stream);
+    LoadZeroInsert<T2> lz{Zf.p, S.nfft, S.Ninterp, S.nfft / 2 + 1, g.pf->L1, g.pf->L2};
+    launch_col<T, +1>(*g.pi, lz, Zi.p, nb, hi.p, lo.p, stream);
+    launch_row<T, +1>(*g.pi, Zi.p, nb, stream);
+    const int count = std::min(S.Ninterp, g.nspl);
+    const int rows_k1 = std::min(g.pi->L1, (count + g.pi->L2 - 1) / g.pi->L2);
+    dim3 gt((rows_k1 + 31) / 32, (g.pi->L2 + 31) / 32, nb);
+    const T ft = (T)((double)S.interpftpad / S.Ninterp);
+    hipLaunchKernelGGL((k_untranspose<T>), gt, dim3(NT), 0, stream, Zi.p, Y.p, g.pi->L1, g.pi->L2, count, ft);
     // spline on the first interpftpad*N samples, evaluated at interpftpad*(i - shftf[i]) (auxi.c:98-122)
-    const int nchunk = (nspl + SPL_K - 1) / SPL_K;
+    const int nchunk = (g.nspl + SPL_K - 1) / SPL_K;
     dim3 gs((nchunk + NT - 1) / NT, nb);
-    hipLaunchKernelGGL((k_spline_u<T>), gs, dim3(NT), 0, stream, Y.p, (size_t)sett.Ninterp, nspl, spl.p, u.p,
-                       (size_t)(nspl + 1));
-    hipLaunchKernelGGL(k_spline_y2, gs, dim3(NT), 0, stream, u.p, (size_t)(nspl + 1), nspl, spl.p, y2.p);
-    dim3 ge((sett.N + NT - 1) / NT, nb);
-    hipLaunchKernelGGL((k_spline_eval<T, T>), ge, dim3(NT), 0, stream, Y.p, (size_t)sett.Ninterp, y2.p,
-                       (size_t)(nspl + 1), shftf.p, sett.N, sett.interpftpad, p, xm.p);
+    hipLaunchKernelGGL((k_spline_u<T>), gs, dim3(NT), 0, stream, Y.p, (size_t)S.Ninterp, g.nspl, spl.p, u.p,
+                       (size_t)(g.nspl + 1));
+    hipLaunchKernelGGL(k_spline_y2, gs, dim3(NT), 0, stream, u.p, (size_t)(g.nspl + 1), g.nspl, spl.p, y2.p);
+    dim3 ge((S.N + NT - 1) / NT, nb);
+    hipLaunchKernelGGL((k_spline_eval<T, T>), ge, dim3(NT), 0, stream, Y.p, (size_t)S.Ninterp, y2.p,
+                       (size_t)(g.nspl + 1), shftf.p, S.N, S.interpftpad, p, slot_xm_ptr(slot));
     if (hipGetLastError() != hipSuccess) status = PGW_ERR_DEVICE;
     return 1;
   }
 
-  void spindown_range(int mm, int nn, const pgw_range* r, const pgw_opts* o, int& smin, int& smax) {
+  void spindown_range(const Seg& g, int mm, int nn, const pgw_range* r, const pgw_opts* o, int& smin, int& smax) {
 #pragma clang fp contract(off)
-    const double* M = sett.M;
+    const double* M = g.sett.M;
     smin = r->sst;
     smax = r->spndr[1];
     if (!o->use_range) {  // jobcore.c:424-439
-      smin = (int)trunc((sett.Smin - nn * M[9] - mm * M[13]) / M[5]);
-      smax = (int)trunc(-(nn * M[9] + mm * M[13] + sett.Smax) / M[5]);
+      smin = (int)trunc((g.sett.Smin - nn * M[9] - mm * M[13]) / M[5]);
+      smax = (int)trunc(-(nn * M[9] + mm * M[13] + g.sett.Smax) / M[5]);
       if (smin > smax) std::swap(smin, smax);
     }
     if (o->s0_flag) smin = smax;
   }
 
-  // launches the template stage for spindowns [s0, s0+nt) into candidate buffer buf;
-  // ev (optional) = 4 events bracketing the kernels
-  int launch_templates(int mm, int nn, int s0, int nt, const pgw_opts* o, int buf = 0, hipEvent_t* ev = nullptr) {
-    const double* M = sett.M;
-    SpinParams sp;
-    sp.M0 = M[0]; sp.M4 = M[4]; sp.M5 = M[5]; sp.M9 = M[9]; sp.M13 = M[13];
-    sp.nn = nn; sp.mm = mm; sp.ss0 = s0; sp.s0_flag = o->s0_flag;
-    sp.N = sett.N; sp.L1 = ps.L1; sp.nifo = nifo; sp.twshift = ps.twshift;
+  // het1 and sgnlt[1] of spindown ss (jobcore.c:456,466-467)
+  static void tmpl_params(const double* M, int mm, int nn, int ss, int s0_flag, double& het1, double& sgnlt1) {
+#pragma clang fp contract(off)
+    sgnlt1 = (s0_flag ? 0. : ss * M[5] + nn * M[9] + mm * M[13]);
+    het1 = fmod(ss * M[4], M[0]);
+    if (het1 < 0) het1 += M[0];
+  }
+
+  // descriptors of item `it`'s templates into host buffer `h` (index t0..)
+  void fill_desc(const Item& it, const pgw_opts* o, TmplDesc* h) {
+    const Seg& g = *segs[it.seg];
+    for (int ss = it.smin; ss <= it.smax; ss++) {
+      TmplDesc& d = h[it.t0 + (ss - it.smin)];
+      d.slot = it.slot;
+      tmpl_params(g.sett.M, it.mm, it.nn, ss, o->s0_flag, d.het1, d.sgnlt1);
+      d.N = g.sett.N;
+      d.nifo = nifo;
+      d.pad = 0;
+    }
+  }
+
+  // template stage for descriptors [t0, t0 + nt) of buffer buf; ev (optional) = 4 events around the kernels
+  int launch_templates(const Seg& g, int buf, int t0, int nt, const pgw_opts* o, hipEvent_t* ev = nullptr) {
+    const TmplDesc* d = desc.p + (size_t)buf * kDescCap + t0;
     if (ev) HIPCHK(hipEventRecord(ev[0], stream));
-    launch_spin_col<T>(ps, sp, xm.p, shft.p, Z.p, nt, hi.p, lo.p, col_seq, stream);
+    const SlotBases sb{xm_slots.p, shft_slots.p, aabb_slots.p, slot_xm, slot_shft};
+    launch_spin_col<T>(*g.ps, sb, d, Z.p, nt, hi.p, lo.p, col_seq, stream);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));
-    RowParams rp{ps.L2, sett.nmin, sett.nmax, nblk, nparts};
-    launch_spin_row<T>(ps, rp, Z.p, aabb.p, Ft.p, psum.p, pmax.p, nt, row_na, stream);
+    RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts};
+    launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, stream);
     if (ev) HIPCHK(hipEventRecord(ev[2], stream));
-    ThrParams tp{ps.L1, ps.L2, sett.nmin, sett.nmax, nblk, nparts, o->white_flag, s0, kCandCap, o->trl};
-    hipLaunchKernelGGL((k_threshold<T>), dim3(nparts, nt), dim3(64), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p,
+    ThrParams tp{g.ps->L1, g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, o->white_flag, t0, kCandCap, o->trl};
+    hipLaunchKernelGGL((k_threshold<T>), dim3(g.nparts, nt), dim3(64), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p,
                        cand.p + (size_t)buf * kCandCap, ncand.p + buf);
     if (ev) HIPCHK(hipEventRecord(ev[3], stream));
     return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
   }
 
-  // candidate records in the reference's order and format (jobcore.c:659-695)
-  int emit(const SkyState& sk, int mm, int nn, const pgw_opts* o, int nc, int* sgnlc, double* sgnlv, int cap) {
-#pragma clang fp contract(off)
-    const double* M = sett.M;
-    std::sort(hcand.begin(), hcand.begin() + nc,
-              [](const Cand& a, const Cand& b) { return a.ss != b.ss ? a.ss < b.ss : a.ii < b.ii; });
-    std::vector<double> rec;
-    rec.reserve((size_t)nc * PGW_NPAR);
-    for (int c = 0; c < nc; c++) {
-      const int ss = hcand[c].ss, ii = hcand[c].ii;
-      const double Fc = hcand[c].F;
-      const double sgnlt1 = (o->s0_flag ? 0. : ss * M[5] + nn * M[9] + mm * M[13]);
-      double het1 = fmod(ss * M[4], M[0]);
-      if (het1 < 0) het1 += M[0];
-      const double sgnl0 = sk.het0 + het1;
-      double r[PGW_NPAR];
-      r[0] = 2. * M_PI * ii / ((double)sett.fftpad * sett.nfft) + sgnl0;
-      r[1] = sgnlt1;
-      r[2] = sk.sgnlt2;
-      r[3] = sk.sgnlt3;
-      r[4] = sqrt(2. * (Fc - sett.nd));
-      bool veto = false;
-      for (int k = 0; k < sett.numlines_band; k++)
-        if (r[0] >= sett.lines[k][0] && r[0] <= sett.lines[k][1]) { veto = true; break; }
-      if (!veto) rec.insert(rec.end(), r, r + PGW_NPAR);
+  bool compatible(const Seg& a, const Seg& b) const {
+    return a.sett.nfftf == b.sett.nfftf && a.sett.nmin == b.sett.nmin && a.sett.nmax == b.sett.nmax &&
+           a.sett.fftpad == b.sett.fftpad && a.sett.nfft == b.sett.nfft;
+  }
+
+  // Queues a group: sky stages into slots, descriptors, template launches, async candidate copy.
+  int launch_group(Group& gr, const pgw_opts* o) {
+    int st = PGW_OK;
+    for (Item& it : gr.items) {
+      SkyState sk;
+      if (!sky_stage(*segs[it.seg], it.pm, it.mm, it.nn, it.slot, sk, st) || st) return st ? st : PGW_ERR_DEVICE;
+      it.sk = sk;
     }
-    const int nrec = (int)(rec.size() / PGW_NPAR);
-    if (*sgnlc + nrec > cap) {
-      *sgnlc += nrec;
-      return PGW_ERR_CAPACITY;
-    }
-    memcpy(sgnlv + (size_t)PGW_NPAR * (*sgnlc), rec.data(), sizeof(double) * rec.size());
-    *sgnlc += nrec;
+    TmplDesc* hd = desc_pin.p + (size_t)gr.buf * kDescCap;
+    for (const Item& it : gr.items) fill_desc(it, o, hd);
+    HIPCHK(hipMemcpyAsync(desc.p + (size_t)gr.buf * kDescCap, hd, sizeof(TmplDesc) * gr.ntempl, hipMemcpyHostToDevice,
+                          stream));
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p + gr.buf, 1);
+    const Seg& g0 = *segs[gr.items[0].seg];
+    for (int t0 = 0; t0 < gr.ntempl; t0 += TB)
+      if ((st = launch_templates(g0, gr.buf, t0, std::min(TB, gr.ntempl - t0), o))) return st;
+    HIPCHK(hipMemcpyAsync(hn_pin.p + gr.buf, ncand.p + gr.buf, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(hc_pin.p + (size_t)gr.buf * kCandChunk, cand.p + (size_t)gr.buf * kCandCap,
+                          sizeof(Cand) * kCandChunk, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipEventRecord(evd[gr.buf], stream));
     return PGW_OK;
   }
 
-  struct Pending {
-    SkyState sk;
-    int mm, nn, buf;
-  };
-
-  // Queues all device work of one sky point into candidate buffer buf, followed by the async copy
-  // of the candidate count and the first kCandChunk candidates; returns 0 outside the disk.
-  int launch_sky(int pm, int mm, int nn, pgw_range* r, const pgw_opts* o, int buf, SkyState& sk, int* FNum,
-                 int& status) {
-    if (!sky_stage(pm, mm, nn, sk, status)) return 0;
-    if (status) return 1;
-    int smin, smax;
-    spindown_range(mm, nn, r, o, smin, smax);
-    status = PGW_ERR_DEVICE;
-    if (hipMemsetAsync(ncand.p + buf, 0, sizeof(int), stream) != hipSuccess) return 1;
-    for (int s0 = smin; s0 <= smax; s0 += TB) {
-      const int nt = std::min(TB, smax - s0 + 1);
-      if ((status = launch_templates(mm, nn, s0, nt, o, buf))) return 1;
-    }
-    status = PGW_ERR_DEVICE;
-    if (hipMemcpyAsync(hn_pin + buf, ncand.p + buf, sizeof(int), hipMemcpyDeviceToHost, stream) != hipSuccess ||
-        hipMemcpyAsync(hc_pin + (size_t)buf * kCandChunk, cand.p + (size_t)buf * kCandCap, sizeof(Cand) * kCandChunk,
-                       hipMemcpyDeviceToHost, stream) != hipSuccess ||
-        hipEventRecord(evd[buf], stream) != hipSuccess)
-      return 1;
-    status = PGW_OK;
-    if (smax >= smin) *FNum += smax - smin + 1;
-    return 1;
-  }
-
-  // Waits for a queued sky point's candidates and appends its records.
-  int finish(const Pending& pd, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap) {
-    HIPCHK(hipEventSynchronize(evd[pd.buf]));
-    const int nc = hn_pin[pd.buf];
+  // Waits for a group's candidates and appends its records (jobcore.c:659-695) in (sky point, ss, bin) order.
+  int finish_group(const Group& gr, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap, int* rec_work) {
+#pragma clang fp contract(off)
+    HIPCHK(hipEventSynchronize(evd[gr.buf]));
+    const int nc = hn_pin.p[gr.buf];
     if (nc > kCandCap) return PGW_ERR_CAPACITY;
-    memcpy(hcand.data(), hc_pin + (size_t)pd.buf * kCandChunk, sizeof(Cand) * std::min(nc, kCandChunk));
-    if (nc > kCandChunk) {  // rare: the rest synchronously (buffer pd.buf is not reused before this returns)
-      HIPCHK(hipMemcpyAsync(hcand.data() + kCandChunk, cand.p + (size_t)pd.buf * kCandCap + kCandChunk,
+    memcpy(hcand.data(), hc_pin.p + (size_t)gr.buf * kCandChunk, sizeof(Cand) * std::min(nc, kCandChunk));
+    if (nc > kCandChunk) {  // rare: the rest synchronously (the buffer is not reused before this returns)
+      HIPCHK(hipMemcpyAsync(hcand.data() + kCandChunk, cand.p + (size_t)gr.buf * kCandCap + kCandChunk,
                             sizeof(Cand) * (nc - kCandChunk), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
     }
-    return emit(pd.sk, pd.mm, pd.nn, o, nc, sgnlc, sgnlv, cap);
+    std::sort(hcand.begin(), hcand.begin() + nc,
+              [](const Cand& a, const Cand& b) { return a.t != b.t ? a.t < b.t : a.ii < b.ii; });
+    int st = PGW_OK;
+    size_t k = 0;
+    for (int c = 0; c < nc; c++) {
+      while (k + 1 < gr.items.size() && hcand[c].t >= gr.items[k + 1].t0) k++;
+      const Item& it = gr.items[k];
+      const Seg& g = *segs[it.seg];
+      const pgw_settings& S = g.sett;
+      const double* M = S.M;
+      const int ss = it.smin + (hcand[c].t - it.t0), ii = hcand[c].ii;
+      double het1, sgnlt1;
+      tmpl_params(M, it.mm, it.nn, ss, o->s0_flag, het1, sgnlt1);
+      const double sgnl0 = it.sk.het0 + het1;
+      double r[PGW_NPAR];
+      r[0] = 2. * M_PI * ii / ((double)S.fftpad * S.nfft) + sgnl0;
+      r[1] = sgnlt1;
+      r[2] = it.sk.sgnlt2;
+      r[3] = it.sk.sgnlt3;
+      r[4] = sqrt(2. * (hcand[c].F - S.nd));
+      bool veto = false;
+      for (int l = 0; l < S.numlines_band; l++)
+        if (r[0] >= S.lines[l][0] && r[0] <= S.lines[l][1]) { veto = true; break; }
+      if (veto) continue;
+      if (*sgnlc >= cap) {
+        st = PGW_ERR_CAPACITY;
+        (*sgnlc)++;
+        continue;
+      }
+      memcpy(sgnlv + (size_t)PGW_NPAR * (*sgnlc), r, sizeof(r));
+      if (rec_work) rec_work[*sgnlc] = it.work;
+      (*sgnlc)++;
+    }
+    return st;
   }
 
-  // Sky points in order; the device works on point i while the host turns point i-1 into records.
-  int run_batch(const int* sky, int nsky, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap,
-                int* FNum) {
+  // Work items (seg, pm, mm, nn) in order, grouped; the host turns group g-1 into records while
+  // the device works on group g.
+  int run(const int* work, int n, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap, int* rec_work,
+          int* FNum) override {
     HIPCHK(hipEventRecord(ev0, stream));
-    Pending prev{};
+    Group prev;
     bool have_prev = false;
-    int buf = 0, st = PGW_OK;
-    for (int i = 0; i < nsky && !st; i++) {
-      Pending cur{};
-      cur.mm = sky[3 * i + 1];
-      cur.nn = sky[3 * i + 2];
-      cur.buf = buf;
-      if (!launch_sky(sky[3 * i], cur.mm, cur.nn, r, o, buf, cur.sk, FNum, st)) continue;
-      if (st) break;
-      if (have_prev) st = finish(prev, o, sgnlc, sgnlv, cap);
-      prev = cur;
+    int buf = 0, st = PGW_OK, i = 0;
+    while (i < n && !st) {
+      Group gr;
+      gr.buf = buf;
+      const Seg* g0 = nullptr;
+      while (i < n && (int)gr.items.size() < kSlots) {
+        const int sgi = work[4 * i];
+        if (sgi < 0 || sgi >= (int)segs.size()) { st = PGW_ERR_ARG; break; }
+        const Seg& g = *segs[sgi];
+        Item it;
+        it.work = i;
+        it.seg = sgi;
+        it.pm = work[4 * i + 1];
+        it.mm = work[4 * i + 2];
+        it.nn = work[4 * i + 3];
+        double a, b, c, d;
+        if (!sky_coords(g, it.pm, it.mm, it.nn, it.sk, a, b, c, d)) { i++; continue; }  // outside the disk
+        spindown_range(g, it.mm, it.nn, r, o, it.smin, it.smax);
+        const int ns = std::max(0, it.smax - it.smin + 1);
+        if (ns > kDescCap) { st = PGW_ERR_UNSUPPORTED; break; }
+        if (g0 && (!compatible(*g0, g) || gr.ntempl + ns > kDescCap)) break;
+        if (!g0) g0 = &g;
+        if (ns == 0) { i++; continue; }
+        it.slot = (int)gr.items.size();
+        it.t0 = gr.ntempl;
+        gr.ntempl += ns;
+        gr.items.push_back(it);
+        *FNum += ns;
+        i++;
+      }
+      if (st || gr.items.empty()) continue;
+      if ((st = launch_group(gr, o))) break;
+      if (have_prev) st = finish_group(prev, o, sgnlc, sgnlv, cap, rec_work);
+      prev = std::move(gr);
       have_prev = true;
       buf ^= 1;
     }
     HIPCHK(hipEventRecord(ev1, stream));
-    if (have_prev && !st) st = finish(prev, o, sgnlc, sgnlv, cap);
+    if (have_prev) {
+      const int st2 = finish_group(prev, o, sgnlc, sgnlv, cap, rec_work);
+      if (!st) st = st2;
+    }
     HIPCHK(hipEventSynchronize(ev1));
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, ev0, ev1));
@@ -542,57 +688,76 @@ template <typename T> struct Impl final : pgw_handle {
     return st;
   }
 
-  int job_core(int pm, int mm, int nn, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap,
-               int* FNum) override {
-    const int sky[3] = {pm, mm, nn};
-    return run_batch(sky, 1, r, o, sgnlc, sgnlv, cap, FNum);
-  }
-
-  int job_batch(const int* sky, int nsky, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap,
-                int* FNum) override {
-    return run_batch(sky, nsky, r, o, sgnlc, sgnlv, cap, FNum);
-  }
-
+  // one template (segment 0) through the template stage; F normalised as thresholded
   int debug_fstat(int pm, int mm, int nn, int ss, const pgw_opts* o, double* Fout) override {
-    SkyState sk;
+    const Seg& g = *segs[0];
+    Group gr;
+    Item it{0, 0, pm, mm, nn, 0, ss, ss, 0, SkyState{}};
+    double a, b, c, d;
+    if (!sky_coords(g, pm, mm, nn, it.sk, a, b, c, d)) return 1;
+    gr.items.push_back(it);
+    gr.ntempl = 1;
+    gr.buf = 0;
     int st;
-    if (!sky_stage(pm, mm, nn, sk, st)) return 1;
-    if (st) return st;
-    HIPCHK(hipMemsetAsync(ncand.p, 0, sizeof(int), stream));
-    if ((st = launch_templates(mm, nn, ss, 1, o))) return st;
-    const size_t L = sett.nfftf;
+    if ((st = launch_group(gr, o))) return st;
+    const size_t L = g.sett.nfftf;
     std::vector<T> hF(L);
-    std::vector<double> hmu(nparts);
+    std::vector<double> hmu(g.nparts);
     HIPCHK(hipMemcpyAsync(hF.data(), Ft.p, sizeof(T) * L, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(hmu.data(), mu.p, sizeof(double) * nparts, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(hmu.data(), mu.p, sizeof(double) * g.nparts, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    for (int k = sett.nmin; k < sett.nmax; k++) {
-      const int part = std::min((k - sett.nmin) / PGW_NAV, nparts - 1);
-      Fout[k - sett.nmin] = (double)hF[(size_t)(k % ps.L2) * ps.L1 + k / ps.L2] / hmu[part];
+    for (int k = g.sett.nmin; k < g.sett.nmax; k++) {
+      const int part = std::min((k - g.sett.nmin) / PGW_NAV, g.nparts - 1);
+      Fout[k - g.sett.nmin] = (double)hF[(size_t)(k % g.ps->L2) * g.ps->L1 + k / g.ps->L2] / hmu[part];
     }
     return PGW_OK;
   }
 
-  int time_stages(int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps, double* ms) override {
-    if (nt < 1 || reps < 1) return PGW_ERR_ARG;
-    nt = std::min(nt, TB);  // at most one batch
+  int debug_resampled(int pm, int mm, int nn, int det, double* a, double* b) override {
+    if (det < 0 || det >= nifo) return PGW_ERR_ARG;
+    const Seg& g = *segs[0];
     SkyState sk;
     int st;
-    hipEvent_t ev[4];
+    if (!sky_stage(g, pm, mm, nn, 0, sk, st)) return 1;
+    if (st) return st;
+    const int N = g.sett.N;
+    std::vector<T2> h((size_t)2 * N);
+    HIPCHK(hipMemcpyAsync(h.data(), slot_xm_ptr(0) + (size_t)2 * det * N, sizeof(T2) * 2 * N, hipMemcpyDeviceToHost,
+                          stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int i = 0; i < N; i++) {
+      a[2 * i] = h[i].x;
+      a[2 * i + 1] = h[i].y;
+      b[2 * i] = h[N + i].x;
+      b[2 * i + 1] = h[N + i].y;
+    }
+    return PGW_OK;
+  }
+
+  int time_stages(int sgi, int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps, double* ms) override {
+    if (nt < 1 || reps < 1 || sgi < 0 || sgi >= (int)segs.size()) return PGW_ERR_ARG;
+    nt = std::min(nt, TB);  // at most one launch
+    const Seg& g = *segs[sgi];
+    Item it{0, sgi, pm, mm, nn, 0, s0, s0 + nt - 1, 0, SkyState{}};
+    int st = PGW_OK;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     for (int k = 0; k < 4; k++) HIPCHK(hipEventCreate(&ev[k]));
     HIPCHK(hipEventRecord(ev0, stream));
-    const int inside = sky_stage(pm, mm, nn, sk, st);
+    const int inside = sky_stage(g, pm, mm, nn, 0, it.sk, st);
     HIPCHK(hipEventRecord(ev1, stream));
-    if (!inside) st = 1;
+    if (!inside && !st) st = 1;
     double acc[4] = {0, 0, 0, 0};
     if (!st) {
+      HIPCHK(hipStreamSynchronize(stream));  // the pinned descriptor buffer may still feed a copy
+      fill_desc(it, o, desc_pin.p);
+      HIPCHK(hipMemcpyAsync(desc.p, desc_pin.p, sizeof(TmplDesc) * nt, hipMemcpyHostToDevice, stream));
       HIPCHK(hipStreamSynchronize(stream));
       float t = 0.f;
       HIPCHK(hipEventElapsedTime(&t, ev0, ev1));
       acc[0] = t;
-      for (int r = 0; r < reps && !st; r++) {
-        HIPCHK(hipMemsetAsync(ncand.p, 0, sizeof(int), stream));
-        st = launch_templates(mm, nn, s0, nt, o, 0, ev);
+      for (int rr = 0; rr < reps && !st; rr++) {
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p, 1);
+        st = launch_templates(g, 0, 0, nt, o, ev);
         HIPCHK(hipStreamSynchronize(stream));
         for (int k = 0; k < 3; k++) {
           HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
@@ -607,23 +772,54 @@ template <typename T> struct Impl final : pgw_handle {
     return st;
   }
 
-  int debug_resampled(int pm, int mm, int nn, int det, double* a, double* b) override {
-    if (det < 0 || det >= nifo) return PGW_ERR_ARG;
-    SkyState sk;
-    int st;
-    if (!sky_stage(pm, mm, nn, sk, st)) return 1;
-    if (st) return st;
-    const int N = sett.N;
-    std::vector<T2> h((size_t)2 * N);
-    HIPCHK(hipMemcpyAsync(h.data(), xm.p + (size_t)2 * det * N, sizeof(T2) * 2 * N, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    for (int i = 0; i < N; i++) {
-      a[2 * i] = h[i].x; a[2 * i + 1] = h[i].y;
-      b[2 * i] = h[N + i].x; b[2 * i + 1] = h[N + i].y;
+  // Times the template stage per template for each (TB, row pass, column pass) variant on the
+  // given sky point and keeps the fastest.
+  int autotune(int sgi, int pm, int mm, int nn, const pgw_opts* o, int* tb_out, int* na_out, int* seq_out) override {
+    const int tbs[3] = {64, 128, 256};
+    double best = 1e30;
+    int btb = TB, bna = row_na;
+    bool bseq = col_seq;
+    for (int tb : tbs) {
+      int st = ensure_tb(tb);
+      if (st) return st;
+      for (int na = 1; na <= 2; na++)
+        for (int seq = 0; seq <= 1; seq++) {
+          TB = tb;
+          row_na = na;
+          col_seq = seq != 0;
+          double ms[5];
+          if ((st = time_stages(sgi, pm, mm, nn, 0, tb, o, 3, ms))) return st;
+          const double per = (ms[1] + ms[2] + ms[3]) / ms[4];
+          if (per < best) {
+            best = per;
+            btb = tb;
+            bna = na;
+            bseq = seq != 0;
+          }
+        }
     }
+    TB = btb;
+    row_na = bna;
+    col_seq = bseq;
+    if (tb_out) *tb_out = TB;
+    if (na_out) *na_out = row_na;
+    if (seq_out) *seq_out = col_seq;
     return PGW_OK;
   }
 };
+
+template <typename T>
+int make_engine(pgw_handle** h, const pgw_settings* s, const pgw_detector* ifo, int nseg, int nifo) {
+  auto* q = new (std::nothrow) Engine<T>();
+  if (!q) return PGW_ERR_NOMEM;
+  const int st = q->init(s, ifo, nseg, nifo);
+  if (st) {
+    delete q;
+    return st;
+  }
+  *h = q;
+  return PGW_OK;
+}
 
 }  // namespace
 
@@ -687,45 +883,54 @@ double pgw_sig2(const double* x, int n) {
   return crf0 * var;
 }
 
-int pgw_init(pgw_handle** h, const pgw_settings* sett, const pgw_detector* ifo, int nifo, int precision) {
-  if (!h || !sett || !ifo || nifo < 1 || nifo > PGW_MAX_DETECTORS) return PGW_ERR_ARG;
+int pgw_init_multi(pgw_handle** h, const pgw_settings* sett, const pgw_detector* ifo, int nseg, int nifo,
+                   int precision) {
+  if (!h || !sett || !ifo || nseg < 1 || nifo < 1 || nifo > PGW_MAX_DETECTORS) return PGW_ERR_ARG;
   *h = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PGW_ERR_NODEVICE;
-  pgw_handle* p = nullptr;
-  int st;
-  if (precision == PGW_PREC_FP64) {
-    auto* q = new (std::nothrow) Impl<double>();
-    if (!q) return PGW_ERR_NOMEM;
-    st = q->init(sett, ifo, nifo);
-    p = q;
-  } else if (precision == PGW_PREC_FP32) {
-    auto* q = new (std::nothrow) Impl<float>();
-    if (!q) return PGW_ERR_NOMEM;
-    st = q->init(sett, ifo, nifo);
-    p = q;
-  } else {
-    return PGW_ERR_ARG;
-  }
-  if (st) {
-    delete p;
-    return st;
-  }
-  *h = p;
-  return PGW_OK;
+  if (precision == PGW_PREC_FP64) return make_engine<double>(h, sett, ifo, nseg, nifo);
+  if (precision == PGW_PREC_FP32) return make_engine<float>(h, sett, ifo, nseg, nifo);
+  return PGW_ERR_ARG;
+}
+
+int pgw_init(pgw_handle** h, const pgw_settings* sett, const pgw_detector* ifo, int nifo, int precision) {
+  return pgw_init_multi(h, sett, ifo, 1, nifo, precision);
 }
 
 int pgw_job_core(pgw_handle* h, int pm, int mm, int nn, pgw_range* s_range, const pgw_opts* opts, int* sgnlc,
                  double* sgnlv, int sgnlv_cap, int* FNum) {
   if (!h || !s_range || !opts || !sgnlc || !FNum || (sgnlv_cap > 0 && !sgnlv)) return PGW_ERR_ARG;
-  return h->job_core(pm, mm, nn, s_range, opts, sgnlc, sgnlv, sgnlv_cap, FNum);
+  const int work[4] = {0, pm, mm, nn};
+  return h->run(work, 1, s_range, opts, sgnlc, sgnlv, sgnlv_cap, nullptr, FNum);
 }
 
 int pgw_job_batch(pgw_handle* h, const int* sky, int nsky, pgw_range* s_range, const pgw_opts* opts, int* sgnlc,
                   double* sgnlv, int sgnlv_cap, int* FNum) {
   if (!h || (!sky && nsky > 0) || nsky < 0 || !s_range || !opts || !sgnlc || !FNum || (sgnlv_cap > 0 && !sgnlv))
     return PGW_ERR_ARG;
-  return h->job_batch(sky, nsky, s_range, opts, sgnlc, sgnlv, sgnlv_cap, FNum);
+  std::vector<int> work((size_t)4 * nsky);
+  for (int i = 0; i < nsky; i++) {
+    work[4 * i] = 0;
+    work[4 * i + 1] = sky[3 * i];
+    work[4 * i + 2] = sky[3 * i + 1];
+    work[4 * i + 3] = sky[3 * i + 2];
+  }
+  return h->run(work.data(), nsky, s_range, opts, sgnlc, sgnlv, sgnlv_cap, nullptr, FNum);
+}
+
+int pgw_job_batch_multi(pgw_handle* h, const int* work, int nwork, pgw_range* s_range, const pgw_opts* opts,
+                        int* sgnlc, double* sgnlv, int sgnlv_cap, int* rec_work, int* FNum) {
+  if (!h || (!work && nwork > 0) || nwork < 0 || !s_range || !opts || !sgnlc || !FNum ||
+      (sgnlv_cap > 0 && !sgnlv))
+    return PGW_ERR_ARG;
+  return h->run(work, nwork, s_range, opts, sgnlc, sgnlv, sgnlv_cap, rec_work, FNum);
+}
+
+int pgw_autotune(pgw_handle* h, int seg, int pm, int mm, int nn, const pgw_opts* opts, int* tb, int* row_na,
+                 int* col_seq) {
+  if (!h || !opts) return PGW_ERR_ARG;
+  return h->autotune(seg, pm, mm, nn, opts, tb, row_na, col_seq);
 }
 
 int pgw_debug_fstat(pgw_handle* h, int pm, int mm, int nn, int ss, const pgw_opts* opts, double* F) {
@@ -741,16 +946,19 @@ int pgw_debug_resampled(pgw_handle* h, int pm, int mm, int nn, int det, double* 
 int pgw_time_stages(pgw_handle* h, int pm, int mm, int nn, int s0, int nt, const pgw_opts* opts, int reps,
                     double ms[5]) {
   if (!h || !opts || !ms) return PGW_ERR_ARG;
-  return h->time_stages(pm, mm, nn, s0, nt, opts, reps, ms);
+  return h->time_stages(0, pm, mm, nn, s0, nt, opts, reps, ms);
 }
 
 int pgw_copy_bench(size_t bytes, int reps, double* ms) {
-  if (!ms || bytes < 16 || reps < 1) return PGW_ERR_ARG;
+  if (!ms || bytes < 32 || reps < 1) return PGW_ERR_ARG;
   const size_t n = bytes / sizeof(double4);
   double4 *a = nullptr, *b = nullptr;
   hipEvent_t e0, e1;
   HIPCHK(hipMalloc(&a, n * sizeof(double4)));
-  if (hipMalloc(&b, n * sizeof(double4)) != hipSuccess) { (void)hipFree(a); return PGW_ERR_NOMEM; }
+  if (hipMalloc(&b, n * sizeof(double4)) != hipSuccess) {
+    (void)hipFree(a);
+    return PGW_ERR_NOMEM;
+  }
   HIPCHK(hipMemset(a, 0, n * sizeof(double4)));
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));

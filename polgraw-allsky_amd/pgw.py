@@ -42,6 +42,12 @@ def lib():
                                     _ip, _dp, C.c_int, _ip]
         L.pgw_debug_fstat.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts), _dp]
         L.pgw_debug_resampled.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.pgw_init_multi.argtypes = [C.POINTER(C.c_void_p), C.POINTER(abi.Settings), C.POINTER(abi.Detector),
+                                     C.c_int, C.c_int, C.c_int]
+        L.pgw_job_batch_multi.argtypes = [C.c_void_p, _ip, C.c_int, C.POINTER(abi.Range), C.POINTER(abi.Opts),
+                                          _ip, _dp, C.c_int, _ip, _ip]
+        L.pgw_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts),
+                                   _ip, _ip, _ip]
         L.pgw_time_stages.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.POINTER(abi.Opts), C.c_int, _dp]
         L.pgw_copy_bench.argtypes = [C.c_size_t, C.c_int, _dp]
@@ -55,6 +61,7 @@ def lib():
 
 
 EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_job_core", "pgw_job_batch",
+            "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune",
             "pgw_debug_fstat", "pgw_debug_resampled", "pgw_time_stages", "pgw_copy_bench", "pgw_last_gpu_ms",
             "pgw_destroy", "pgw_strerror"]
 
@@ -72,12 +79,18 @@ class Handle:
     """One segment resident on the GPU (pgw_init ... pgw_destroy)."""
 
     def __init__(self, seg, precision: int = abi.PREC_FP64, lines=None):
-        self.seg = seg
+        segs = seg if isinstance(seg, (list, tuple)) else [seg]
+        self.seg = segs[0]
+        self.segs = segs
         self.precision = precision
-        self.sett = abi.make_settings(seg, lines)
-        self.dets = abi.make_detectors(seg)
+        sett = (abi.Settings * len(segs))()
+        for k, s in enumerate(segs):
+            sett[k] = abi.make_settings(s, lines)
+        self._setts = sett
+        self.sett = sett[0]
+        self.dets = abi.make_detectors_multi(segs)
         self.h = C.c_void_p()
-        st = lib().pgw_init(C.byref(self.h), C.byref(self.sett), self.dets, len(seg.dets), precision)
+        st = lib().pgw_init_multi(C.byref(self.h), sett, self.dets, len(segs), len(segs[0].dets), precision)
         if st != 0:
             raise PGWError(st, "pgw_init")
 
@@ -110,6 +123,27 @@ class Handle:
         if st != 0:
             raise PGWError(st, "pgw_job_batch")
         return sgnlv[:abi.NPAR * sgnlc.value].reshape(-1, abi.NPAR), fnum.value
+
+    def job_batch_multi(self, work, rng, opts, cap=1 << 18):
+        """work rows (segment, pm, mm, nn); returns (records, work index per record, templates)."""
+        work = np.ascontiguousarray(np.asarray(work, dtype=np.int32).reshape(-1, 4))
+        sgnlv = np.zeros(abi.NPAR * cap)
+        owner = np.zeros(cap, np.int32)
+        sgnlc, fnum = C.c_int(0), C.c_int(0)
+        st = lib().pgw_job_batch_multi(self.h, work.ctypes.data_as(_ip), len(work), C.byref(rng), C.byref(opts),
+                                       C.byref(sgnlc), sgnlv.ctypes.data_as(_dp), cap,
+                                       owner.ctypes.data_as(_ip), C.byref(fnum))
+        if st != 0:
+            raise PGWError(st, "pgw_job_batch_multi")
+        n = sgnlc.value
+        return sgnlv[:abi.NPAR * n].reshape(-1, abi.NPAR), owner[:n].copy(), fnum.value
+
+    def autotune(self, seg, pm, mm, nn, opts):
+        tb, na, seq = C.c_int(), C.c_int(), C.c_int()
+        st = lib().pgw_autotune(self.h, seg, pm, mm, nn, C.byref(opts), C.byref(tb), C.byref(na), C.byref(seq))
+        if st != 0:
+            raise PGWError(st, "pgw_autotune")
+        return dict(tb=tb.value, row_na=na.value, col_seq=seq.value)
 
     def fstat(self, pm, mm, nn, ss, opts):
         F = np.zeros(self.sett.nmax - self.sett.nmin)

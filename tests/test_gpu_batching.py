@@ -1,0 +1,74 @@
+"""Batching form: segments of uneven length in one handle, templates of many sky points per
+launch, and the autotuner; every record checked against the CPU oracle run per segment."""
+import numpy as np
+import pytest
+
+from polgraw_allsky_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pgw():
+    from polgraw_allsky_amd import pgw as p
+    p.lib()
+    return p
+
+
+def _segments():
+    # three detectors, gaps 5-30 %, N spanning both nfft = 2^16 and 2^17
+    out = []
+    for k, (N, gap) in enumerate([(60000, 0.05), (86400, 0.3), (64000, 0.15), (70000, 0.1)]):
+        out.append(synth.make_segment(N=N, dets=("H1", "L1", "V1"), sigmas=(1.0, 1.3, 1.1), fftpad=1,
+                                      seed=100 + k, h0=0.15, gap_frac=gap, nsky=(2, 2), nspin=6))
+    return out
+
+
+def test_multi_segment_batch_equals_oracle(pgw, abi, orc):
+    segs = _segments()
+    assert {s.sett["nfft"] for s in segs} == {65536, 131072}
+    opts = abi.make_opts(trl=15.0)
+    h = pgw.Handle(segs, abi.PREC_FP64)
+    work, expect, fn_r = [], [], 0
+    for k, seg in enumerate(segs):
+        for pm, mm, nn in synth.sky_list(seg):
+            work.append((k, pm, mm, nn))
+    # interleave segments so groups have to split on nfft changes
+    work = work[::2] + work[1::2]
+    # one shared range (the spindown window of segment 0) for all items
+    rng = abi.make_range(segs[0])
+    oracles = [orc.Search(s, nthreads=8) for s in segs]
+    for i, (k, pm, mm, nn) in enumerate(work):
+        r, f = oracles[k].job_core(pm, mm, nn, rng, opts)
+        expect.append((i, r))
+        fn_r += f
+    got, owner, fn_g = h.job_batch_multi(work, rng, opts)
+    assert fn_g == fn_r
+    ref = np.concatenate([r for _, r in expect])
+    ref_owner = np.concatenate([np.full(len(r), i) for i, r in expect])
+    print("work items", len(work), "templates", fn_r, "records", len(ref), flush=True)
+    assert got.shape == ref.shape
+    assert np.array_equal(owner, ref_owner)
+    assert np.array_equal(got[:, :4], ref[:, :4])
+    Fg, Fr = got[:, 4] ** 2 / 2 + 2, ref[:, 4] ** 2 / 2 + 2
+    assert np.max(np.abs(Fg - Fr) / Fr) <= 1e-9
+
+
+def test_autotune_keeps_results(pgw, abi):
+    from cases import injected_sky, segment
+    seg = segment("net-2det-32k")
+    rng = abi.make_range(seg)
+    opts = abi.make_opts(trl=12.0)
+    h = pgw.Handle(seg, abi.PREC_FP32)
+    sky = synth.sky_list(seg)
+    before, fa = h.job_batch(sky, rng, opts)
+    choice = h.autotune(0, *injected_sky(seg), opts)
+    print("autotune", choice, flush=True)
+    assert choice["tb"] in (64, 128, 256) and choice["row_na"] in (1, 2) and choice["col_seq"] in (0, 1)
+    after, fb = h.job_batch(sky, rng, opts)
+    # the variants compute the same sums; the compiler's FMA contraction differs between them, so
+    # FP32 values move by rounding only: same templates and bins, F within 1e-6 relative
+    assert fa == fb and before.shape == after.shape
+    assert np.array_equal(before[:, :4], after[:, :4])
+    Fa, Fb = before[:, 4] ** 2 / 2 + 2, after[:, 4] ** 2 / 2 + 2
+    assert np.max(np.abs(Fa - Fb) / Fa) <= 1e-6
