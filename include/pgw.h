@@ -144,6 +144,16 @@ int pgw_job_batch_multi(pgw_handle *h, const int *work, int nwork, pgw_range *s_
                         const pgw_opts *opts, int *sgnlc, double *sgnlv, int sgnlv_cap, int *rec_work,
                         int *FNum);
 
+/* Exact network F-statistic at continuous parameters: the followup's Fstatnet
+ * (followup/src/followup.c:413-534, detector responses from modvir at the point as
+ * followup.c:676-685 builds them) for npts points pts[4*i..4*i+3] = (omega [rad/sample],
+ * omegadot [rad/sample^2], delta, alpha [rad]) over segment `seg`.  out[11*i..] receives the
+ * reference's 11 values: amplitude estimates [0..3], sqrt(2(F-2)) [4], -F [5], the point [6..9],
+ * amplitude SNR [10].  grad (optional, 4*npts) receives dF/d(omega, omegadot, delta, alpha) of
+ * F = -out[5] (the analytic gradient; no counterpart in the reference, whose followup searches with
+ * Nelder-Mead and MADS, followup.c:537-1173).  FP64 on the device. */
+int pgw_fstatnet(pgw_handle *h, int seg, const double *pts, int npts, double *out, double *grad);
+
 /* Autotuner (no counterpart in the reference): times the template stage of sky point
  * (pm, mm, nn) of segment `seg` for every kernel variant (templates per launch 64/128/256,
  * row pass with both spectra together or in turn, column pass LDS sharing) and keeps the

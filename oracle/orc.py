@@ -59,6 +59,8 @@ def lib():
         L.orc_var.restype = C.c_double
         L.orc_var.argtypes = [_dp, C.c_int]
         L.orc_fstat_norm.argtypes = [_dp, C.c_int, C.c_int]
+        L.orc_fstatnet_batch.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp]
+        L.orc_amoeba.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -151,6 +153,26 @@ class Search:
         b = np.zeros(N, np.complex128)
         st = lib().orc_resampled(self.h, pm, mm, nn, det, cptr(a), cptr(b))
         return (a, b) if st == 0 else None
+
+    def fstatnet(self, pts, grad=True):
+        """Fstatnet at points (n, 4) = (omega, omegadot, delta, alpha): (out (n, 11), dF/dpt (n, 4))."""
+        pts = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(-1, 4))
+        out = np.zeros((len(pts), 11))
+        g = np.zeros((len(pts), 4)) if grad else None
+        st = lib().orc_fstatnet_batch(self.h, pts.ctypes.data_as(_dp), len(pts), out.ctypes.data_as(_dp),
+                                      g.ctypes.data_as(_dp) if grad else None)
+        if st != 0:
+            raise RuntimeError("orc_fstatnet failed")
+        return out, g
+
+    def amoeba(self, point, pc2, tol=1e-7, maxeval=20000):
+        """followup's Nelder-Mead from point; returns (best point, its 11 values, evaluations)."""
+        x = np.ascontiguousarray(np.asarray(point, dtype=np.float64).copy())
+        step = np.ascontiguousarray(np.asarray(pc2, dtype=np.float64))
+        out = np.zeros(11)
+        nev = lib().orc_amoeba(self.h, x.ctypes.data_as(_dp), step.ctypes.data_as(_dp), tol, maxeval,
+                               out.ctypes.data_as(_dp))
+        return x, out, nev
 
     def spindown_range(self, mm, nn, rng, opts):
         lo, hi = C.c_int(), C.c_int()

@@ -568,3 +568,193 @@ int orc_resampled(orc_search *s, int pm, int mm, int nn, int det, double complex
   memcpy(xb, s->det[det].xDatmb, sizeof(double complex) * (size_t)s->sett.N);
   return 0;
 }
+
+/* ------------------------------------------------------------ Fstatnet (followup) */
+
+/* modvir at one sample plus the derivatives of a and b with respect to delta and alpha
+ * (followup settings.c:275-318 differentiated: dc/dalpha = -s, ds/dalpha = c,
+ * d cos^2(delta)/d delta = -2 sin cos, d(sin cos)/d delta = cos 2 delta). */
+static void modvir_d(const double c[9], double sindel, double cosdel, double cosalfr, double sinalfr,
+                     double cm, double sm, double *a, double *b, double *da_d, double *da_a,
+                     double *db_d, double *db_a) {
+  const double c2d = SQR(cosdel), c2sd = sindel * cosdel, cos2d = SQR(cosdel) - SQR(sindel);
+  const double cc = cosalfr * cm + sinalfr * sm, ss = sinalfr * cm - cosalfr * sm;
+  const double c2s = 2. * SQR(cc), cs = cc * ss;
+  *a = c[0] * (2. - c2d) * c2s + c[1] * (2. - c2d) * 2. * cs + c[2] * c2sd * cc + c[3] * c2sd * ss -
+       c[0] * (2. - c2d) + c[4] * c2d;
+  *b = c[5] * sindel * c2s + c[6] * sindel * 2. * cs + c[7] * cosdel * cc + c[8] * cosdel * ss -
+       c[5] * sindel;
+  /* d/d delta: c2d' = -2 c2sd, c2sd' = cos 2 delta */
+  *da_d = c[0] * 2. * c2sd * c2s + c[1] * 2. * c2sd * 2. * cs + c[2] * cos2d * cc + c[3] * cos2d * ss -
+          c[0] * 2. * c2sd - 2. * c[4] * c2sd;
+  *db_d = c[5] * cosdel * c2s + c[6] * cosdel * 2. * cs - c[7] * sindel * cc - c[8] * sindel * ss -
+          c[5] * cosdel;
+  /* d/d alpha: c' = -s, s' = c, (c^2)' = -2 c s, (c s)' = c^2 - s^2 */
+  *da_a = c[0] * (2. - c2d) * (-4. * cs) + c[1] * (2. - c2d) * 2. * (SQR(cc) - SQR(ss)) - c[2] * c2sd * ss +
+          c[3] * c2sd * cc;
+  *db_a = c[5] * sindel * (-4. * cs) + c[6] * sindel * 2. * (SQR(cc) - SQR(ss)) - c[7] * cosdel * ss +
+          c[8] * cosdel * cc;
+}
+
+/* Fstatnet, followup.c:413-534: the F-statistic of the whole network at continuous parameters
+ * pt = (omega, omegadot, delta, alpha), with the detector-response arrays from modvir at
+ * (delta, alpha) as the callers build them (followup.c:676-685).  out[11] as the reference:
+ * [0..3] amplitude estimates, [4] sqrt(2(F-2)), [5] -F, [6..9] the point, [10] amplitude SNR.
+ * grad (optional): dF/d(omega, omegadot, delta, alpha) of F = -out[5], accumulated in the same
+ * sample order (analytic; checked against central differences of out[5]).
+ * Sums run per detector in sample order, as the reference's. */
+int orc_fstatnet(const orc_search *s, const double pt[4], double out[11], double grad[4]) {
+  const int N = s->sett.N, nifo = s->nifo;
+  const double oms = s->sett.oms;
+  const double sinalt = sin(pt[3]), cosalt = cos(pt[3]), sindelt = sin(pt[2]), cosdelt = cos(pt[2]);
+  const double nS[3] = {cosalt * cosdelt, sinalt * cosdelt, sindelt};
+  const double nSd[3] = {-cosalt * sindelt, -sinalt * sindelt, cosdelt};  /* d nS / d delta */
+  const double nSa[3] = {-sinalt * cosdelt, cosalt * cosdelt, 0.};         /* d nS / d alpha */
+  double complex *xa = malloc(sizeof(double complex) * (size_t)N);
+  double complex *xb = malloc(sizeof(double complex) * (size_t)N);
+  double *sa = malloc(sizeof(double) * (size_t)N), *sb = malloc(sizeof(double) * (size_t)N);
+  if (!xa || !xb || !sa || !sb) { free(xa); free(xb); free(sa); free(sb); return -1; }
+  double complex xasum = 0, xbsum = 0, dxa[4] = {0, 0, 0, 0}, dxb[4] = {0, 0, 0, 0};
+  double aa = 0., bb = 0., daa[2] = {0, 0}, dbb[2] = {0, 0};
+  for (int n = 0; n < nifo; ++n) {
+    const orc_det *d = &s->det[n];
+    const double cosalfr = cosalt * d->cphir + sinalt * d->sphir;
+    const double sinalfr = sinalt * d->cphir - cosalt * d->sphir;
+    double aatemp = 0., bbtemp = 0., daat[2] = {0, 0}, dbbt[2] = {0, 0};
+    double complex dxat[4] = {0, 0, 0, 0}, dxbt[4] = {0, 0, 0, 0};
+    for (int i = 0; i < N; ++i) {
+      const double *R = d->DetSSB + 3 * (size_t)i;
+      const double shft = nS[0] * R[0] + nS[1] * R[1] + nS[2] * R[2];
+      const double phase = pt[0] * (double)(i + shft) + (pt[1] * i * i) + ((oms + 2. * pt[1] * i) * shft);
+      const double complex exph = cos(phase) - I * sin(phase);
+      double a, b, dad, daa_, dbd, dba;
+      modvir_d(d->amod, sindelt, cosdelt, cosalfr, sinalfr, s->cosmodf[i], s->sinmodf[i], &a, &b, &dad,
+               &daa_, &dbd, &dba);
+      sa[i] = a;
+      sb[i] = b;
+      xa[i] = d->xDat[i] * a * exph;
+      xb[i] = d->xDat[i] * b * exph;
+      if (grad) {
+        /* phase derivatives: i + shft, i^2 + 2 i shft, (omega + oms + 2 omegadot i) dshft */
+        const double K = pt[0] + oms + 2. * pt[1] * i;
+        const double ph[4] = {i + shft, (double)i * i + 2. * i * shft,
+                              K * (nSd[0] * R[0] + nSd[1] * R[1] + nSd[2] * R[2]),
+                              K * (nSa[0] * R[0] + nSa[1] * R[1] + nSa[2] * R[2])};
+        const double dav[4] = {0, 0, dad, daa_}, dbv[4] = {0, 0, dbd, dba};
+        const double complex xe = d->xDat[i] * exph / d->sig2;
+        for (int k = 0; k < 4; k++) {
+          dxat[k] += xe * (dav[k] - I * a * ph[k]);
+          dxbt[k] += xe * (dbv[k] - I * b * ph[k]);
+        }
+        daat[0] += 2. * a * dad; daat[1] += 2. * a * daa_;
+        dbbt[0] += 2. * b * dbd; dbbt[1] += 2. * b * dba;
+      }
+    }
+    for (int i = 0; i < N; i++) {
+      aatemp += SQR(sa[i]);
+      bbtemp += SQR(sb[i]);
+    }
+    for (int i = 0; i < N; ++i) {
+      xa[i] /= d->sig2;
+      xb[i] /= d->sig2;
+      xasum += xa[i];
+      xbsum += xb[i];
+    }
+    aa += aatemp / d->sig2;
+    bb += bbtemp / d->sig2;
+    for (int k = 0; k < 4; k++) { dxa[k] += dxat[k]; dxb[k] += dxbt[k]; }
+    for (int k = 0; k < 2; k++) { daa[k] += daat[k] / d->sig2; dbb[k] += dbbt[k] / d->sig2; }
+  }
+  const double Fa = SQR(creal(xasum)) + SQR(cimag(xasum)), Fb = SQR(creal(xbsum)) + SQR(cimag(xbsum));
+  out[5] = -((Fa / aa) + (Fb / bb));
+  out[0] = 2 * creal(xasum) / aa;
+  out[1] = 2 * creal(xbsum) / bb;
+  out[2] = -2 * cimag(xasum) / aa;
+  out[3] = -2 * cimag(xbsum) / bb;
+  out[4] = sqrt(2 * (-out[5] - 2));
+  for (int k = 0; k < 4; k++) out[6 + k] = pt[k];
+  out[10] = sqrt(SQR(2 * creal(xasum)) + SQR(2 * creal(xbsum)) + SQR(2 * cimag(xasum)) + SQR(2 * cimag(xbsum)));
+  if (grad) {
+    for (int k = 0; k < 4; k++) {
+      const double dA = k >= 2 ? daa[k - 2] : 0., dB = k >= 2 ? dbb[k - 2] : 0.;
+      grad[k] = 2. * creal(conj(xasum) * dxa[k]) / aa - Fa * dA / SQR(aa) + 2. * creal(conj(xbsum) * dxb[k]) / bb -
+                Fb * dB / SQR(bb);
+    }
+  }
+  free(xa); free(xb); free(sa); free(sb);
+  return 0;
+}
+
+int orc_fstatnet_batch(const orc_search *s, const double *pts, int n, double *out, double *grad) {
+  int st = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(s->nthreads) reduction(| : st)
+  for (int p = 0; p < n; p++) st |= orc_fstatnet(s, pts + 4 * (size_t)p, out + 11 * (size_t)p, grad ? grad + 4 * (size_t)p : NULL);
+  return st;
+}
+
+/* Nelder-Mead on -F as followup's amoeba (followup.c:971-1173): initial simplex point + pc2[i] e_i,
+ * reflection (-1), expansion (-2) from the same centroid and line, contraction (0.5), else
+ * shrink toward the best vertex; stop when |f_hi - f_lo| < (|f_hi| + |f_lo|) tol + 1e-14.
+ * point is replaced by the best vertex, out[11] receives its Fstatnet values.  Returns the number
+ * of Fstatnet evaluations, or -1 if maxeval is reached first (the reference has no such limit). */
+/* update_simplex, followup.c:1055-1087: vertex ihi moves to mid + scale line if that lowers -F */
+static int try_vertex(const orc_search *s, double sx[5][4], double fx[5][11], int ihi, const double mid[4],
+                      const double line[4], double scale, int *nev) {
+  double next[4], f2[11];
+  const double fmax = fx[ihi][5];
+  for (int j = 0; j < 4; j++) next[j] = mid[j] + scale * line[j];
+  orc_fstatnet(s, next, f2, NULL);
+  (*nev)++;
+  if (!(f2[5] < fmax)) return 0;
+  memcpy(sx[ihi], next, sizeof next);
+  memcpy(fx[ihi], f2, sizeof f2);
+  return 1;
+}
+
+int orc_amoeba(const orc_search *s, double point[4], const double pc2[4], double tol, int maxeval,
+               double out[11]) {
+  enum { D = 4 };
+  double sx[D + 1][D], fx[D + 1][11], mid[D], line[D];
+  int nev = 0;
+  for (int i = 0; i < D + 1; i++)
+    for (int j = 0; j < D; j++) sx[i][j] = point[j];
+  for (int i = 0; i < D; i++) sx[i][i] += pc2[i];
+  for (int i = 0; i < D + 1; i++, nev++) orc_fstatnet(s, sx[i], fx[i], NULL);
+  int ilo = 0;
+  for (;;) {
+    int ihi, inhi;
+    if (fx[0][5] > fx[1][5]) { ihi = 0; ilo = inhi = 1; }
+    else { ihi = 1; ilo = inhi = 0; }
+    for (int i = 2; i < D + 1; i++) {
+      if (fx[i][5] <= fx[ilo][5]) ilo = i;
+      else if (fx[i][5] > fx[ihi][5]) { inhi = ihi; ihi = i; }
+      else if (fx[i][5] > fx[inhi][5]) inhi = i;
+    }
+    for (int j = 0; j < D; j++) mid[j] = 0.;
+    for (int i = 0; i < D + 1; i++)
+      if (i != ihi)
+        for (int j = 0; j < D; j++) mid[j] += sx[i][j];
+    for (int j = 0; j < D; j++) { mid[j] /= D; line[j] = sx[ihi][j] - mid[j]; }
+    const double fmax = fx[ihi][5], fmin = fx[ilo][5];
+    if (fabs(fmax - fmin) < (fabs(fmax) + fabs(fmin)) * tol + 1e-14) break;
+    if (nev >= maxeval) return -1;
+    /* reflection; then expansion if the reflected vertex is the new best, else a contraction
+     * when it is still the worst, and a shrink if that contraction fails (followup.c:1147-1156) */
+    try_vertex(s, sx, fx, ihi, mid, line, -1.0, &nev);
+    if (fx[ihi][5] < fx[ilo][5]) {
+      try_vertex(s, sx, fx, ihi, mid, line, -2.0, &nev);
+    } else if (fx[ihi][5] > fx[inhi][5]) {
+      if (!try_vertex(s, sx, fx, ihi, mid, line, 0.5, &nev)) {
+        for (int i = 0; i < D + 1; i++) {
+          if (i == ilo) continue;
+          for (int j = 0; j < D; j++) sx[i][j] = (sx[ilo][j] + sx[i][j]) * 0.5;
+          orc_fstatnet(s, sx[i], fx[i], NULL);
+          nev++;
+        }
+      }
+    }
+  }
+  for (int j = 0; j < D; j++) point[j] = sx[ilo][j];
+  memcpy(out, fx[ilo], sizeof(double) * 11);
+  return nev;
+}

@@ -13,7 +13,11 @@
  *     (a^2+b^2 = F+^2 + Fx^2);
  *   job_core composition - resampling against band-limited closed forms,
  *     threshold against a brute-force run scan, and end-to-end injection
- *     recovery (candidate at the injected frequency/spindown/sky).
+ *     recovery (candidate at the injected frequency/spindown/sky);
+ *   fstatnet - noise-free injection (F = rho^2/2 + 2 at the true point, amplitude
+ *     estimates equal the injected amplitudes), agreement with the grid search's F
+ *     at the grid candidate, gradient against central differences, phase against
+ *     an mpmath evaluation; amoeba - converges to the injected point.
  */
 #ifndef PGW_ORACLE_H
 #define PGW_ORACLE_H
@@ -81,6 +85,15 @@ int orc_resampled(orc_search *s, int pm, int mm, int nn, int det, double complex
 /* spindown range of a sky point as job_core computes it (jobcore.c:204,424-444) */
 void orc_spindown_range(const pgw_settings *sett, int mm, int nn, const pgw_range *s_range,
                         const pgw_opts *opts, int *smin, int *smax);
+
+/* Fstatnet (followup.c:413-534) at continuous pt = (omega, omegadot, delta, alpha): out[11] as the
+ * reference (out[5] = -F); grad (optional) = dF/dpt of F = -out[5].  Thread-safe. */
+int orc_fstatnet(const orc_search *s, const double pt[4], double out[11], double grad[4]);
+/* n points pts[4n] -> out[11n], grad[4n] (optional); OpenMP over points */
+int orc_fstatnet_batch(const orc_search *s, const double *pts, int n, double *out, double *grad);
+/* followup's Nelder-Mead (amoeba, followup.c:971-1173) on -F from point with initial steps pc2 */
+int orc_amoeba(const orc_search *s, double point[4], const double pc2[4], double tol, int maxeval,
+               double out[11]);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,7 @@ template <> struct ColE<double, 256> { static constexpr int E = 16; static const
 // grid (L1/C, ntemplates), 256 threads.  SEQ: the two spectra share one LDS buffer in turn.  For template t (ss = ss0 + t) and detector d:
 //   x_{a,b}[n] += xm[d][{a,b}][n] exp(-i phi_d(n)),  phi_d(n) = het1 n + sgnlt1 (n^2 + 2 n shft_d[n]),
 // zero for n >= N; the column FFT and the W_L^{n1 k2} twiddle follow; Z[t][{a,b}][k2][n1].
-template <typename T, int P2, bool SEQ>
+template <typename T, int P2, bool SEQ, bool UNI>
 __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, const TmplDesc* __restrict__ desc,
                                                    cplx<T>* __restrict__ Z, const cplx<T>* __restrict__ twc,
                                                    const cplx<T>* __restrict__ hi, const cplx<T>* __restrict__ lo) {
@@ -28,11 +28,22 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
   using S = ColShape<P2, E>;
   __shared__ T2 lds[(SEQ ? 1 : 2) * P2 * S::C];
   const int t = blockIdx.y;
-  const TmplDesc& dsc = desc[t];
-  const T2* __restrict__ xm = static_cast<const T2*>(sb.xm) + (size_t)dsc.slot * sb.xm_stride;
-  const double* __restrict__ shft = sb.shft + (size_t)dsc.slot * sb.shft_stride;
-  const double het1 = dsc.het1, sgnlt1 = dsc.sgnlt1;
-  const int N = dsc.N, nifo = dsc.nifo;
+  int slot, N;
+  double het1, sgnlt1;
+  if constexpr (UNI) {
+    slot = sb.uslot;
+    N = sb.uN;
+    spin_coeffs(sb.q, sb.q.ss0 + t, het1, sgnlt1);
+  } else {
+    const TmplDesc& dsc = desc[t];
+    slot = dsc.slot;
+    N = dsc.N;
+    het1 = dsc.het1;
+    sgnlt1 = dsc.sgnlt1;
+  }
+  const int nifo = sb.nifo;
+  const T2* __restrict__ xm = static_cast<const T2*>(sb.xm) + (size_t)slot * sb.xm_stride;
+  const double* __restrict__ shft = sb.shft + (size_t)slot * sb.shft_stride;
   const int c = threadIdx.x % S::C, g = threadIdx.x / S::C;
   const int n1 = blockIdx.x * S::C + c;
   const int L1 = p.L1;

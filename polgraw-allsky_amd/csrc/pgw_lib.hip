@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/pgw.h"
+#include "pgw_fstatnet.hpp"
 #include "pgw_kernels2.hpp"
 
 using namespace pgw;
@@ -75,6 +76,8 @@ template <typename T> struct DevBuf {
     if (hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) { p = nullptr; return PGW_ERR_NOMEM; }
     return PGW_OK;
   }
+  // keeps the buffer if it already holds count elements (the caller has synchronised its users)
+  int alloc_min(size_t count) { return p && n >= count ? PGW_OK : alloc(count); }
   void release() {
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -160,14 +163,16 @@ template <typename T>
 void launch_spin_col(const Plan<T>& pl, const SlotBases& sb, const TmplDesc* desc, cplx<T>* Z, int nt, const cplx<T>* hi,
                      const cplx<T>* lo, bool seq, hipStream_t st) {
   const SpinParams sp{pl.L1, pl.twshift};
+#define PGW_SCOL_GO(P2, SQ, U)                                                                           \
+  hipLaunchKernelGGL((k_spin_col2<T, P2, SQ, U>), dim3(pl.L1 / col_tile<T, P2>(), nt), dim3(256), 0, st, sp, sb, \
+                     desc, Z, pl.twc.p, hi, lo)
 #define PGW_SCOL_CASE(P2)                                                                                \
   case P2:                                                                                               \
-    if (seq || ColE<T, P2>::SEQ)                                                                         \
-      hipLaunchKernelGGL((k_spin_col2<T, P2, true>), dim3(pl.L1 / col_tile<T, P2>(), nt), dim3(256), 0, st, sp, sb, desc, \
-                         Z, pl.twc.p, hi, lo);                                                           \
-    else                                                                                                 \
-      hipLaunchKernelGGL((k_spin_col2<T, P2, false>), dim3(pl.L1 / col_tile<T, P2>(), nt), dim3(256), 0, st, sp, sb, desc, \
-                         Z, pl.twc.p, hi, lo);                                                           \
+    if (seq || ColE<T, P2>::SEQ) {                                                                       \
+      if (sb.uslot >= 0) PGW_SCOL_GO(P2, true, true); else PGW_SCOL_GO(P2, true, false);                 \
+    } else {                                                                                             \
+      if (sb.uslot >= 0) PGW_SCOL_GO(P2, false, true); else PGW_SCOL_GO(P2, false, false);               \
+    }                                                                                                    \
     break;
   switch (pl.L2) {
     PGW_SCOL_CASE(16)
@@ -177,6 +182,7 @@ void launch_spin_col(const Plan<T>& pl, const SlotBases& sb, const TmplDesc* des
     PGW_SCOL_CASE(256)
   }
 #undef PGW_SCOL_CASE
+#undef PGW_SCOL_GO
 }
 
 template <typename T>
@@ -221,6 +227,7 @@ struct pgw_handle {
   virtual int time_stages(int seg, int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps,
                           double* ms) = 0;
   virtual int autotune(int seg, int pm, int mm, int nn, const pgw_opts* o, int* tb, int* row_na, int* col_seq) = 0;
+  virtual int fstatnet(int seg, const double* pts, int n, double* out, double* grad) = 0;
   double last_ms = 0.0;
 };
 
@@ -279,6 +286,7 @@ template <typename T> struct Engine final : pgw_handle {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evd[2] = {nullptr, nullptr};
   int TB = 128, TBalloc = 0;
+  int group = kSlots;  // sky points per group
   int row_na = 1;
   bool col_seq = false;
 
@@ -381,12 +389,13 @@ template <typename T> struct Engine final : pgw_handle {
     int st;
     for (int k = 0; k < nseg; k++)
       if ((st = add_segment(&s[k], ifo + (size_t)k * ndet))) return st;
-    // kernel variants; PGW_TUNE="row_na=2,col_seq=1,tb=64" overrides (used when measuring variants)
+    // kernel variants; PGW_TUNE="row_na=2,col_seq=1,tb=64,group=4" overrides (used when measuring variants)
     if (const char* tune = getenv("PGW_TUNE")) {
       int v;
       if (const char* q = strstr(tune, "row_na=")) if (sscanf(q, "row_na=%d", &v) == 1 && (v == 1 || v == 2)) row_na = v;
       if (const char* q = strstr(tune, "col_seq=")) if (sscanf(q, "col_seq=%d", &v) == 1) col_seq = v != 0;
       if (const char* q = strstr(tune, "tb=")) if (sscanf(q, "tb=%d", &v) == 1 && v >= 1 && v <= kTBMax) TB = v;
+      if (const char* q = strstr(tune, "group=")) if (sscanf(q, "group=%d", &v) == 1 && v >= 1 && v <= kSlots) group = v;
     }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
@@ -518,12 +527,11 @@ template <typename T> struct Engine final : pgw_handle {
     if (o->s0_flag) smin = smax;
   }
 
-  // het1 and sgnlt[1] of spindown ss (jobcore.c:456,466-467)
+  static SpinGrid spin_grid(const double* M, int mm, int nn, int ss0, int s0_flag) {
+    return SpinGrid{M[0], M[4], M[5], M[9], M[13], mm, nn, ss0, s0_flag};
+  }
   static void tmpl_params(const double* M, int mm, int nn, int ss, int s0_flag, double& het1, double& sgnlt1) {
-#pragma clang fp contract(off)
-    sgnlt1 = (s0_flag ? 0. : ss * M[5] + nn * M[9] + mm * M[13]);
-    het1 = fmod(ss * M[4], M[0]);
-    if (het1 < 0) het1 += M[0];
+    spin_coeffs(spin_grid(M, mm, nn, 0, s0_flag), ss, het1, sgnlt1);
   }
 
   // descriptors of item `it`'s templates into host buffer `h` (index t0..)
@@ -540,10 +548,17 @@ template <typename T> struct Engine final : pgw_handle {
   }
 
   // template stage for descriptors [t0, t0 + nt) of buffer buf; ev (optional) = 4 events around the kernels
-  int launch_templates(const Seg& g, int buf, int t0, int nt, const pgw_opts* o, hipEvent_t* ev = nullptr) {
+  // uni: all nt templates are consecutive spindowns of one item (from spindown ss0), else nullptr
+  int launch_templates(const Seg& g, int buf, int t0, int nt, const Item* uni, int ss0, const pgw_opts* o,
+                       hipEvent_t* ev = nullptr) {
     const TmplDesc* d = desc.p + (size_t)buf * kDescCap + t0;
     if (ev) HIPCHK(hipEventRecord(ev[0], stream));
-    const SlotBases sb{xm_slots.p, shft_slots.p, aabb_slots.p, slot_xm, slot_shft};
+    SlotBases sb{xm_slots.p, shft_slots.p, aabb_slots.p, slot_xm, slot_shft, nifo, -1, 0, 0, SpinGrid{}};
+    if (uni) {
+      sb.uslot = uni->slot;
+      sb.uN = segs[uni->seg]->sett.N;
+      sb.q = spin_grid(segs[uni->seg]->sett.M, uni->mm, uni->nn, ss0, o->s0_flag);
+    }
     launch_spin_col<T>(*g.ps, sb, d, Z.p, nt, hi.p, lo.p, col_seq, stream);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));
     RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts};
@@ -575,8 +590,14 @@ template <typename T> struct Engine final : pgw_handle {
                           stream));
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p + gr.buf, 1);
     const Seg& g0 = *segs[gr.items[0].seg];
-    for (int t0 = 0; t0 < gr.ntempl; t0 += TB)
-      if ((st = launch_templates(g0, gr.buf, t0, std::min(TB, gr.ntempl - t0), o))) return st;
+    size_t k = 0;
+    for (int t0 = 0; t0 < gr.ntempl; t0 += TB) {
+      const int nt = std::min(TB, gr.ntempl - t0);
+      while (k + 1 < gr.items.size() && gr.items[k + 1].t0 <= t0) k++;
+      const Item& it = gr.items[k];
+      const bool one = t0 + nt <= it.t0 + (it.smax - it.smin + 1);
+      if ((st = launch_templates(g0, gr.buf, t0, nt, one ? &it : nullptr, it.smin + (t0 - it.t0), o))) return st;
+    }
     HIPCHK(hipMemcpyAsync(hn_pin.p + gr.buf, ncand.p + gr.buf, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(hc_pin.p + (size_t)gr.buf * kCandChunk, cand.p + (size_t)gr.buf * kCandCap,
                           sizeof(Cand) * kCandChunk, hipMemcpyDeviceToHost, stream));
@@ -644,7 +665,7 @@ template <typename T> struct Engine final : pgw_handle {
       Group gr;
       gr.buf = buf;
       const Seg* g0 = nullptr;
-      while (i < n && (int)gr.items.size() < kSlots) {
+      while (i < n && (int)gr.items.size() < group) {
         const int sgi = work[4 * i];
         if (sgi < 0 || sgi >= (int)segs.size()) { st = PGW_ERR_ARG; break; }
         const Seg& g = *segs[sgi];
@@ -713,6 +734,52 @@ template <typename T> struct Engine final : pgw_handle {
     return PGW_OK;
   }
 
+  // Fstatnet (followup.c:413-534) and its gradient at n points of segment sgi, in batches of
+  // kFsnBatch points: partial sums per (point, chunk), then a fixed-order final sum per point.
+  DevBuf<double> fsn_pts, fsn_part, fsn_out, fsn_grad;
+  int fstatnet(int sgi, const double* pts, int n, double* out, double* grad) override {
+    constexpr int kFsnBatch = 4096;
+    if (sgi < 0 || sgi >= (int)segs.size() || n < 0 || (n && (!pts || !out))) return PGW_ERR_ARG;
+    const Seg& g = *segs[sgi];
+    FsnParams q{};
+    q.N = g.sett.N;
+    q.nifo = nifo;
+    q.nchunk = (g.sett.N + FSN_CHUNK - 1) / FSN_CHUNK;
+    q.oms = g.sett.oms;
+    for (int d = 0; d < nifo; d++) {
+      for (int k = 0; k < 9; k++) q.det[d].amod[k] = g.amod[9 * d + k];
+      q.det[d].sphir = g.sphir[d];
+      q.det[d].cphir = g.cphir[d];
+      q.det[d].w = 1.0 / g.sig2[d];
+    }
+    const int nb = std::min(n, kFsnBatch);
+    int st;
+    if (nb > 0 && ((st = fsn_pts.alloc_min((size_t)4 * nb)) || (st = fsn_part.alloc_min((size_t)nb * q.nchunk * FSN_NSUM)) ||
+                   (st = fsn_out.alloc_min((size_t)11 * nb)) || (st = fsn_grad.alloc_min((size_t)4 * nb))))
+      return st;
+    HIPCHK(hipEventRecord(ev0, stream));
+    for (int p0 = 0; p0 < n; p0 += kFsnBatch) {
+      const int m = std::min(kFsnBatch, n - p0);
+      HIPCHK(hipMemcpyAsync(fsn_pts.p, pts + 4 * (size_t)p0, sizeof(double) * 4 * m, hipMemcpyHostToDevice, stream));
+      hipLaunchKernelGGL(k_fstatnet_part, dim3(m, q.nchunk), dim3(FSN_NT), 0, stream, q, fsn_pts.p, g.xDat.p,
+                         g.DetSSB.p, g.cosmodf.p, g.sinmodf.p, fsn_part.p);
+      hipLaunchKernelGGL(k_fstatnet_final, dim3((m + 63) / 64), dim3(64), 0, stream, m, q.nchunk, fsn_part.p,
+                         fsn_pts.p, fsn_out.p, grad ? fsn_grad.p : nullptr);
+      if (hipGetLastError() != hipSuccess) return PGW_ERR_DEVICE;
+      HIPCHK(hipMemcpyAsync(out + 11 * (size_t)p0, fsn_out.p, sizeof(double) * 11 * m, hipMemcpyDeviceToHost, stream));
+      if (grad)
+        HIPCHK(hipMemcpyAsync(grad + 4 * (size_t)p0, fsn_grad.p, sizeof(double) * 4 * m, hipMemcpyDeviceToHost,
+                              stream));
+      HIPCHK(hipStreamSynchronize(stream));  // the next batch reuses the device buffers
+    }
+    HIPCHK(hipEventRecord(ev1, stream));
+    HIPCHK(hipEventSynchronize(ev1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, ev0, ev1));
+    last_ms = t;
+    return PGW_OK;
+  }
+
   int debug_resampled(int pm, int mm, int nn, int det, double* a, double* b) override {
     if (det < 0 || det >= nifo) return PGW_ERR_ARG;
     const Seg& g = *segs[0];
@@ -757,7 +824,7 @@ template <typename T> struct Engine final : pgw_handle {
       acc[0] = t;
       for (int rr = 0; rr < reps && !st; rr++) {
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p, 1);
-        st = launch_templates(g, 0, 0, nt, o, ev);
+        st = launch_templates(g, 0, 0, nt, &it, it.smin, o, ev);
         HIPCHK(hipStreamSynchronize(stream));
         for (int k = 0; k < 3; k++) {
           HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
@@ -947,6 +1014,11 @@ int pgw_time_stages(pgw_handle* h, int pm, int mm, int nn, int s0, int nt, const
                     double ms[5]) {
   if (!h || !opts || !ms) return PGW_ERR_ARG;
   return h->time_stages(0, pm, mm, nn, s0, nt, opts, reps, ms);
+}
+
+int pgw_fstatnet(pgw_handle* h, int seg, const double* pts, int npts, double* out, double* grad) {
+  if (!h) return PGW_ERR_ARG;
+  return h->fstatnet(seg, pts, npts, out, grad);
 }
 
 int pgw_copy_bench(size_t bytes, int reps, double* ms) {

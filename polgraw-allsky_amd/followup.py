@@ -1,0 +1,129 @@
+"""Followup of candidates with the exact network F-statistic (followup/src/followup.c).
+
+`FStatNet` is the reference's Fstatnet (followup.c:413-534) for batches of points on the device,
+with its analytic gradient; `fstat` makes it differentiable in torch; `ascend` is a batched
+quasi-Newton ascent started from the phase metric, an alternative to the simplex search of
+followup.c:1126-1173 (`amoeba`) when the gradient is available.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import pgw
+
+
+def _phase_derivatives(DetSSB, N: int, oms: float, point, stride: int = 1):
+    """d phi / d(omega, omegadot, delta, alpha) at every stride-th sample of every detector, for
+    phi = omega (i + shft) + omegadot i^2 + (oms + 2 omegadot i) shft (followup.c:455-457)."""
+    om, omd, de, al = (float(v) for v in point)
+    i = np.arange(0, N, stride, dtype=np.float64)
+    nS = np.array([np.cos(al) * np.cos(de), np.sin(al) * np.cos(de), np.sin(de)])
+    nSd = np.array([-np.cos(al) * np.sin(de), -np.sin(al) * np.sin(de), np.cos(de)])
+    nSa = np.array([-np.sin(al) * np.cos(de), np.cos(al) * np.cos(de), 0.0])
+    d = [[], [], [], []]
+    for R in DetSSB:
+        R = np.asarray(R, dtype=np.float64)[:3 * N].reshape(N, 3)[::stride]
+        shft = R @ nS
+        K = om + oms + 2 * omd * i
+        d[0].append(i + shft)
+        d[1].append(i * i + 2 * i * shft)
+        d[2].append(K * (R @ nSd))
+        d[3].append(K * (R @ nSa))
+    return np.stack([np.concatenate(v) for v in d])
+
+
+def phase_scales(DetSSB, N: int, oms: float, point) -> np.ndarray:
+    """Parameter steps that change the phase by about one radian rms over the segment: for
+    parameter k, 1 / std_i(d phi / d theta_k) (the mean is removed: a constant phase does not
+    change F).  DetSSB: the detectors' 3N barycentric positions."""
+    return 1.0 / np.std(_phase_derivatives(DetSSB, N, oms, point), axis=1)
+
+
+def phase_metric(DetSSB, N: int, oms: float, point, stride: int = 8) -> np.ndarray:
+    """Phase metric G_jk = Cov_i(d phi/d theta_j, d phi/d theta_k) over the samples of all detectors.
+    Near a maximum F(theta + d) ~ F(theta) (1 - d^T G d), so G^-1 grad F / (2 F) is a Newton step
+    and sqrt(d^T G d) is the rms phase change of a step in radians.  The derivatives are smooth in
+    time, so every stride-th sample gives the covariance to high accuracy."""
+    D = _phase_derivatives(DetSSB, N, oms, point, stride)
+    D = D - D.mean(axis=1, keepdims=True)
+    return D @ D.T / D.shape[1]
+
+
+class FStatNet:
+    """Fstatnet at batches of points of one segment of a pgw.Handle (FP64 on the device)."""
+
+    def __init__(self, handle: pgw.Handle, seg: int = 0):
+        self.h = handle
+        self.seg = seg
+
+    def __call__(self, pts, grad: bool = True):
+        """pts (n, 4) = (omega, omegadot, delta, alpha) -> (out (n, 11), dF/dpts (n, 4) or None).
+        out[:, 5] = -F as the reference; the gradient is of F."""
+        return self.h.fstatnet(pts, seg=self.seg, grad=grad)
+
+
+def _torch_fn():
+    import torch
+
+    class FstatFn(torch.autograd.Function):
+        """F(pts) with the device's analytic gradient: pts (n, 4) float64 tensor -> F (n,)."""
+
+        @staticmethod
+        def forward(ctx, pts, fs):
+            out, g = fs(pts.detach().cpu().numpy(), grad=True)
+            ctx.save_for_backward(torch.from_numpy(g).to(pts.device))
+            return torch.from_numpy(-out[:, 5]).to(pts.device)
+
+        @staticmethod
+        def backward(ctx, gF):
+            (g,) = ctx.saved_tensors
+            return gF[:, None] * g, None
+
+    return FstatFn
+
+
+def fstat(pts, fs: FStatNet):
+    """Differentiable F at pts (torch float64 (n, 4))."""
+    return _torch_fn().apply(pts, fs)
+
+
+def ascend(fs: FStatNet, start, metric, steps: int = 20, trust0: float = 1.0, trust_max: float = 4.0):
+    """Batched quasi-Newton ascent of F from start (n, 4), one device evaluation per step.
+
+    metric: (4, 4) or (n, 4, 4) phase metrics (phase_metric at each start).  The inverse Hessian
+    starts from (2 F G)^-1, the Newton step of the quadratic model F (1 - d^T G d), and BFGS updates
+    it from the exact gradients, which brings in the curvature of the amplitude modulation that the
+    phase metric leaves out.  Each step is cut to a trust radius in rms radians of phase
+    (sqrt(d^T G d)); it is kept if F increases (radius doubles, up to trust_max), else rejected
+    (radius / 4).  All candidates are evaluated in one device batch per step.
+    Returns (points, F, evaluations per candidate)."""
+    x = np.array(start, dtype=np.float64).reshape(-1, 4)
+    n = len(x)
+    G = np.broadcast_to(np.asarray(metric, dtype=np.float64), (n, 4, 4)).copy()
+    # regularise the near-degenerate directions of short segments
+    G += 1e-9 * np.trace(G, axis1=1, axis2=2)[:, None, None] * np.eye(4)
+    out, g = fs(x)
+    F = -out[:, 5]
+    H = np.linalg.inv(2 * F[:, None, None] * G)
+    trust = np.full(n, float(trust0))
+    eye = np.eye(4)
+    nev = 1
+    for _ in range(steps):
+        d = np.einsum("nij,nj->ni", H, g)
+        rms = np.sqrt(np.maximum(np.einsum("ni,nij,nj->n", d, G, d), 1e-300))
+        d *= np.minimum(1.0, trust / rms)[:, None]
+        out_t, g_t = fs(x + d)
+        nev += 1
+        Ft = -out_t[:, 5]
+        ok = Ft > F
+        for k in np.nonzero(ok)[0]:
+            y = g[k] - g_t[k]                  # change of the gradient of -F
+            ys = y @ d[k]
+            if ys > 0:                         # curvature condition; skip the update otherwise
+                r = 1.0 / ys
+                A = eye - r * np.outer(d[k], y)
+                H[k] = A @ H[k] @ A.T + r * np.outer(d[k], d[k])
+        x[ok] += d[ok]
+        F[ok], g[ok] = Ft[ok], g_t[ok]
+        trust = np.where(ok, np.minimum(2 * trust, trust_max), trust / 4)
+    return x, F, nev

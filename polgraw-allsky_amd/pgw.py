@@ -48,6 +48,7 @@ def lib():
                                           _ip, _dp, C.c_int, _ip, _ip]
         L.pgw_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts),
                                    _ip, _ip, _ip]
+        L.pgw_fstatnet.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp]
         L.pgw_time_stages.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.POINTER(abi.Opts), C.c_int, _dp]
         L.pgw_copy_bench.argtypes = [C.c_size_t, C.c_int, _dp]
@@ -61,7 +62,7 @@ def lib():
 
 
 EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_job_core", "pgw_job_batch",
-            "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune",
+            "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_fstatnet",
             "pgw_debug_fstat", "pgw_debug_resampled", "pgw_time_stages", "pgw_copy_bench", "pgw_last_gpu_ms",
             "pgw_destroy", "pgw_strerror"]
 
@@ -137,6 +138,17 @@ class Handle:
             raise PGWError(st, "pgw_job_batch_multi")
         n = sgnlc.value
         return sgnlv[:abi.NPAR * n].reshape(-1, abi.NPAR), owner[:n].copy(), fnum.value
+
+    def fstatnet(self, pts, seg=0, grad=True):
+        """Fstatnet at points (n, 4): (out (n, 11) with out[:, 5] = -F, dF/dpts (n, 4) or None)."""
+        pts = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(-1, 4))
+        out = np.zeros((len(pts), 11))
+        g = np.zeros((len(pts), 4)) if grad else None
+        st = lib().pgw_fstatnet(self.h, seg, pts.ctypes.data_as(_dp), len(pts), out.ctypes.data_as(_dp),
+                                g.ctypes.data_as(_dp) if grad else None)
+        if st != 0:
+            raise PGWError(st, "pgw_fstatnet")
+        return out, g
 
     def autotune(self, seg, pm, mm, nn, opts):
         tb, na, seq = C.c_int(), C.c_int(), C.c_int()

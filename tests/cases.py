@@ -1,6 +1,8 @@
 """Seeded segments shared by the parity tests (sizes the oracle finishes in seconds)."""
 import functools
 
+import numpy as np
+
 import pgwload
 
 pgwload.load()
@@ -40,3 +42,38 @@ def injected_sky(seg):
     """(pm, mm, nn) of the grid point nearest the injection."""
     g = seg.injection["grid"]
     return seg.injection["pm"], int(g[3]), int(g[2])
+
+
+def phase_scales(seg, point):
+    """Steps of about one radian of phase for each of (omega, omegadot, delta, alpha) at point."""
+    from polgraw_allsky_amd import followup
+    return followup.phase_scales(seg.DetSSB, seg.sett["N"], seg.sett["oms"], point)
+
+
+def noise_free_signal(N=32768, dets=("H1", "L1"), seed=5, h0=0.05, fftpad=2, amp=None):
+    """Segment whose data are a noise-free signal at off-grid parameters (sig2 = 1): the
+    injection formula of init.c:600-615 with amplitudes amp.  Returns (seg, truth, amp, rho2)."""
+    import math
+    seg = synth.make_segment(N=N, dets=dets, sigmas=(1.0,) * len(dets), fftpad=fftpad, seed=seed,
+                             inject=False, nsky=(1, 1), nspin=3)
+    rng = np.random.default_rng(seed + 1000)
+    s = seg.sett
+    f = rng.uniform(0.2 * math.pi, 0.8 * math.pi)
+    fdot = -rng.uniform(0.1, 0.9) * 1e-8
+    delta = math.asin(rng.uniform(-0.9, 0.9))
+    alpha = rng.uniform(0.0, 2 * math.pi)
+    amp = rng.uniform(-1.0, 1.0, 4) if amp is None else np.asarray(amp, dtype=float)
+    t = np.arange(N, dtype=float)
+    cm, sm = np.cos(s["omr"] * t), np.sin(s["omr"] * t)
+    sa, ca, sd, cd = math.sin(alpha), math.cos(alpha), math.sin(delta), math.cos(delta)
+    rho2 = 0.0
+    for k in range(len(dets)):
+        aa, bb = synth.modvir(sa, ca, sd, cd, math.sin(seg.phir[k]), math.cos(seg.phir[k]), seg.amod[k], cm, sm)
+        shift = seg.DetSSB[k].reshape(-1, 3) @ np.array([ca * cd, sa * cd, sd])
+        ph = f * t + fdot * t * t + (s["oms"] + f + 2 * fdot * t) * shift
+        sig = h0 * (amp[0] * aa * np.cos(ph) + amp[2] * aa * np.sin(ph) + amp[1] * bb * np.cos(ph)
+                    + amp[3] * bb * np.sin(ph))
+        seg.xDat[k] = sig
+        seg.sig2[k] = 1.0
+        rho2 += float(np.sum(sig ** 2))
+    return seg, np.array([f, fdot, delta, alpha]), amp * h0, rho2

@@ -1,0 +1,117 @@
+"""Pins the oracle's Fstatnet (followup.c:413-534), its gradient and the Nelder-Mead followup.
+
+Independent checks: the noise-free closed form (F = rho^2/2 and amplitude estimates = injected
+amplitudes at the true point), central differences of F for the gradient, the grid search's F
+at its grid candidate, and the simplex converging onto the injected parameters."""
+import math
+
+import numpy as np
+import pytest
+
+from cases import noise_free_signal, phase_scales
+from polgraw_allsky_amd import synth
+
+
+@pytest.fixture(scope="module")
+def clean(orc):
+    seg, truth, amp, rho2 = noise_free_signal()
+    return seg, truth, amp, rho2, orc.Search(seg, nthreads=8)
+
+
+def closed_form(seg, truth, amp):
+    """Noise-free limit of Fstatnet at the true point: with A = sum a^2, B = sum b^2, C = sum a b
+    over detectors (modvir at the true sky, sig2 = 1), Xa = (amp0 A + amp1 C - i (amp2 A + amp3 C))/2
+    and Xb = (amp0 C + amp1 B - i (amp2 C + amp3 B))/2 up to the sums of cos 2phi, sin 2phi terms."""
+    s = seg.sett
+    t = np.arange(s["N"], dtype=float)
+    cm, sm = np.cos(s["omr"] * t), np.sin(s["omr"] * t)
+    d, al = truth[2], truth[3]
+    A = B = Cab = 0.0
+    for k in range(len(seg.dets)):
+        a, b = synth.modvir(math.sin(al), math.cos(al), math.sin(d), math.cos(d), math.sin(seg.phir[k]),
+                            math.cos(seg.phir[k]), seg.amod[k], cm, sm)
+        A += np.sum(a * a)
+        B += np.sum(b * b)
+        Cab += np.sum(a * b)
+    Xa = 0.5 * complex(amp[0] * A + amp[1] * Cab, -(amp[2] * A + amp[3] * Cab))
+    Xb = 0.5 * complex(amp[0] * Cab + amp[1] * B, -(amp[2] * Cab + amp[3] * B))
+    F = abs(Xa) ** 2 / A + abs(Xb) ** 2 / B
+    est = np.array([2 * Xa.real / A, 2 * Xb.real / B, -2 * Xa.imag / A, -2 * Xb.imag / B])
+    return F, est, Xa, Xb
+
+
+def test_noise_free_closed_form(clean):
+    seg, truth, amp, rho2, S = clean
+    out, _ = S.fstatnet(truth[None, :], grad=False)
+    out = out[0]
+    F = -out[5]
+    Fc, est, Xa, Xb = closed_form(seg, truth, amp)
+    assert F == pytest.approx(Fc, rel=1e-3)
+    assert np.allclose(out[:4], est, rtol=0, atol=1e-3 * np.max(np.abs(est)))
+    assert np.array_equal(out[6:10], truth)
+    assert out[4] == pytest.approx(math.sqrt(2 * (F - 2)))
+    # out[10] = sqrt((2 Re Xa)^2 + (2 Re Xb)^2 + (2 Im Xa)^2 + (2 Im Xb)^2)
+    assert out[10] == pytest.approx(2 * math.sqrt(abs(Xa) ** 2 + abs(Xb) ** 2), rel=1e-3)
+
+
+def test_gradient_central_differences(clean):
+    seg, truth, amp, rho2, S = clean
+    sc = phase_scales(seg, truth)
+    rng = np.random.default_rng(3)
+    pts = truth[None, :] + (rng.uniform(-0.3, 0.3, (4, 4)) * sc)
+    out, g = S.fstatnet(pts)
+    h = 1e-4
+    for k in range(4):
+        e = np.zeros(4)
+        e[k] = h * sc[k]
+        fp, _ = S.fstatnet(pts + e, grad=False)
+        fm, _ = S.fstatnet(pts - e, grad=False)
+        fd = (-fp[:, 5] + fm[:, 5]) / (2 * e[k])
+        # compared per unit scaled step, relative to F
+        err = np.abs(g[:, k] - fd) * sc[k] / (-out[:, 5])
+        assert np.max(err) < 1e-6, (k, err)
+
+
+def test_maximum_at_truth(orc):
+    """With the signal in the a-channel only (amp1 = amp3 = 0) the statistic peaks at the true
+    parameters: the scaled gradient there is small next to its size a quarter step away.  (With
+    both channels and a nonzero a-b overlap the peak of this statistic moves off the truth.)"""
+    seg, truth, amp, rho2 = noise_free_signal(seed=8, amp=[0.7, 0.0, -0.4, 0.0])
+    S = orc.Search(seg, nthreads=8)
+    sc = phase_scales(seg, truth)
+    _, g0 = S.fstatnet(truth[None, :])
+    for k in range(4):
+        p = truth.copy()
+        p[k] += 0.25 * sc[k]
+        _, g1 = S.fstatnet(p[None, :])
+        assert abs(g0[0, k] * sc[k]) < 0.1 * abs(g1[0, k] * sc[k]), k
+
+
+def test_agrees_with_grid_search(orc, abi, synth):
+    """Fstatnet at the grid search's best candidate reproduces that candidate's F (no FStat
+    normalisation) to the grid search's own approximation (barycentring by resampling and spline,
+    about 1 %); a sign or index slip in the phase model would leave F at the noise level."""
+    from cases import injected_sky, segment
+    seg = segment("net-2det-32k")
+    S = orc.Search(seg, nthreads=8)
+    pm, mm, nn = injected_sky(seg)
+    recs, _ = S.job_core(pm, mm, nn, abi.make_range(seg), abi.make_opts(trl=10.0, white=True))
+    best = recs[np.argmax(recs[:, 4])]
+    Fgrid = best[4] ** 2 / 2 + 2
+    out, _ = S.fstatnet(best[None, :4], grad=False)
+    assert -out[0, 5] == pytest.approx(Fgrid, rel=0.03)
+
+
+def test_amoeba_reaches_a_maximum(orc):
+    """The simplex started half a scaled step from the injection ends at a stationary point of F
+    (scaled gradient below 1e-3 F) with F at least F(truth)."""
+    seg, truth, amp, rho2 = noise_free_signal(seed=8, amp=[0.7, 0.0, -0.4, 0.0])
+    S = orc.Search(seg, nthreads=8)
+    sc = phase_scales(seg, truth)
+    start = truth + np.array([0.6, -0.5, 0.4, -0.4]) * sc
+    x, out, nev = S.amoeba(start, 0.5 * sc, tol=1e-12)
+    assert nev > 0
+    F0, _ = S.fstatnet(truth[None, :], grad=False)
+    assert -out[5] >= -F0[0, 5] * (1 - 1e-9)
+    _, g = S.fstatnet(x[None, :])
+    assert np.max(np.abs(g[0] * sc)) < 1e-3 * -out[5]
