@@ -40,6 +40,7 @@ extern "C" {
 #define PGW_ERR_NOMEM (-4)      /* host or device allocation failed */
 #define PGW_ERR_UNSUPPORTED (-5)/* size outside what the kernels are built for */
 #define PGW_ERR_NODEVICE (-6)   /* no HIP device visible */
+#define PGW_ERR_IO (-7)         /* a trigger or state file could not be written or read */
 
 /* arithmetic of the modulated series and the FFTs; phases are FP64 in both */
 #define PGW_PREC_FP64 0
@@ -95,6 +96,16 @@ typedef struct pgw_opts {
 } pgw_opts;
 
 typedef struct pgw_handle pgw_handle;
+
+/* Where pgw_search writes (the Command_line_opts fields of search(), jobcore.c:51-184). */
+typedef struct pgw_output {
+  const char *prefix;    /* directory of the trigger files (opts->prefix) */
+  int ident, band;       /* frame and band numbers in the file names (opts->ident, opts->band) */
+  const char *label;     /* file-name label (opts->label); NULL or "" for none */
+  const char *qname;     /* checkpoint state file (opts->qname); NULL: no checkpointing */
+  int checkpoint_every;  /* sky points between checkpoints and trigger flushes (<= 0: 64) */
+  int max_sky;           /* stop after this many sky points, state saved (<= 0: no limit) */
+} pgw_output;
 
 /* search_settings() + the size part of plan_fftw()
  * (search/network/src-cpu/settings.c:16-81, init.c:835-844) for a segment of N samples.
@@ -186,6 +197,28 @@ int pgw_copy_bench(size_t bytes, int reps, double *ms);
 
 /* Elapsed device time (ms) of the last pgw_job_core / pgw_job_batch call, from HIP events. */
 double pgw_last_gpu_ms(const pgw_handle *h);
+
+/* The settings of segment `seg` as the handle holds them (pgw_init's input). */
+int pgw_get_settings(const pgw_handle *h, int seg, pgw_settings *sett);
+
+/* search(), search/network/src-cpu/jobcore.c:51-184: hemispheres pm = pst..pmr[1], then
+ * mm = mst..mr[1], nn = nst..nr[1] (the starts fall back to mr[0], nr[0] and sst to spndr[0]
+ * after the first), every sky point through pgw_job_core's semantics.  Records are appended to
+ * <prefix>/triggers_<ident %03d>_<band %04d><label>_<pm>.bin (5 doubles each) under an fcntl
+ * write lock whenever more than nfft are buffered and at the end of each hemisphere (the file is
+ * created even when empty), as the reference does.  Sky points run in batches of up to
+ * checkpoint_every through the batched engine.  With a state file, after each batch the buffered
+ * records are appended first and then "pm mm nn sst FNum" of the next sky point is written (the
+ * reference writes the state before its buffer is flushed, so a restart loses the buffered
+ * records); at the end the state file is emptied (search/network/src-cpu/main.c:141-147).
+ * s_range's start fields are updated as the reference's are. */
+int pgw_search(pgw_handle *h, pgw_range *s_range, const pgw_opts *opts, const pgw_output *out, int *FNum);
+
+/* read_checkpoints(), search/network/src-cpu/init.c:868-917, for the state file `qname`:
+ * no file: starts at the range's beginning, *FNum = 0, returns PGW_OK; a state: pst, mst, nst,
+ * sst and *FNum from it, returns PGW_OK; an empty file (a finished run): returns 1;
+ * unreadable contents: PGW_ERR_IO. */
+int pgw_read_checkpoint(const char *qname, pgw_range *s_range, int *FNum);
 
 /* Release everything owned by the handle.  Safe on NULL. */
 void pgw_destroy(pgw_handle *h);

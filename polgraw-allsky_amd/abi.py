@@ -16,6 +16,7 @@ PGW_ERR_DEVICE = -3
 PGW_ERR_NOMEM = -4
 PGW_ERR_UNSUPPORTED = -5
 PGW_ERR_NODEVICE = -6
+PGW_ERR_IO = -7
 
 PREC_FP64 = 0
 PREC_FP32 = 1
@@ -48,6 +49,11 @@ class Range(C.Structure):
         ("pmr", C.c_int * 2), ("mr", C.c_int * 2), ("nr", C.c_int * 2), ("spndr", C.c_int * 2),
         ("pst", C.c_int), ("mst", C.c_int), ("nst", C.c_int), ("sst", C.c_int),
     ]
+
+
+class Output(C.Structure):
+    _fields_ = [("prefix", C.c_char_p), ("ident", C.c_int), ("band", C.c_int), ("label", C.c_char_p),
+                ("qname", C.c_char_p), ("checkpoint_every", C.c_int), ("max_sky", C.c_int)]
 
 
 class Opts(C.Structure):

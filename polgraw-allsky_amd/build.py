@@ -8,7 +8,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
-SRC = PKG / "csrc" / "pgw_lib.hip"
+SRCS = [PKG / "csrc" / "pgw_lib.hip", PKG / "csrc" / "pgw_driver.cpp"]
 OUT = PKG / "libpgw_hip.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -21,7 +21,7 @@ FLAGS = [
 
 
 def build(verbose: bool = False) -> Path:
-    cmd = [HIPCC, *FLAGS, "-I", str(ROOT / "include"), "-o", str(OUT) + ".tmp", str(SRC)]
+    cmd = [HIPCC, *FLAGS, "-I", str(ROOT / "include"), "-o", str(OUT) + ".tmp", *map(str, SRCS)]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -228,6 +228,7 @@ struct pgw_handle {
                           double* ms) = 0;
   virtual int autotune(int seg, int pm, int mm, int nn, const pgw_opts* o, int* tb, int* row_na, int* col_seq) = 0;
   virtual int fstatnet(int seg, const double* pts, int n, double* out, double* grad) = 0;
+  virtual int settings(int seg, pgw_settings* out) const = 0;
   double last_ms = 0.0;
 };
 
@@ -736,6 +737,12 @@ template <typename T> struct Engine final : pgw_handle {
 
   // Fstatnet (followup.c:413-534) and its gradient at n points of segment sgi, in batches of
   // kFsnBatch points: partial sums per (point, chunk), then a fixed-order final sum per point.
+  int settings(int sgi, pgw_settings* out) const override {
+    if (sgi < 0 || sgi >= (int)segs.size()) return PGW_ERR_ARG;
+    *out = segs[sgi]->sett;
+    return PGW_OK;
+  }
+
   DevBuf<double> fsn_pts, fsn_part, fsn_out, fsn_grad;
   int fstatnet(int sgi, const double* pts, int n, double* out, double* grad) override {
     constexpr int kFsnBatch = 4096;
@@ -1052,6 +1059,11 @@ int pgw_copy_bench(size_t bytes, int reps, double* ms) {
 
 double pgw_last_gpu_ms(const pgw_handle* h) { return h ? h->last_ms : 0.0; }
 
+int pgw_get_settings(const pgw_handle* h, int seg, pgw_settings* sett) {
+  if (!h || !sett) return PGW_ERR_ARG;
+  return h->settings(seg, sett);
+}
+
 void pgw_destroy(pgw_handle* h) { delete h; }
 
 const char* pgw_strerror(int status) {
@@ -1063,6 +1075,7 @@ const char* pgw_strerror(int status) {
     case PGW_ERR_NOMEM: return "out of memory";
     case PGW_ERR_UNSUPPORTED: return "unsupported size";
     case PGW_ERR_NODEVICE: return "no HIP device";
+    case PGW_ERR_IO: return "trigger or state file I/O error";
     default: return "unknown status";
   }
 }

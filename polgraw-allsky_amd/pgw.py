@@ -49,6 +49,10 @@ def lib():
         L.pgw_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts),
                                    _ip, _ip, _ip]
         L.pgw_fstatnet.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp]
+        L.pgw_get_settings.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.Settings)]
+        L.pgw_search.argtypes = [C.c_void_p, C.POINTER(abi.Range), C.POINTER(abi.Opts), C.POINTER(abi.Output),
+                                 _ip]
+        L.pgw_read_checkpoint.argtypes = [C.c_char_p, C.POINTER(abi.Range), _ip]
         L.pgw_time_stages.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.POINTER(abi.Opts), C.c_int, _dp]
         L.pgw_copy_bench.argtypes = [C.c_size_t, C.c_int, _dp]
@@ -62,9 +66,19 @@ def lib():
 
 
 EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_job_core", "pgw_job_batch",
-            "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_fstatnet",
+            "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_fstatnet", "pgw_get_settings",
+            "pgw_search", "pgw_read_checkpoint",
             "pgw_debug_fstat", "pgw_debug_resampled", "pgw_time_stages", "pgw_copy_bench", "pgw_last_gpu_ms",
             "pgw_destroy", "pgw_strerror"]
+
+
+def read_checkpoint(qname, rng):
+    """read_checkpoints(): (status, FNum); status 1 = the state file is empty (run finished)."""
+    fn = C.c_int(0)
+    st = lib().pgw_read_checkpoint(str(qname).encode(), C.byref(rng), C.byref(fn))
+    if st < 0:
+        raise PGWError(st, "pgw_read_checkpoint")
+    return st, fn.value
 
 
 def copy_bench(nbytes: int, reps: int = 20) -> float:
@@ -149,6 +163,16 @@ class Handle:
         if st != 0:
             raise PGWError(st, "pgw_fstatnet")
         return out, g
+
+    def search(self, rng, opts, prefix, ident=1, band=666, label="", qname=None, every=64, max_sky=0, FNum=0):
+        """search(): trigger files under prefix, optional checkpoint state; returns FNum."""
+        out = abi.Output(prefix=str(prefix).encode(), ident=ident, band=band, label=label.encode(),
+                         qname=str(qname).encode() if qname else None, checkpoint_every=every, max_sky=max_sky)
+        fn = C.c_int(FNum)
+        st = lib().pgw_search(self.h, C.byref(rng), C.byref(opts), C.byref(out), C.byref(fn))
+        if st != 0:
+            raise PGWError(st, "pgw_search")
+        return fn.value
 
     def autotune(self, seg, pm, mm, nn, opts):
         tb, na, seq = C.c_int(), C.c_int(), C.c_int()
