@@ -1,15 +1,23 @@
 #!/usr/bin/env python3
 """Throughput of the coherent F-statistic search core on one MI355X.
 
-Workload (the network configuration): two detectors (H1-, L1-like; white noise sigma 1 and 1.3,
-one injection h0 = 0.1), N = 86400 samples at dt = 2 s, nfft = 2^17, fftpad = 2 (F-statistic FFT
-length L = 2^18), 256 sky points x 512 spindowns = 131072 templates, threshold F = 20.
+Default workload (the network configuration): two detectors (H1-, L1-like; white noise sigma 1
+and 1.3, one injection h0 = 0.1), N = 86400 samples at dt = 2 s, nfft = 2^17, fftpad = 2
+(F-statistic FFT length L = 2^18), 256 sky points x 512 spindowns = 131072 templates, threshold
+F = 20.
 
 A step is one pgw_job_batch call over `--sky-per-step` sky points (all their spindowns); the
 default 16 steps x 16 sky points cover the 256 sky points once.  Inputs are resident in HBM
 before the timed region.  The headline value is the FP32 form (FP64 phases and spline
 coefficients, FP32 series and FFTs); the FP64 path is timed the same way beside it, and the FP32
 candidates and F values are compared with the FP64 ones.
+
+--workload selects the other configurations, each printed as its own JSON line:
+  plain            1 detector, N=86400, nfft=L=2^17, 4 sky x 32 spindowns
+  precision        2 detectors, N=259200, nfft=2^18, fftpad 2 (L=2^19), 1024 sky x 256 spindowns
+  batching         3 detectors, 64 segments of N in [60000, 86400] with 5-30 % gaps (nfft 2^16 or
+                   2^17, fftpad 1), 32 sky x 512 spindowns each (1,048,576 templates), one handle
+  differentiation  2 detectors, N=86400: Fstatnet and its gradient at 1024 candidate points
 
 Prints one JSON line.
 """
@@ -35,10 +43,21 @@ METRIC = "F-stat templates/s on 1 MI355X (FP32-batched vs FP64 plain); max rel Î
 HBM_PEAK_GBS = 8000.0
 
 
-def workload(args):
-    seg = synth.make_segment(N=86400, dets=("H1", "L1"), sigmas=(1.0, 1.3), fftpad=2, seed=args.seed, h0=0.1,
-                             nsky=(16, 16), nspin=512)
-    return seg
+# template-search workloads: make_segment arguments and the config text
+TEMPLATE_WORKLOADS = {
+    "network": (dict(N=86400, dets=("H1", "L1"), sigmas=(1.0, 1.3), fftpad=2, h0=0.1, nsky=(16, 16), nspin=512),
+                "network: 2 detectors (H1, L1), N=86400, nfft=2^17, fftpad=2, L=2^18, 256 sky x 512 spindowns, trl=20"),
+    "plain": (dict(N=86400, dets=("H1",), sigmas=(1.0,), fftpad=1, h0=0.3, nsky=(2, 2), nspin=32),
+              "plain: 1 detector (H1), N=86400, nfft=L=2^17, fftpad=1, 4 sky x 32 spindowns, trl=20"),
+    "precision": (dict(N=259200, dets=("H1", "L1"), sigmas=(1.0, 1.3), fftpad=2, h0=0.06, nsky=(32, 32), nspin=256),
+                  "precision: 2 detectors (H1, L1), N=259200, nfft=2^18, fftpad=2, L=2^19, 1024 sky x 256 spindowns, "
+                  "trl=20"),
+}
+
+
+def workload(args, name="network"):
+    kw, _ = TEMPLATE_WORKLOADS[name]
+    return synth.make_segment(seed=args.seed, **kw)
 
 
 def run_path(h, sky, rng, opts, steps, warmup, per_step):
@@ -120,65 +139,64 @@ def roofline(h, seg, opts, prec_bytes, nt=128, reps=10):
                 template_stage_GBps=round(b_t / (stage_ms * 1e-3) / 1e9, 1))
 
 
-def cpu_baseline(seg, rng, opts):
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import orc
-    threads = orc.default_threads()
-    o = orc.Search(seg, nthreads=threads)
-    pm, mm, nn = seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2])
-    t0 = time.perf_counter()
-    _, fn = o.job_core(pm, mm, nn, rng, opts)
-    dt = time.perf_counter() - t0
-    model = "unknown"
+def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
+                return line.split(":", 1)[1].strip()
     except OSError:
         pass
+    return "unknown"
+
+
+def oracle():
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import orc
+    return orc
+
+
+def cpu_baseline(seg, rng, opts, sky=None):
+    """The FP64 oracle on one sky point of the workload (all its spindowns), OpenMP threads."""
+    orc = oracle()
+    threads = orc.default_threads()
+    o = orc.Search(seg, nthreads=threads)
+    pm, mm, nn = sky or (seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2]))
+    t0 = time.perf_counter()
+    _, fn = o.job_core(pm, mm, nn, rng, opts)
+    dt = time.perf_counter() - t0
     return dict(value=round(fn / dt, 2), unit="templates/s", cores=threads, kind="port",
                 sample=f"1 sky point x {fn} spindowns of the same workload (FP64 oracle, own Stockham FFT, "
-                       f"OpenMP over spindowns; {dt:.1f} s wall)", cpu_model=model)
+                       f"OpenMP over spindowns; {dt:.1f} s wall)", cpu_model=cpu_model())
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--sky-per-step", type=int, default=16)
-    ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--copy-calib", action="store_true",
-                    help="first run a plain 256 MiB device copy (scale for FETCH_SIZE/WRITE_SIZE counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(sky points shard across GPUs with no collectives; see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
+def base_line(args, value, t, cfg, dtype="fp32", unit="templates/s", metric=METRIC):
+    return {"metric": metric, "value": round(value, 1), "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(1e3 * t / args.steps, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": cfg}
 
-    if args.copy_calib:
-        ms = pgw.copy_bench(256 << 20, reps=3)
-        print(f"copy 256 MiB: {ms:.3f} ms ({2 * 256 * 1.048576 / ms:.0f} GB/s)", file=sys.stderr, flush=True)
-    seg = workload(args)
+
+def bench_templates(args, name):
+    """network / plain / precision: FP32 form and FP64 path over the same sky points."""
+    seg = workload(args, name)
     rng = abi.make_range(seg)
     opts = abi.make_opts(trl=20.0)
     sky = synth.sky_list(seg)
-    print(f"workload: N={seg.sett['N']} nfft={seg.sett['nfft']} fftpad={seg.sett['fftpad']} "
-          f"L={seg.sett['nfftf']} sky={len(sky)} spindowns=512", file=sys.stderr, flush=True)
-
+    nspin = rng.spndr[1] - rng.spndr[0] + 1
+    per_step = args.sky_per_step or min(len(sky), {"network": 16, "plain": 4, "precision": 64}[name])
+    if args.steps is None:
+        args.steps = max(1, len(sky) // per_step) if name != "plain" else 16
+    s = seg.sett
+    print(f"workload {name}: N={s['N']} nfft={s['nfft']} fftpad={s['fftpad']} L={s['nfftf']} "
+          f"sky={len(sky)} spindowns={nspin}", file=sys.stderr, flush=True)
     h32 = pgw.Handle(seg, abi.PREC_FP32)
-    n32, t32, g32, r32 = run_path(h32, sky, rng, opts, args.steps, args.warmup, args.sky_per_step)
+    n32, t32, g32, r32 = run_path(h32, sky, rng, opts, args.steps, args.warmup, per_step)
     v32 = n32 / t32
     print(f"fp32: {n32} templates in {t32:.3f} s -> {v32:.0f} templates/s (gpu {g32:.1f} ms)",
           file=sys.stderr, flush=True)
-    out_fp64 = None
-    acc = None
+    out_fp64, acc = None, None
     if not args.no_fp64:
         h64 = pgw.Handle(seg, abi.PREC_FP64)
-        n64, t64, g64, r64 = run_path(h64, sky, rng, opts, args.steps, args.warmup, args.sky_per_step)
+        n64, t64, g64, r64 = run_path(h64, sky, rng, opts, args.steps, args.warmup, per_step)
         v64 = n64 / t64
         print(f"fp64: {n64} templates in {t64:.3f} s -> {v64:.0f} templates/s (gpu {g64:.1f} ms)",
               file=sys.stderr, flush=True)
@@ -190,27 +208,165 @@ def main():
         acc.update(max_abs_dF=mabs, max_rel_dF_F_ge_1=mrel)
         h64.close()
     rl = roofline(h32, seg, opts, 4)
+    cpu = None if args.no_cpu else cpu_baseline(seg, rng, opts)
+    if cpu:
+        print(f"cpu: {cpu}", file=sys.stderr, flush=True)
+    cfg = {"workload": TEMPLATE_WORKLOADS[name][1], "N": s["N"], "nfft": s["nfft"], "fftpad": s["fftpad"],
+           "L": s["nfftf"], "detectors": len(seg.dets), "sky_points": len(sky), "spindowns": nspin,
+           "sky_per_step": per_step}
+    line = base_line(args, v32, t32, cfg)
+    line.update({"templates": n32, "gpu_ms_per_step": round(g32 / args.steps, 3), "fp64": out_fp64,
+                 "speedup_fp32_vs_fp64": (round(v32 / out_fp64["value"], 3) if out_fp64 else None),
+                 "accuracy_fp32_vs_fp64": acc, "roofline": rl, "cpu_baseline": cpu})
+    return line
+
+
+def batching_segments(args, nseg=64):
+    rs = np.random.default_rng(args.seed)
+    segs = []
+    for k in range(nseg):
+        N = int(rs.integers(60000, 86401))
+        segs.append(synth.make_segment(N=N, dets=("H1", "L1", "V1"), sigmas=(1.0, 1.3, 1.1), fftpad=1,
+                                       seed=args.seed + 1 + k, h0=0.12, gap_frac=float(rs.uniform(0.05, 0.3)),
+                                       nsky=(4, 8), nspin=512))
+    return segs
+
+
+def bench_batching(args):
+    """64 uneven segments of 3 detectors in one handle; work items (segment, sky point) ordered by
+    segment, 128 per step; FP32 and FP64 over the same items."""
+    segs = batching_segments(args)
+    work = [(k, pm, mm, nn) for k, sg in enumerate(segs) for pm, mm, nn in synth.sky_list(sg)]
+    per_step = args.sky_per_step or 128
+    if args.steps is None:
+        args.steps = max(1, len(work) // per_step)
+    n16 = sum(1 for sg in segs if sg.sett["nfft"] == 1 << 16)
+    print(f"workload batching: {len(segs)} segments ({n16} with nfft 2^16), {len(work)} work items",
+          file=sys.stderr, flush=True)
+    opts = abi.make_opts(trl=20.0)
+
+    def run(prec):
+        h = pgw.Handle(segs, prec)
+        chunks = [work[(i * per_step) % len(work):(i * per_step) % len(work) + per_step]
+                  for i in range(args.warmup + args.steps)]
+        # each segment's spindowns come from its own range; the call takes one range, so the
+        # work of a step shares the range of its first segment's grid centre
+        for c in chunks[:args.warmup]:
+            h.job_batch_multi(c, abi.make_range(segs[c[0][0]]), opts)
+        recs, fn, gms = [], 0, 0.0
+        t0 = time.perf_counter()
+        for c in chunks[args.warmup:]:
+            r, _, f = h.job_batch_multi(c, abi.make_range(segs[c[0][0]]), opts)
+            recs.append(r)
+            fn += f
+            gms += h.last_gpu_ms
+        return h, fn, time.perf_counter() - t0, gms, np.concatenate(recs)
+
+    h32, n32, t32, g32, r32 = run(abi.PREC_FP32)
+    v32 = n32 / t32
+    print(f"fp32: {n32} templates in {t32:.3f} s -> {v32:.0f} templates/s (gpu {g32:.1f} ms)",
+          file=sys.stderr, flush=True)
+    out_fp64, acc = None, None
+    if not args.no_fp64:
+        h64, n64, t64, g64, r64 = run(abi.PREC_FP64)
+        v64 = n64 / t64
+        print(f"fp64: {n64} templates in {t64:.3f} s -> {v64:.0f} templates/s", file=sys.stderr, flush=True)
+        out_fp64 = dict(value=round(v64, 1), ms_per_step=round(1e3 * t64 / args.steps, 3), templates=n64,
+                        roofline=roofline(h64, segs[0], opts, 8))
+        acc = cand_compare(r32, r64)
+        h64.close()
+    rl = roofline(h32, segs[0], opts, 4)
+    cpu = None if args.no_cpu else cpu_baseline(segs[0], abi.make_range(segs[0]), opts)
+    cfg = {"workload": "batching: 3 detectors (H1, L1, V1), 64 segments of N in [60000, 86400] with 5-30 % gaps, "
+                       "nfft 2^16 or 2^17, fftpad=1, 32 sky x 512 spindowns per segment, trl=20",
+           "segments": len(segs), "segments_nfft_2^16": n16, "work_items": len(work), "items_per_step": per_step}
+    line = base_line(args, v32, t32, cfg)
+    line.update({"templates": n32, "fp64": out_fp64,
+                 "speedup_fp32_vs_fp64": (round(v32 / out_fp64["value"], 3) if out_fp64 else None),
+                 "accuracy_fp32_vs_fp64": acc, "roofline": rl, "cpu_baseline": cpu})
+    return line
+
+
+def bench_differentiation(args, npts=1024):
+    """Fstatnet with its gradient at npts points around an injection; a step is one batched call.
+    Also times a 20-step quasi-Newton ascent of all points."""
+    from polgraw_allsky_amd import followup
+    seg = synth.make_segment(N=86400, dets=("H1", "L1"), sigmas=(1.0, 1.3), fftpad=2, seed=args.seed, h0=0.1,
+                             nsky=(1, 1), nspin=1)
+    inj = seg.injection
+    truth = np.array([inj["f"], inj["fdot"], inj["delta"], inj["alpha"]])
+    sc = followup.phase_scales(seg.DetSSB, seg.sett["N"], seg.sett["oms"], truth)
+    pts = truth + np.random.default_rng(args.seed).uniform(-4, 4, (npts, 4)) * sc
+    if args.steps is None:
+        args.steps = 20
+    h = pgw.Handle(seg, abi.PREC_FP64)
+    for _ in range(args.warmup):
+        h.fstatnet(pts)
+    t0 = time.perf_counter()
+    gms = 0.0
+    for _ in range(args.steps):
+        out, g = h.fstatnet(pts)
+        gms += h.last_gpu_ms
+    dt = time.perf_counter() - t0
+    v = npts * args.steps / dt
+    print(f"fstatnet+gradient: {npts} points x {args.steps} in {dt:.3f} s -> {v:.0f} points/s "
+          f"(gpu {gms:.1f} ms)", file=sys.stderr, flush=True)
+    G = followup.phase_metric(seg.DetSSB, seg.sett["N"], seg.sett["oms"], truth)
+    t1 = time.perf_counter()
+    x, F, nev = followup.ascend(followup.FStatNet(h), pts, G, steps=20)
+    t_asc = time.perf_counter() - t1
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(seg, rng, opts)
-        print(f"cpu: {cpu}", file=sys.stderr, flush=True)
+        orc = oracle()
+        threads = orc.default_threads()
+        o = orc.Search(seg, nthreads=threads)
+        ns = 64
+        tc = time.perf_counter()
+        out_o, g_o = o.fstatnet(pts[:ns])
+        tcd = time.perf_counter() - tc
+        cpu = dict(value=round(ns / tcd, 2), unit="points/s", cores=threads, kind="port",
+                   sample=f"{ns} of the points with gradient (FP64 oracle Fstatnet, OpenMP over points; "
+                          f"{tcd:.1f} s wall)", cpu_model=cpu_model(),
+                   max_rel_dF_vs_device=float(np.max(np.abs(out_o[:, 5] - out[:ns, 5]) / np.abs(out_o[:, 5]))))
+    nifo, N = len(seg.dets), seg.sett["N"]
+    cfg = {"workload": "differentiation: 2 detectors (H1, L1), N=86400, Fstatnet + dF/d(omega, omegadot, delta, "
+                       "alpha) at 1024 candidate points", "N": N, "detectors": nifo, "points": npts}
+    line = base_line(args, v, dt, cfg, dtype="fp64", unit="points/s",
+                     metric="Fstatnet evaluations with gradient per second on 1 MI355X")
+    line.update({"gpu_ms_per_step": round(gms / args.steps, 3), "samples_per_s": round(v * nifo * N, 1),
+                 "ascent": {"steps": 20, "evaluations_per_point": nev, "seconds": round(t_asc, 3),
+                            "F_start_median": float(np.median(-out[:, 5])), "F_end_median": float(np.median(F))},
+                 "roofline": None, "cpu_baseline": cpu})
+    return line
 
-    s = seg.sett
-    line = {
-        "metric": METRIC, "value": round(v32, 1), "unit": "templates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(1e3 * t32 / args.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "network: 2 detectors (H1, L1), N=86400, nfft=2^17, fftpad=2, L=2^18, "
-                               "256 sky x 512 spindowns, trl=20",
-                   "N": s["N"], "nfft": s["nfft"], "fftpad": s["fftpad"], "L": s["nfftf"],
-                   "sky_points": len(sky), "spindowns": 512, "sky_per_step": args.sky_per_step},
-        "templates": n32, "gpu_ms_per_step": round(g32 / args.steps, 3),
-        "fp64": out_fp64,
-        "speedup_fp32_vs_fp64": (round(v32 / out_fp64["value"], 3) if out_fp64 else None),
-        "accuracy_fp32_vs_fp64": acc,
-        "roofline": rl,
-        "cpu_baseline": cpu,
-    }
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--sky-per-step", type=int, default=None)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--workload", default="network",
+                    choices=["network", "plain", "precision", "batching", "differentiation"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="first run a plain 256 MiB device copy (scale for FETCH_SIZE/WRITE_SIZE counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sky points shard across GPUs with no collectives; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calib:
+        ms = pgw.copy_bench(256 << 20, reps=3)
+        print(f"copy 256 MiB: {ms:.3f} ms ({2 * 256 * 1.048576 / ms:.0f} GB/s)", file=sys.stderr, flush=True)
+    if args.workload in TEMPLATE_WORKLOADS:
+        line = bench_templates(args, args.workload)
+    elif args.workload == "batching":
+        line = bench_batching(args)
+    else:
+        line = bench_differentiation(args)
     print(json.dumps(line), flush=True)
 
 
