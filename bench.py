@@ -115,6 +115,18 @@ def measured_traffic(kernel, prec):
     return None
 
 
+def measured_flops(kernel):
+    """FP64 FLOPs per (point, sample) of a kernel from the committed rocprofv3 counter summary
+    (profiles/*.md lines 'bench-flops: {json}'), or None."""
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            if line.startswith("bench-flops:"):
+                d = json.loads(line.split(":", 1)[1])
+                if d.get("kernel") == kernel:
+                    return d["flops_per_sample"]
+    return None
+
+
 def roofline(h, seg, opts, prec_bytes, nt=128, reps=10):
     s = seg.sett
     pm, mm, nn = seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2])
@@ -329,6 +341,14 @@ def bench_differentiation(args, npts=1024):
                           f"{tcd:.1f} s wall)", cpu_model=cpu_model(),
                    max_rel_dF_vs_device=float(np.max(np.abs(out_o[:, 5] - out[:ns, 5]) / np.abs(out_o[:, 5]))))
     nifo, N = len(seg.dets), seg.sett["N"]
+    fps = measured_flops("k_fstatnet_part")
+    rl = None
+    if fps:
+        # FP64 vector roofline: counter-measured FLOPs per (point, sample) x samples per call over
+        # the device time of a call (HIP events around the two kernels and the small copies)
+        ach = fps * npts * nifo * N / (gms / args.steps * 1e-3) / 1e12
+        rl = dict(bound="fp64-valu", kernel="k_fstatnet_part", achieved=round(ach, 2), peak=78.6, unit="TFLOP/s",
+                  frac=round(ach / 78.6, 4), traffic=None, flops_per_sample=fps)
     cfg = {"workload": "differentiation: 2 detectors (H1, L1), N=86400, Fstatnet + dF/d(omega, omegadot, delta, "
                        "alpha) at 1024 candidate points", "N": N, "detectors": nifo, "points": npts}
     line = base_line(args, v, dt, cfg, dtype="fp64", unit="points/s",
@@ -336,7 +356,7 @@ def bench_differentiation(args, npts=1024):
     line.update({"gpu_ms_per_step": round(gms / args.steps, 3), "samples_per_s": round(v * nifo * N, 1),
                  "ascent": {"steps": 20, "evaluations_per_point": nev, "seconds": round(t_asc, 3),
                             "F_start_median": float(np.median(-out[:, 5])), "F_end_median": float(np.median(F))},
-                 "roofline": None, "cpu_baseline": cpu})
+                 "roofline": rl, "cpu_baseline": cpu})
     return line
 
 
