@@ -331,7 +331,7 @@ template <> struct PhaseRot<float> {
 };
 
 struct RowParams {
-  int L2, nmin, nmax, nblk, nparts;
+  int L2, nmin, nmax, nblk, nparts, lgL2;  // L2 = 1 << lgL2
 };
 
 struct Cand {

@@ -96,8 +96,22 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
 // Row pass + F-statistic + FStat partials (jobcore.c:629-651, auxi.c:214-240): grid (L2, ntemplates),
 // P1/16 threads.  F(k), k = L2 k1 + k2 in [nmin, nmax), at Ft[t][k2][k1]; psum/pmax[t][part][k2].
 // NA = 2 transforms both spectra together; NA = 1 one after the other through the same LDS.
+// minimum waves per SIMD for the row pass (register budget 512 / W VGPRs per lane), and whether
+// the second spectrum's row is loaded into registers before the first transform (NA = 1)
+template <typename T, int P1, int NA> struct RowOcc {
+  static constexpr int W = 1;
+  static constexpr bool PREF = true;
+};
+#ifndef PGW_ROW_VARIANT
+#define PGW_ROW_VARIANT 0
+#endif
+template <> struct RowOcc<float, 4096, 1> {
+  static constexpr int W = PGW_ROW_VARIANT == 0 ? 1 : 4;
+  static constexpr bool PREF = PGW_ROW_VARIANT != 2;
+};
+
 template <typename T, int P1, int NA>
-__global__ __launch_bounds__(P1 / 16) void k_spin_row2(RowParams p, const double* __restrict__ aabb_slots,
+__global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(RowParams p, const double* __restrict__ aabb_slots,
                                                       const TmplDesc* __restrict__ desc,
                                                       const cplx<T>* __restrict__ Z,
                                                       const cplx<T>* __restrict__ twr, T* __restrict__ Ft,
@@ -111,7 +125,9 @@ __global__ __launch_bounds__(P1 / 16) void k_spin_row2(RowParams p, const double
   const T2* rowa = Z + (size_t)t * 2 * L + (size_t)k2 * P1;
   const T2* rowb = rowa + L;
   const double* aabb = aabb_slots + 2 * desc[t].slot;
-  const T aa = (T)ld_global(aabb, 0), bb = (T)ld_global(aabb, 1);
+  // F = |xa|^2/aa + |xb|^2/bb (jobcore.c:635-638) with the two reciprocals taken once: each
+  // product differs from the quotient by at most one rounding
+  const T raa = (T)(1.0 / ld_global(aabb, 0)), rbb = (T)(1.0 / ld_global(aabb, 1));
   T F[16];
   if constexpr (NA == 2) {
     T2 x[2][16];
@@ -120,22 +136,25 @@ __global__ __launch_bounds__(P1 / 16) void k_spin_row2(RowParams p, const double
     row_fft<T2, P1, 2, -1>(x, lds, twr);
 #pragma unroll
     for (int m = 0; m < 16; m++)
-      F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) / aa + (x[1][m].x * x[1][m].x + x[1][m].y * x[1][m].y) / bb;
+      F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * raa + (x[1][m].x * x[1][m].x + x[1][m].y * x[1][m].y) * rbb;
   } else {
-    T2 x[1][16], y[1][16];
+    constexpr bool PREF = RowOcc<T, P1, NA>::PREF;
+    T2 x[1][16], y[1][PREF ? 16 : 1];
 #pragma unroll
     for (int m = 0; m < 16; m++) x[0][m] = rowa[tid + NT * m];
+    if constexpr (PREF) {
 #pragma unroll
-    for (int m = 0; m < 16; m++) y[0][m] = rowb[tid + NT * m];  // in flight during the first FFT
+      for (int m = 0; m < 16; m++) y[0][m] = rowb[tid + NT * m];  // in flight during the first FFT
+    }
     row_fft<T2, P1, 1, -1>(x, lds, twr);
 #pragma unroll
-    for (int m = 0; m < 16; m++) F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) / aa;
+    for (int m = 0; m < 16; m++) F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * raa;
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < 16; m++) x[0][m] = y[0][m];
+    for (int m = 0; m < 16; m++) x[0][m] = PREF ? y[0][PREF ? m : 0] : rowb[tid + NT * m];
     row_fft<T2, P1, 1, -1>(x, lds, twr);
 #pragma unroll
-    for (int m = 0; m < 16; m++) F[m] = F[m] + (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) / bb;
+    for (int m = 0; m < 16; m++) F[m] = F[m] + (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * rbb;
   }
   __syncthreads();
   T* Fs = reinterpret_cast<T*>(lds);
@@ -149,24 +168,50 @@ __global__ __launch_bounds__(P1 / 16) void k_spin_row2(RowParams p, const double
     Fs[k1] = ok ? F[m] : (T)0;
   }
   __syncthreads();
-  const int w = tid >> 6, lane = tid & 63;
-  for (int part = w; part < p.nparts; part += NT / 64) {
+  // FStat block partials of this row: block b holds bins k in [nmin + 4096 b, nmin + 4096 (b+1))
+  // (the last one up to nmax), i.e. 4096 / L2 >= 16 consecutive k1.  Thread tid sums its 16
+  // consecutive bins j = 16 tid + i, which fall into at most two blocks b0 <= b1 (bins outside
+  // [nmin, nmax) are 0 and clamped into the first or last block), ...
+  T* rs = Fs + P1;   // [2][NT] sums, then [2][NT] maxima
+  {
+    const int j0 = 16 * tid;
+    const int nb = p.nparts - 1;
+    const int b0 = min(max((p.L2 * j0 + k2 - p.nmin) >> 12, 0), nb);
+    // first bin of block b0 + 1 (none when b0 is the last block)
+    const int jsplit = b0 == nb ? P1 : (PGW_NAV_DEV * (b0 + 1) + p.nmin - k2 + p.L2 - 1) >> p.lgL2;
+    T s0 = 0, s1 = 0, m0 = 0, m1 = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int j = j0 + ((i + tid) & 15);  // rotated start: lanes spread over the LDS banks
+      const T f = Fs[j];
+      if (j < jsplit) { s0 += f; m0 = f > m0 ? f : m0; }
+      else { s1 += f; m1 = f > m1 ? f : m1; }
+    }
+    rs[tid] = s0;
+    rs[NT + tid] = s1;
+    rs[2 * NT + tid] = m0;
+    rs[3 * NT + tid] = m1;
+  }
+  __syncthreads();
+  // ... and one thread per block adds the threads whose bins cover it, in thread order.
+  for (int part = tid; part < p.nparts; part += NT) {
     const int lok = p.nmin + PGW_NAV_DEV * part;
     const int hik = (part < p.nblk) ? lok + PGW_NAV_DEV : p.nmax;
-    const int jlo = max(0, (lok - k2 + p.L2 - 1) / p.L2);
-    const int jhi = min((hik - k2 + p.L2 - 1) / p.L2, P1);
+    const int jlo = max(0, (lok - k2 + p.L2 - 1) >> p.lgL2);
+    const int jhi = min((hik - k2 + p.L2 - 1) >> p.lgL2, P1);
     T sum = 0, mx = 0;
-    for (int j = jlo + lane; j < jhi; j += 64) {
-      const T f = Fs[j];
-      sum += f;
-      mx = f > mx ? f : mx;
+    if (jlo < jhi) {
+      const int nb = p.nparts - 1;
+      for (int u = jlo >> 4; u <= (jhi - 1) >> 4; u++) {
+        const int b0 = min(max((p.L2 * (16 * u) + k2 - p.nmin) >> 12, 0), nb);
+        const int h = b0 == part ? 0 : 1;
+        sum += rs[h * NT + u];
+        const T v = rs[(2 + h) * NT + u];
+        mx = v > mx ? v : mx;
+      }
     }
-    sum = wave_sum(sum);
-    mx = wave_max(mx);
-    if (lane == 0) {
-      psum[((size_t)t * p.nparts + part) * p.L2 + k2] = sum;
-      pmax[((size_t)t * p.nparts + part) * p.L2 + k2] = mx;
-    }
+    psum[((size_t)t * p.nparts + part) * p.L2 + k2] = sum;
+    pmax[((size_t)t * p.nparts + part) * p.L2 + k2] = mx;
   }
 }
 

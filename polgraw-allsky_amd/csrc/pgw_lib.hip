@@ -562,7 +562,7 @@ template <typename T> struct Engine final : pgw_handle {
     }
     launch_spin_col<T>(*g.ps, sb, d, Z.p, nt, hi.p, lo.p, col_seq, stream);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));
-    RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts};
+    RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2)};
     launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, stream);
     if (ev) HIPCHK(hipEventRecord(ev[2], stream));
     ThrParams tp{g.ps->L1, g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, o->white_flag, t0, kCandCap, o->trl};
