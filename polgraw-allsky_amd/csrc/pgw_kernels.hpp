@@ -280,29 +280,33 @@ struct TmplDesc {
 static_assert(sizeof(TmplDesc) == 32, "descriptor layout");
 
 // Sky-slot bases: xm cplx<T>[slot][nifo][{a,b}][N], shft double[slot][nifo][N], aabb double[slot][2].
-// het1 and sgnlt[1] of spindown ss (jobcore.c:456,466-467), uncontracted; host and device give
-// the same bits (fmod is exact).
+// het1 and sgnlt[1] of spindown ss (jobcore.c:456,466-467), uncontracted.
 struct SpinGrid {
   double M0, M4, M5, M9, M13;
   int mm, nn, ss0, s0_flag;
 };
-__host__ __device__ inline void spin_coeffs(const SpinGrid& q, int ss, double& het1, double& sgnlt1) {
+inline void spin_coeffs(const SpinGrid& q, int ss, double& het1, double& sgnlt1) {
 #pragma clang fp contract(off)
   sgnlt1 = (q.s0_flag ? 0. : ss * q.M5 + q.nn * q.M9 + q.mm * q.M13);
   het1 = fmod(ss * q.M4, q.M0);
   if (het1 < 0) het1 += q.M0;
 }
 
-// uslot >= 0: the launch's templates are consecutive spindowns ss0 + t (grid q) of the sky point in
-// slot uslot with data length uN, so the kernels take addresses and phase coefficients from these
-// arguments instead of waiting for the descriptor.
+// uslot >= 0: the launch's templates all belong to the sky point in slot uslot with data length
+// uN, so the kernels take addresses from these arguments instead of waiting for the descriptor.
 struct SlotBases {
   const void* xm;
   const double* shft;
   const double* aabb;
   unsigned long long xm_stride, shft_stride;  // elements per slot
   int nifo, uslot, uN, pad;
-  SpinGrid q;
+};
+
+// het1 and sgnlt[1] of the templates of a launch of at most kSpinTab templates, passed by value as
+// a kernel argument: a block reads its entry with one scalar load from the argument segment.
+constexpr int kSpinTab = 128;
+struct SpinTab {
+  double het1[kSpinTab], sgnlt1[kSpinTab];
 };
 
 struct SpinParams {

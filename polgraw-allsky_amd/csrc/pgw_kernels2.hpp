@@ -20,7 +20,8 @@ template <> struct ColE<double, 256> { static constexpr int E = 16; static const
 //   x_{a,b}[n] += xm[d][{a,b}][n] exp(-i phi_d(n)),  phi_d(n) = het1 n + sgnlt1 (n^2 + 2 n shft_d[n]),
 // zero for n >= N; the column FFT and the W_L^{n1 k2} twiddle follow; Z[t][{a,b}][k2][n1].
 template <typename T, int P2, bool SEQ, bool UNI>
-__global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, const TmplDesc* __restrict__ desc,
+__global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, const SpinTab tab,
+                                                   const TmplDesc* __restrict__ desc,
                                                    cplx<T>* __restrict__ Z, const cplx<T>* __restrict__ twc,
                                                    const cplx<T>* __restrict__ hi, const cplx<T>* __restrict__ lo) {
   using T2 = cplx<T>;
@@ -33,7 +34,8 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
   if constexpr (UNI) {
     slot = sb.uslot;
     N = sb.uN;
-    spin_coeffs(sb.q, sb.q.ss0 + t, het1, sgnlt1);
+    het1 = tab.het1[t];
+    sgnlt1 = tab.sgnlt1[t];
   } else {
     const TmplDesc& dsc = desc[t];
     slot = dsc.slot;

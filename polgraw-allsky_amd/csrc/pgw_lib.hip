@@ -160,12 +160,12 @@ void launch_row(const Plan<T>& pl, cplx<T>* Z, int nbatch, hipStream_t st) {
 }
 
 template <typename T>
-void launch_spin_col(const Plan<T>& pl, const SlotBases& sb, const TmplDesc* desc, cplx<T>* Z, int nt, const cplx<T>* hi,
-                     const cplx<T>* lo, bool seq, hipStream_t st) {
+void launch_spin_col(const Plan<T>& pl, const SlotBases& sb, const SpinTab& tab, const TmplDesc* desc, cplx<T>* Z,
+                     int nt, const cplx<T>* hi, const cplx<T>* lo, bool seq, hipStream_t st) {
   const SpinParams sp{pl.L1, pl.twshift};
 #define PGW_SCOL_GO(P2, SQ, U)                                                                           \
   hipLaunchKernelGGL((k_spin_col2<T, P2, SQ, U>), dim3(pl.L1 / col_tile<T, P2>(), nt), dim3(256), 0, st, sp, sb, \
-                     desc, Z, pl.twc.p, hi, lo)
+                     tab, desc, Z, pl.twc.p, hi, lo)
 #define PGW_SCOL_CASE(P2)                                                                                \
   case P2:                                                                                               \
     if (seq || ColE<T, P2>::SEQ) {                                                                       \
@@ -554,13 +554,15 @@ template <typename T> struct Engine final : pgw_handle {
                        hipEvent_t* ev = nullptr) {
     const TmplDesc* d = desc.p + (size_t)buf * kDescCap + t0;
     if (ev) HIPCHK(hipEventRecord(ev[0], stream));
-    SlotBases sb{xm_slots.p, shft_slots.p, aabb_slots.p, slot_xm, slot_shft, nifo, -1, 0, 0, SpinGrid{}};
-    if (uni) {
+    SlotBases sb{xm_slots.p, shft_slots.p, aabb_slots.p, slot_xm, slot_shft, nifo, -1, 0, 0};
+    SpinTab tab;
+    if (uni && nt <= kSpinTab) {
       sb.uslot = uni->slot;
       sb.uN = segs[uni->seg]->sett.N;
-      sb.q = spin_grid(segs[uni->seg]->sett.M, uni->mm, uni->nn, ss0, o->s0_flag);
+      const SpinGrid q = spin_grid(segs[uni->seg]->sett.M, uni->mm, uni->nn, ss0, o->s0_flag);
+      for (int i = 0; i < nt; i++) spin_coeffs(q, ss0 + i, tab.het1[i], tab.sgnlt1[i]);
     }
-    launch_spin_col<T>(*g.ps, sb, d, Z.p, nt, hi.p, lo.p, col_seq, stream);
+    launch_spin_col<T>(*g.ps, sb, tab, d, Z.p, nt, hi.p, lo.p, col_seq, stream);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));
     RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2)};
     launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, stream);
