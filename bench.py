@@ -115,6 +115,18 @@ def measured_traffic(kernel, prec):
     return None
 
 
+def measured_valu(kernel, prec):
+    """VALU busy fraction of a kernel from the committed rocprofv3 counter summary
+    (profiles/*.md lines 'bench-valu: {json}'), or None."""
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            if line.startswith("bench-valu:"):
+                d = json.loads(line.split(":", 1)[1])
+                if d.get("kernel") == kernel and d.get("precision") == prec:
+                    return d["valu_busy"]
+    return None
+
+
 def measured_flops(kernel):
     """FP64 FLOPs per (point, sample) of a kernel from the committed rocprofv3 counter summary
     (profiles/*.md lines 'bench-flops: {json}'), or None."""
@@ -143,10 +155,14 @@ def roofline(h, seg, opts, prec_bytes, nt=128, reps=10):
     achieved = b / (ms[k] * 1e-3) / 1e9
     stage_ms = ms["col"] + ms["row"] + ms["thr"]
     b_t = (b_col + b_row) * nt
-    tr = measured_traffic(kname, "fp32" if prec_bytes == 4 else "fp64")
+    prec = "fp32" if prec_bytes == 4 else "fp64"
+    tr = measured_traffic(kname, prec)
+    # the kernels also issue close to the VALU limit: counter-measured fraction of VALU issue
+    # cycles (SQ_INSTS_VALU x 4 / SIMD cycles) from the committed profile
     return dict(bound="hbm", kernel=kname, achieved=round(achieved, 1),
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
                 traffic=(int(tr * nt) if tr else None), traffic_unit="bytes per launch",
+                valu_busy=measured_valu(kname, prec),
                 templates_per_launch=nt, ms_per_launch=ms, bytes_per_template=dict(col=b_col, row=b_row),
                 template_stage_GBps=round(b_t / (stage_ms * 1e-3) / 1e9, 1))
 
