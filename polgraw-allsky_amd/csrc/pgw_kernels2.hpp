@@ -104,14 +104,6 @@ template <typename T, int P1, int NA> struct RowOcc {
   static constexpr int W = 1;
   static constexpr bool PREF = true;
 };
-#ifndef PGW_ROW_VARIANT
-#define PGW_ROW_VARIANT 0
-#endif
-template <> struct RowOcc<float, 4096, 1> {
-  static constexpr int W = PGW_ROW_VARIANT == 0 ? 1 : 4;
-  static constexpr bool PREF = PGW_ROW_VARIANT != 2;
-};
-
 template <typename T, int P1, int NA>
 __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(RowParams p, const double* __restrict__ aabb_slots,
                                                       const TmplDesc* __restrict__ desc,
