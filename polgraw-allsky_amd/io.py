@@ -1,0 +1,158 @@
+"""The reference's file formats on either side of the search core.
+
+Inputs of a frame (one time segment) as init_arrays()/read_grid() read them
+(search/network/src-cpu/init.c:287-457, settings.c:134-232):
+
+  <dtaprefix>/<ident %03d>/grid.bin                         int fftpad, 16 doubles M (column-major)
+  <dtaprefix>/<ident %03d>/<IFO>/xdatc_<ident>_<band><label>.bin   N doubles, zeros mark gaps
+  <dtaprefix>/<ident %03d>/<IFO>/DetSSB.bin                  3N doubles (detector position w.r.t. the
+                                                              solar-system barycentre, units of c dt),
+                                                              then phir, then epsm
+  range file                                                 8 ints: spndr0 spndr1 nr0 nr1 mr0 mr1 pmr0 pmr1
+
+Output: triggers_<ident %03d>_<band %04d><label>_<pm>.bin, records of 5 doubles
+(jobcore.c:92-93,141), written by pgw_search.
+
+`load_frame` builds the same segment object the tests and the benchmark use (settings, grid
+matrix, detector arrays), with the settings, noise variances and amplitude-modulation
+coefficients from the library's own C entry points.
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+RAD_TO_DEG = 180.0 / math.pi
+C_OMEGA_R = 7.2921151467064e-5           # search/network/src-cpu/settings.h:31
+C_SIDDAY = 2.0 * math.pi / C_OMEGA_R     # settings.h:32
+
+# Detector sites: geographical latitude, longitude and orientation gamma [rad]
+# (search/network/src-cpu/settings.c:174-232)
+DETECTORS = {
+    "H1": ((46 + (27 + 18.528 / 60.0) / 60.0) / RAD_TO_DEG,
+           -(119 + (24 + 27.5657 / 60.0) / 60.0) / RAD_TO_DEG,
+           170.9994 / RAD_TO_DEG),
+    "L1": ((30 + (33 + 46.4196 / 60.0) / 60.0) / RAD_TO_DEG,
+           -(90 + (46 + 27.2654 / 60.0) / 60.0) / RAD_TO_DEG,
+           242.7165 / RAD_TO_DEG),
+    "V1": ((43.0 + 37.0 / 60.0 + 53.0880 / 3600.0) / RAD_TO_DEG,
+           (10.0 + 30.0 / 60.0 + 16.1885 / 3600.0) / RAD_TO_DEG,
+           (135.0 - (19.0 + 25.0 / 60.0 + 57.96 / 3600.0)) / RAD_TO_DEG),
+}
+
+
+def _read_exact(path, dtype, count: int) -> np.ndarray:
+    a = np.fromfile(path, dtype=dtype, count=count)
+    if len(a) != count:
+        raise ValueError(f"{path}: {len(a)} values, expected {count}")
+    return a
+
+
+def read_grid(path):
+    """grid.bin (init.c:306-315): (fftpad, M[16])."""
+    with open(path, "rb") as f:
+        head = f.read(4 + 16 * 8)
+    if len(head) != 4 + 16 * 8:
+        raise ValueError(f"{path}: short grid file")
+    fftpad = int(np.frombuffer(head[:4], dtype=np.int32)[0])
+    M = np.frombuffer(head[4:], dtype=np.float64).copy()
+    return fftpad, M
+
+
+def read_xdat(path, N: int) -> np.ndarray:
+    """xdatc_*.bin (init.c:349-352): the first N doubles."""
+    return _read_exact(path, np.float64, N)
+
+
+def read_detssb(path, N: int):
+    """DetSSB.bin (init.c:381-397): (DetSSB[3N], phir, epsm)."""
+    a = _read_exact(path, np.float64, 3 * N + 2)
+    return a[:3 * N].copy(), float(a[3 * N]), float(a[3 * N + 1])
+
+
+def read_range(path) -> dict:
+    """Range file (init.c:679-683): 8 integers spndr0 spndr1 nr0 nr1 mr0 mr1 pmr0 pmr1."""
+    v = [int(x) for x in Path(path).read_text().split()[:8]]
+    if len(v) != 8:
+        raise ValueError(f"{path}: expected 8 integers")
+    return dict(spndr=v[0:2], nr=v[2:4], mr=v[4:6], pmr=v[6:8])
+
+
+def read_triggers(path) -> np.ndarray:
+    """triggers_*.bin: (n, 5) records (frequency, spindown, delta, alpha, SNR)."""
+    a = np.fromfile(path, dtype=np.float64)
+    if len(a) % 5:
+        raise ValueError(f"{path}: not a whole number of 5-double records")
+    return a.reshape(-1, 5)
+
+
+def fpo_from_band(band: int, dt: float) -> float:
+    """Reference band frequency when -fpo is not given (init.c:232-240)."""
+    return 10.0 + 0.96875 * band * (0.5 / dt)
+
+
+def samples(nod: int, dt: float) -> int:
+    """N = round(nod C_SIDDAY / dt) with the reference's round (settings.c:32, auxi.h:14)."""
+    return int(math.floor(nod * C_SIDDAY / dt + 0.5))
+
+
+@dataclass
+class Frame:
+    """One frame's inputs; the fields of the synthetic Segment, so the same ABI builders apply."""
+    sett: dict
+    M: np.ndarray
+    dets: list
+    xDat: list
+    DetSSB: list
+    phir: list
+    epsm: float
+    amod: list
+    sig2: list
+    injection: dict = field(default_factory=dict)
+    range: dict = field(default_factory=dict)
+
+
+def _settings_dict(s) -> dict:
+    keys = ("fpo", "dt", "B", "oms", "omr", "Smin", "Smax", "nfft", "N", "nfftf", "nmax", "nmin", "nd",
+            "interpftpad", "fftpad", "Ninterp")
+    return {k: getattr(s, k) for k in keys}
+
+
+def load_frame(dtaprefix, ident: int, band: int, dt: float = 2.0, nod: int = 2, label: str = "",
+               fpo: float | None = None, dets=None, range_file=None) -> Frame:
+    """Reads a frame the way main() does (read_grid, search_settings, detectors_settings,
+    init_arrays, rogcvir and set_search_range; search/network/src-cpu/main.c:72-120).
+    dets: detector names in order (default: the subdirectories holding this band's data, sorted)."""
+    from . import pgw
+    base = Path(dtaprefix) / f"{ident:03d}"
+    xname = f"xdatc_{ident:03d}_{band:04d}{label}.bin"
+    if dets is None:
+        dets = sorted(d for d in os.listdir(base) if (base / d / xname).is_file())
+    if not dets:
+        raise FileNotFoundError(f"no detector data {xname} under {base}")
+    for d in dets:
+        if d not in DETECTORS:
+            raise ValueError(f"unknown detector {d}")
+    fftpad, M = read_grid(base / "grid.bin")
+    N = samples(nod, dt)
+    s = pgw.search_settings(fpo if fpo is not None else fpo_from_band(band, dt), dt, N, fftpad)
+    xs, ssb, phir, amod, sig2 = [], [], [], [], []
+    epsm = None
+    for d in dets:
+        x = read_xdat(base / d / xname, N)
+        e, ph, ep = read_detssb(base / d / "DetSSB.bin", N)
+        if epsm is not None and math.sin(ep) != math.sin(epsm):  # init.c:430-436
+            raise ValueError(f"epsm of {d} differs from the other detectors")
+        epsm = ep
+        xs.append(x)
+        ssb.append(e)
+        phir.append(ph)
+        amod.append(pgw.rogcvir(DETECTORS[d][0], DETECTORS[d][2]))
+        sig2.append(pgw.sig2(x))
+    rng = read_range(range_file) if range_file else {}
+    return Frame(sett=_settings_dict(s), M=M, dets=list(dets), xDat=xs, DetSSB=ssb, phir=phir, epsm=epsm,
+                 amod=amod, sig2=sig2, range=rng)

@@ -72,6 +72,30 @@ EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_j
             "pgw_destroy", "pgw_strerror"]
 
 
+def search_settings(fpo: float, dt: float, N: int, fftpad: int) -> abi.Settings:
+    """search_settings() sizes for a segment of N samples (pgw_search_settings)."""
+    s = abi.Settings()
+    st = lib().pgw_search_settings(fpo, dt, N, fftpad, C.byref(s))
+    if st != 0:
+        raise PGWError(st, "pgw_search_settings")
+    return s
+
+
+def rogcvir(ephi: float, egam: float) -> np.ndarray:
+    """Amplitude-modulation coefficients c1..c9 of a detector (pgw_rogcvir)."""
+    c = np.zeros(9)
+    st = lib().pgw_rogcvir(ephi, egam, c.ctypes.data_as(_dp))
+    if st != 0:
+        raise PGWError(st, "pgw_rogcvir")
+    return c
+
+
+def sig2(x) -> float:
+    """crf0 * var(xDat) (pgw_sig2)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return float(lib().pgw_sig2(x.ctypes.data_as(_dp), len(x)))
+
+
 def read_checkpoint(qname, rng):
     """read_checkpoints(): (status, FNum); status 1 = the state file is empty (run finished)."""
     fn = C.c_int(0)

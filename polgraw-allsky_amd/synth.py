@@ -22,6 +22,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from .io import DETECTORS
+
 RAD_TO_DEG = 180.0 / math.pi
 C_OMEGA_R = 7.2921151467064e-5          # settings.h:31
 C_EPSMA = 84381.448 / 3600.0 / RAD_TO_DEG  # settings.h:20
@@ -30,19 +32,8 @@ C_AU = 1.49597870691e8                  # km
 C_EARTH_R = 6378.140                    # km
 C_YEARSEC = 365.25 * 86400.0
 
-# Detector sites: latitude, longitude, orientation gamma [rad]
-# (search/network/src-cpu/settings.c:174-232; physical constants of the sites)
-SITES = {
-    "H1": ((46 + (27 + 18.528 / 60.0) / 60.0) / RAD_TO_DEG,
-           -(119 + (24 + 27.5657 / 60.0) / 60.0) / RAD_TO_DEG,
-           170.9994 / RAD_TO_DEG),
-    "L1": ((30 + (33 + 46.4196 / 60.0) / 60.0) / RAD_TO_DEG,
-           -(90 + (46 + 27.2654 / 60.0) / 60.0) / RAD_TO_DEG,
-           242.7165 / RAD_TO_DEG),
-    "V1": ((43.0 + 37.0 / 60.0 + 53.0880 / 3600.0) / RAD_TO_DEG,
-           (10.0 + 30.0 / 60.0 + 16.1885 / 3600.0) / RAD_TO_DEG,
-           (135.0 - (19.0 + 25.0 / 60.0 + 57.96 / 3600.0)) / RAD_TO_DEG),
-}
+# Detector sites: latitude, longitude, orientation gamma [rad] (settings.c:174-232)
+SITES = DETECTORS
 
 # grid.bin of testdata/test-gpu-cpu/001 (N=86164, nfft=2^17, fftpad=1), column-major
 _FIXTURE_N = 86164
