@@ -339,6 +339,11 @@ def bench_differentiation(args, npts=1024):
     v = npts * args.steps / dt
     print(f"fstatnet+gradient: {npts} points x {args.steps} in {dt:.3f} s -> {v:.0f} points/s "
           f"(gpu {gms:.1f} ms)", file=sys.stderr, flush=True)
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        h.fstatnet(pts, grad=False)
+    v_nograd = npts * args.steps / (time.perf_counter() - t2)
+    print(f"fstatnet values only: {v_nograd:.0f} points/s", file=sys.stderr, flush=True)
     G = followup.phase_metric(seg.DetSSB, seg.sett["N"], seg.sett["oms"], truth)
     t1 = time.perf_counter()
     x, F, nev = followup.ascend(followup.FStatNet(h), pts, G, steps=20)
@@ -370,6 +375,7 @@ def bench_differentiation(args, npts=1024):
     line = base_line(args, v, dt, cfg, dtype="fp64", unit="points/s",
                      metric="Fstatnet evaluations with gradient per second on 1 MI355X")
     line.update({"gpu_ms_per_step": round(gms / args.steps, 3), "samples_per_s": round(v * nifo * N, 1),
+                 "values_only_points_per_s": round(v_nograd, 1),
                  "ascent": {"steps": 20, "evaluations_per_point": nev, "seconds": round(t_asc, 3),
                             "F_start_median": float(np.median(-out[:, 5])), "F_end_median": float(np.median(F))},
                  "roofline": rl, "cpu_baseline": cpu})

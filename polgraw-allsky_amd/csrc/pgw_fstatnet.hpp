@@ -38,7 +38,9 @@ __device__ __forceinline__ double fsn_phase(double om, double omd, double oms, i
 
 // grid (npoints, nchunk), FSN_NT threads: block (p, c) sums samples [c FSN_CHUNK, (c+1) FSN_CHUNK)
 // of every detector for point p into part[p][c][FSN_NSUM].  Consecutive blocks are different points
-// of the same samples, so a chunk's inputs are read from HBM once and then from L2.
+// of the same samples, so a chunk's inputs are read from HBM once and then from L2.  GRAD = false
+// accumulates only the sums of F (0-3, 20, 21).
+template <bool GRAD>
 __global__ __launch_bounds__(FSN_NT) void k_fstatnet_part(FsnParams q, const double* __restrict__ pts,
                                                           const double* __restrict__ xDat,
                                                           const double* __restrict__ DetSSB,
@@ -80,6 +82,13 @@ __global__ __launch_bounds__(FSN_NT) void k_fstatnet_part(FsnParams q, const dou
                        cf[3] * c2sd * ss - cf[0] * (2. - c2d) + cf[4] * c2d;
       const double b = cf[5] * sindel * c2s + cf[6] * sindel * 2. * cs + cf[7] * cosdel * cc +
                        cf[8] * cosdel * ss - cf[5] * sindel;
+      const double xw = x[i] * D.w;
+      const double er = xw * cp, ei = -xw * sp;
+      const double ur = a * er, ui = a * ei, vr = b * er, vi = b * ei;
+      s[0] += ur; s[1] += ui; s[2] += vr; s[3] += vi;
+      s[20] += D.w * a * a;
+      s[21] += D.w * b * b;
+      if constexpr (!GRAD) continue;
       const double da_d = cf[0] * 2. * c2sd * c2s + cf[1] * 2. * c2sd * 2. * cs + cf[2] * cos2d * cc +
                           cf[3] * cos2d * ss - cf[0] * 2. * c2sd - 2. * cf[4] * c2sd;
       const double db_d = cf[5] * cosdel * c2s + cf[6] * cosdel * 2. * cs - cf[7] * sindel * cc -
@@ -88,11 +97,6 @@ __global__ __launch_bounds__(FSN_NT) void k_fstatnet_part(FsnParams q, const dou
                           cf[2] * c2sd * ss + cf[3] * c2sd * cc;
       const double db_a = cf[5] * sindel * (-4. * cs) + cf[6] * sindel * 2. * cdiff - cf[7] * cosdel * ss +
                           cf[8] * cosdel * cc;
-      // xw e^{-i phi}, and the terms of Xa, Xb
-      const double xw = x[i] * D.w;
-      const double er = xw * cp, ei = -xw * sp;
-      const double ur = a * er, ui = a * ei, vr = b * er, vi = b * ei;
-      s[0] += ur; s[1] += ui; s[2] += vr; s[3] += vi;
       // d phi / d theta: i + shft, i^2 + 2 i shft, K d shft / d delta, K d shft / d alpha
       const double di = (double)i;
       const double K = om + q.oms + 2. * omd * di;
@@ -107,8 +111,6 @@ __global__ __launch_bounds__(FSN_NT) void k_fstatnet_part(FsnParams q, const dou
       s[14] += ph1 * vi; s[15] -= ph1 * vr;
       s[16] += ph2 * vi + db_d * er; s[17] += -ph2 * vr + db_d * ei;
       s[18] += ph3 * vi + db_a * er; s[19] += -ph3 * vr + db_a * ei;
-      s[20] += D.w * a * a;
-      s[21] += D.w * b * b;
       s[22] += 2. * D.w * a * da_d;
       s[23] += 2. * D.w * a * da_a;
       s[24] += 2. * D.w * b * db_d;
@@ -119,13 +121,17 @@ __global__ __launch_bounds__(FSN_NT) void k_fstatnet_part(FsnParams q, const dou
   const int wv = tid / 64, ln = tid % 64;
 #pragma unroll
   for (int k = 0; k < FSN_NSUM; k++) {
+    if (!GRAD && !(k < 4 || k == 20 || k == 21)) continue;  // only the sums of F
     const double v = wave_sum(s[k]);
     if (ln == 0) red[wv][k] = v;
   }
   __syncthreads();
   if (tid < FSN_NSUM) {
-    double v = red[0][tid];
-    for (int w = 1; w < FSN_NT / 64; w++) v += red[w][tid];
+    double v = 0.;
+    if (GRAD || tid < 4 || tid == 20 || tid == 21) {
+      v = red[0][tid];
+      for (int w = 1; w < FSN_NT / 64; w++) v += red[w][tid];
+    }
     part[((size_t)p * q.nchunk + c) * FSN_NSUM + tid] = v;
   }
 }

@@ -770,8 +770,12 @@ template <typename T> struct Engine final : pgw_handle {
     for (int p0 = 0; p0 < n; p0 += kFsnBatch) {
       const int m = std::min(kFsnBatch, n - p0);
       HIPCHK(hipMemcpyAsync(fsn_pts.p, pts + 4 * (size_t)p0, sizeof(double) * 4 * m, hipMemcpyHostToDevice, stream));
-      hipLaunchKernelGGL(k_fstatnet_part, dim3(m, q.nchunk), dim3(FSN_NT), 0, stream, q, fsn_pts.p, g.xDat.p,
-                         g.DetSSB.p, g.cosmodf.p, g.sinmodf.p, fsn_part.p);
+      if (grad)
+        hipLaunchKernelGGL(k_fstatnet_part<true>, dim3(m, q.nchunk), dim3(FSN_NT), 0, stream, q, fsn_pts.p,
+                           g.xDat.p, g.DetSSB.p, g.cosmodf.p, g.sinmodf.p, fsn_part.p);
+      else
+        hipLaunchKernelGGL(k_fstatnet_part<false>, dim3(m, q.nchunk), dim3(FSN_NT), 0, stream, q, fsn_pts.p,
+                           g.xDat.p, g.DetSSB.p, g.cosmodf.p, g.sinmodf.p, fsn_part.p);
       hipLaunchKernelGGL(k_fstatnet_final, dim3((m + 63) / 64), dim3(64), 0, stream, m, q.nchunk, fsn_part.p,
                          fsn_pts.p, fsn_out.p, grad ? fsn_grad.p : nullptr);
       if (hipGetLastError() != hipSuccess) return PGW_ERR_DEVICE;
