@@ -4,7 +4,7 @@
 //   k_sky_prologue    modvir + shft + first demodulation    (jobcore.c:270-313, settings.c:281-325)
 //   k_fft_col/k_fft_row + loaders   resampling FFTs          (jobcore.c:319-362)
 //   k_untranspose     T-layout -> natural order, times interpftpad/Ninterp
-//   k_spline_u/k_spline_y2/k_spline_eval  natural cubic spline (auxi.c:55-122), /sig2 (jobcore.c:386-389)
+//   k_spline/k_spline_eval               natural cubic spline (auxi.c:55-122), /sig2 (jobcore.c:386-389)
 //   k_aabb            noise-weighted sums aa, bb              (jobcore.c:375-393)
 // Template stage (per spindown, the hot loop):
 //   k_spin_col        phase modulation fused into the column pass of the F-statistic FFT
@@ -189,49 +189,87 @@ struct SplineTab {
   double invp_inf, c_inf;
 };
 
-template <typename T>
-__global__ __launch_bounds__(NT) void k_spline_u(const cplx<T>* __restrict__ y, size_t ystride, int n,
-                                                 const SplineTab* __restrict__ tab, double2* __restrict__ u,
-                                                 size_t ustride) {
-  const int b = blockIdx.y;
-  const int a = (blockIdx.x * NT + threadIdx.x) * SPL_K;
-  if (a >= n - 1) return;
-  const int e = min(a + SPL_K, n - 1);
-  const cplx<T>* Y = y + (size_t)b * ystride;
-  double2* U = u + (size_t)b * ustride;
-  int i = (a - SPL_W <= 1) ? 1 : a - SPL_W;
-  if (a == 0) U[0] = make_double2(0.0, 0.0);
-  double ur = 0.0, ui = 0.0;
-  double ym_r = (double)Y[i - 1].x, ym_i = (double)Y[i - 1].y;
-  double y0_r = (double)Y[i].x, y0_i = (double)Y[i].y;
-  for (; i < e; i++) {
-    const double yp_r = (double)Y[i + 1].x, yp_i = (double)Y[i + 1].y;
-    const double invp = i < SPL_NTAB ? tab->invp[i] : tab->invp_inf;
-    const double dr = ym_r - 2. * y0_r + yp_r, di = ym_i - 2. * y0_i + yp_i;
-    ur = (-.5 * ur + 3. * dr) * invp;
-    ui = (-.5 * ui + 3. * di) * invp;
-    if (i >= a) U[i] = make_double2(ur, ui);
-    ym_r = y0_r; ym_i = y0_i; y0_r = yp_r; y0_i = yp_i;
-  }
-}
+// Both recursions of the spline for SPF_OUT consecutive outputs of one series per workgroup:
+// the input window is read into LDS with coalesced loads, u (forward) and then y2 (backward) run
+// in chunks of SPL_K per thread from LDS, each chunk starting SPL_W samples early from a zero
+// state, and y2 leaves through LDS with coalesced stores.  grid (ceil(n / SPF_OUT), series).
+// LDS arrays are padded by one element per 16 so the lanes' chunk starts fall on different banks.
+constexpr int SPF_OUT = NT * SPL_K;              // y2 outputs per workgroup
+constexpr int SPF_UW = SPF_OUT + SPL_W;          // u values per workgroup (outputs + warm-up above)
+constexpr int SPF_YW = SPF_OUT + 2 * SPL_W + 2;  // y samples per workgroup
+__device__ __forceinline__ constexpr int spad(int i) { return i + (i >> 4); }
 
-__global__ __launch_bounds__(NT) void k_spline_y2(const double2* __restrict__ u, size_t stride, int n,
-                                                  const SplineTab* __restrict__ tab, double2* __restrict__ y2) {
-  const int b = blockIdx.y;
-  const int a = (blockIdx.x * NT + threadIdx.x) * SPL_K;
-  if (a >= n) return;
-  const int e = min(a + SPL_K, n);  // outputs [a, e)
-  const double2* U = u + (size_t)b * stride;
+template <typename T>
+__global__ __launch_bounds__(NT) void k_spline(const cplx<T>* __restrict__ y, size_t ystride, int n,
+                                               const SplineTab* __restrict__ tab, double2* __restrict__ y2,
+                                               size_t stride) {
+  __shared__ cplx<T> Ys[spad(SPF_YW) + 1];
+  __shared__ double2 Us[spad(SPF_UW) + 1];
+  const int b = blockIdx.y, tid = threadIdx.x;
+  const int A0 = blockIdx.x * SPF_OUT;
+  const int A1 = min(A0 + SPF_OUT, n);         // y2 outputs [A0, A1)
+  const int U1 = min(A1 + SPL_W - 1, n - 2);   // u needed on [A0, U1]
+  const int i0 = max(1, A0 - SPL_W);           // first recursion index of any chunk
+  const int Ylo = i0 - 1, Yhi = U1 + 1;        // y window [Ylo, Yhi] (Yhi <= n - 1)
+  const cplx<T>* Y = y + (size_t)b * ystride;
   double2* Y2 = y2 + (size_t)b * stride;
-  if (e == n) { Y2[n - 1] = make_double2(0.0, 0.0); Y2[n] = make_double2(0.0, 0.0); }
-  int k = (e + SPL_W >= n - 1) ? n - 2 : e + SPL_W - 1;
-  double zr = 0.0, zi = 0.0;  // y2[k+1]
-  for (; k >= a; k--) {
-    const double c = k < SPL_NTAB ? tab->c[k] : tab->c_inf;
-    const double2 uk = U[k];
-    zr = c * zr + uk.x;
-    zi = c * zi + uk.y;
-    if (k < e) Y2[k] = make_double2(zr, zi);
+  for (int j = tid; j <= Yhi - Ylo; j += NT) Ys[spad(j)] = Y[Ylo + j];
+  if (A0 == 0 && tid == 0) Us[spad(0)] = make_double2(0.0, 0.0);  // u_0 = 0 (auxi.c:62)
+  __syncthreads();
+  // u_i = (-u_{i-1}/2 + 3 (y_{i-1} - 2 y_i + y_{i+1})) invp_i  (auxi.c:64-72)
+  for (int ch = tid; A0 + ch * SPL_K <= U1; ch += NT) {
+    const int ua = A0 + ch * SPL_K, ue = min(ua + SPL_K, U1 + 1);
+    int i = max(1, ua - SPL_W);
+    double ur = 0.0, ui = 0.0;
+    double ym_r = (double)Ys[spad(i - 1 - Ylo)].x, ym_i = (double)Ys[spad(i - 1 - Ylo)].y;
+    double y0_r = (double)Ys[spad(i - Ylo)].x, y0_i = (double)Ys[spad(i - Ylo)].y;
+    for (; i < ue; i++) {
+      const cplx<T> yp = Ys[spad(i + 1 - Ylo)];
+      const double yp_r = (double)yp.x, yp_i = (double)yp.y;
+      const double invp = i < SPL_NTAB ? tab->invp[i] : tab->invp_inf;
+      const double dr = ym_r - 2. * y0_r + yp_r, di = ym_i - 2. * y0_i + yp_i;
+      ur = (-.5 * ur + 3. * dr) * invp;
+      ui = (-.5 * ui + 3. * di) * invp;
+      if (i >= ua) Us[spad(i - A0)] = make_double2(ur, ui);
+      ym_r = y0_r; ym_i = y0_i; y0_r = yp_r; y0_i = yp_i;
+    }
+  }
+  __syncthreads();
+  // y2_k = c_k y2_{k+1} + u_k, y2_{n-1} = 0  (auxi.c:73-80), outputs [a, e) of this thread
+  const int a = A0 + tid * SPL_K, e = min(a + SPL_K, A1);
+  double2 out[SPL_K];
+  if (a < A1) {
+    int k = (e + SPL_W >= n - 1) ? n - 2 : e + SPL_W - 1;
+    double zr = 0.0, zi = 0.0;
+    for (; k >= a + SPL_K; k--) {  // warm-up above the chunk
+      const double c = k < SPL_NTAB ? tab->c[k] : tab->c_inf;
+      const double2 uk = Us[spad(k - A0)];
+      zr = c * zr + uk.x;
+      zi = c * zi + uk.y;
+    }
+#pragma unroll
+    for (int j = SPL_K - 1; j >= 0; j--) {
+      const int kk = a + j;
+      if (kk <= k) {
+        const double c = kk < SPL_NTAB ? tab->c[kk] : tab->c_inf;
+        const double2 uk = Us[spad(kk - A0)];
+        zr = c * zr + uk.x;
+        zi = c * zi + uk.y;
+      }
+      out[j] = make_double2(zr, zi);
+    }
+  }
+  __syncthreads();  // every u read is done: Us now stages y2
+  if (a < A1) {
+#pragma unroll
+    for (int j = 0; j < SPL_K; j++)
+      if (a + j < e) Us[spad(a + j - A0)] = out[j];
+  }
+  __syncthreads();
+  for (int j = tid; j < A1 - A0; j += NT) Y2[A0 + j] = Us[spad(j)];
+  if (A1 == n && tid == 0) {  // y2_{n-1} = 0 and the guard entry y2_n read when x == n-1
+    Y2[n - 1] = make_double2(0.0, 0.0);
+    Y2[n] = make_double2(0.0, 0.0);
   }
 }
 

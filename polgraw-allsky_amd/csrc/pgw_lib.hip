@@ -266,7 +266,7 @@ template <typename T> struct Engine final : pgw_handle {
   DevBuf<T2> hi, lo;
   // shared sky-stage scratch
   DevBuf<T2> R, Zf, Zi, Y;
-  DevBuf<double2> u, y2;
+  DevBuf<double2> y2;
   DevBuf<double> shftf, paa, pbb;
   DevBuf<SplineTab> spl;
   // sky slots
@@ -409,7 +409,7 @@ template <typename T> struct Engine final : pgw_handle {
     slot_shft = (size_t)nifo * Nmax;
     if ((st = R.alloc((size_t)2 * nifo * nfftmax)) || (st = Zf.alloc((size_t)2 * nifo * nfftmax)) ||
         (st = Zi.alloc((size_t)2 * nifo * Nimax)) || (st = Y.alloc((size_t)2 * nifo * Nimax)) ||
-        (st = u.alloc((size_t)2 * nifo * (nsplmax + 1))) || (st = y2.alloc((size_t)2 * nifo * (nsplmax + 1))) ||
+        (st = y2.alloc((size_t)2 * nifo * (nsplmax + 1))) ||
         (st = shftf.alloc((size_t)nifo * Nmax)) || (st = paa.alloc((size_t)nifo * npromax)) ||
         (st = pbb.alloc((size_t)nifo * npromax)) || (st = spl.alloc(1)) ||
         (st = xm_slots.alloc((size_t)kSlots * slot_xm)) || (st = shft_slots.alloc((size_t)kSlots * slot_shft)) ||
@@ -503,11 +503,8 @@ template <typename T> struct Engine final : pgw_handle {
     const T ft = (T)((double)S.interpftpad / S.Ninterp);
     hipLaunchKernelGGL((k_untranspose<T>), gt, dim3(NT), 0, stream, Zi.p, Y.p, g.pi->L1, g.pi->L2, count, ft);
     // spline on the first interpftpad*N samples, evaluated at interpftpad*(i - shftf[i]) (auxi.c:98-122)
-    const int nchunk = (g.nspl + SPL_K - 1) / SPL_K;
-    dim3 gs((nchunk + NT - 1) / NT, nb);
-    hipLaunchKernelGGL((k_spline_u<T>), gs, dim3(NT), 0, stream, Y.p, (size_t)S.Ninterp, g.nspl, spl.p, u.p,
-                       (size_t)(g.nspl + 1));
-    hipLaunchKernelGGL(k_spline_y2, gs, dim3(NT), 0, stream, u.p, (size_t)(g.nspl + 1), g.nspl, spl.p, y2.p);
+    hipLaunchKernelGGL((k_spline<T>), dim3((g.nspl + SPF_OUT - 1) / SPF_OUT, nb), dim3(NT), 0, stream, Y.p,
+                       (size_t)S.Ninterp, g.nspl, spl.p, y2.p, (size_t)(g.nspl + 1));
     dim3 ge((S.N + NT - 1) / NT, nb);
     hipLaunchKernelGGL((k_spline_eval<T, T>), ge, dim3(NT), 0, stream, Y.p, (size_t)S.Ninterp, y2.p,
                        (size_t)(g.nspl + 1), shftf.p, S.N, S.interpftpad, p, slot_xm_ptr(slot));
