@@ -106,9 +106,11 @@ __global__ __launch_bounds__(NT) void k_sky_prologue(SkyParams p, const double* 
 }
 
 // one block: aa = sum_d (sum_i aa_d^2)/sig2_d, bb likewise (jobcore.c:375-393)
-__global__ __launch_bounds__(NT) void k_aabb(SkyParams p, const double* __restrict__ paa,
-                                             const double* __restrict__ pbb, int nb, double* __restrict__ aabb) {
-  __shared__ double red[2][NT];
+// aa, bb (jobcore.c:375-393): the per-block partial sums of a^2, b^2 of every detector, summed in
+// a fixed order and divided by sig2; one workgroup, red = 2 x NT doubles of LDS.
+__device__ __forceinline__ void aabb_reduce(const SkyParams& p, const double* __restrict__ paa,
+                                            const double* __restrict__ pbb, int nb, double* __restrict__ aabb,
+                                            double (*red)[NT]) {
   double aa = 0.0, bb = 0.0;
   for (int d = 0; d < p.nifo; d++) {
     double sa = 0.0, sb = 0.0;
@@ -202,7 +204,8 @@ __device__ __forceinline__ constexpr int spad(int i) { return i + (i >> 4); }
 template <typename T>
 __global__ __launch_bounds__(NT) void k_spline(const cplx<T>* __restrict__ y, size_t ystride, int n,
                                                const SplineTab* __restrict__ tab, double2* __restrict__ y2,
-                                               size_t stride) {
+                                               size_t stride, SkyParams p, const double* __restrict__ paa,
+                                               const double* __restrict__ pbb, int npro, double* __restrict__ aabb) {
   __shared__ cplx<T> Ys[spad(SPF_YW) + 1];
   __shared__ double2 Us[spad(SPF_UW) + 1];
   const int b = blockIdx.y, tid = threadIdx.x;
@@ -270,6 +273,11 @@ __global__ __launch_bounds__(NT) void k_spline(const cplx<T>* __restrict__ y, si
   if (A1 == n && tid == 0) {  // y2_{n-1} = 0 and the guard entry y2_n read when x == n-1
     Y2[n - 1] = make_double2(0.0, 0.0);
     Y2[n] = make_double2(0.0, 0.0);
+  }
+  // the sky point's aa, bb, by the first workgroup (saves a launch)
+  if (blockIdx.x == 0 && blockIdx.y == 0) {
+    __shared__ double red[2][NT];
+    aabb_reduce(p, paa, pbb, npro, aabb, red);
   }
 }
 

@@ -489,7 +489,6 @@ template <typename T> struct Engine final : pgw_handle {
     double* shft = slot_shft_ptr(slot);
     hipLaunchKernelGGL((k_sky_prologue<T>), dim3(g.npro, nifo), dim3(NT), 0, stream, p, g.xDat.p, g.DetSSB.p,
                        g.cosmodf.p, g.sinmodf.p, R.p, shft, shftf.p, paa.p, pbb.p);
-    hipLaunchKernelGGL(k_aabb, dim3(1), dim3(NT), 0, stream, p, paa.p, pbb.p, g.npro, slot_aabb_ptr(slot));
     // resampling: forward FFT (nfft), zero-insert, backward FFT (Ninterp), scale (jobcore.c:319-362)
     LoadPlain<T2> lp{R.p, S.nfft, S.nfft};
     launch_col<T, -1>(*g.pf, lp, Zf.p, nb, hi.p, lo.p, stream);
@@ -504,7 +503,8 @@ template <typename T> struct Engine final : pgw_handle {
     hipLaunchKernelGGL((k_untranspose<T>), gt, dim3(NT), 0, stream, Zi.p, Y.p, g.pi->L1, g.pi->L2, count, ft);
     // spline on the first interpftpad*N samples, evaluated at interpftpad*(i - shftf[i]) (auxi.c:98-122)
     hipLaunchKernelGGL((k_spline<T>), dim3((g.nspl + SPF_OUT - 1) / SPF_OUT, nb), dim3(NT), 0, stream, Y.p,
-                       (size_t)S.Ninterp, g.nspl, spl.p, y2.p, (size_t)(g.nspl + 1));
+                       (size_t)S.Ninterp, g.nspl, spl.p, y2.p, (size_t)(g.nspl + 1), p, paa.p, pbb.p, g.npro,
+                       slot_aabb_ptr(slot));
     dim3 ge((S.N + NT - 1) / NT, nb);
     hipLaunchKernelGGL((k_spline_eval<T, T>), ge, dim3(NT), 0, stream, Y.p, (size_t)S.Ninterp, y2.p,
                        (size_t)(g.nspl + 1), shftf.p, S.N, S.interpftpad, p, slot_xm_ptr(slot));

@@ -144,30 +144,32 @@ class Handle:
         except Exception:
             pass
 
+    # record buffers are np.empty: the library writes the first sgnlc records and only those are
+    # read back, so no page of the rest is ever touched
     def job_core(self, pm, mm, nn, rng, opts, cap=1 << 16):
-        sgnlv = np.zeros(abi.NPAR * cap)
+        sgnlv = np.empty(abi.NPAR * cap)
         sgnlc, fnum = C.c_int(0), C.c_int(0)
         st = lib().pgw_job_core(self.h, pm, mm, nn, C.byref(rng), C.byref(opts), C.byref(sgnlc),
                                 sgnlv.ctypes.data_as(_dp), cap, C.byref(fnum))
         if st != 0:
             raise PGWError(st, "pgw_job_core")
-        return sgnlv[:abi.NPAR * sgnlc.value].reshape(-1, abi.NPAR), fnum.value
+        return sgnlv[:abi.NPAR * sgnlc.value].reshape(-1, abi.NPAR).copy(), fnum.value
 
     def job_batch(self, sky, rng, opts, cap=1 << 18):
         sky = np.ascontiguousarray(np.asarray(sky, dtype=np.int32).reshape(-1, 3))
-        sgnlv = np.zeros(abi.NPAR * cap)
+        sgnlv = np.empty(abi.NPAR * cap)
         sgnlc, fnum = C.c_int(0), C.c_int(0)
         st = lib().pgw_job_batch(self.h, sky.ctypes.data_as(_ip), len(sky), C.byref(rng), C.byref(opts),
                                  C.byref(sgnlc), sgnlv.ctypes.data_as(_dp), cap, C.byref(fnum))
         if st != 0:
             raise PGWError(st, "pgw_job_batch")
-        return sgnlv[:abi.NPAR * sgnlc.value].reshape(-1, abi.NPAR), fnum.value
+        return sgnlv[:abi.NPAR * sgnlc.value].reshape(-1, abi.NPAR).copy(), fnum.value
 
     def job_batch_multi(self, work, rng, opts, cap=1 << 18):
         """work rows (segment, pm, mm, nn); returns (records, work index per record, templates)."""
         work = np.ascontiguousarray(np.asarray(work, dtype=np.int32).reshape(-1, 4))
-        sgnlv = np.zeros(abi.NPAR * cap)
-        owner = np.zeros(cap, np.int32)
+        sgnlv = np.empty(abi.NPAR * cap)
+        owner = np.empty(cap, np.int32)
         sgnlc, fnum = C.c_int(0), C.c_int(0)
         st = lib().pgw_job_batch_multi(self.h, work.ctypes.data_as(_ip), len(work), C.byref(rng), C.byref(opts),
                                        C.byref(sgnlc), sgnlv.ctypes.data_as(_dp), cap,
@@ -175,7 +177,7 @@ class Handle:
         if st != 0:
             raise PGWError(st, "pgw_job_batch_multi")
         n = sgnlc.value
-        return sgnlv[:abi.NPAR * n].reshape(-1, abi.NPAR), owner[:n].copy(), fnum.value
+        return sgnlv[:abi.NPAR * n].reshape(-1, abi.NPAR).copy(), owner[:n].copy(), fnum.value
 
     def fstatnet(self, pts, seg=0, grad=True):
         """Fstatnet at points (n, 4): (out (n, 11) with out[:, 5] = -F, dF/dpts (n, 4) or None)."""
