@@ -141,6 +141,16 @@ int pgw_job_core(pgw_handle *h, int pm, int mm, int nn, pgw_range *s_range, cons
 int pgw_job_batch(pgw_handle *h, const int *sky, int nsky, pgw_range *s_range, const pgw_opts *opts,
                   int *sgnlc, double *sgnlv, int sgnlv_cap, int *FNum);
 
+/* The spotlight search (search/spotlight/src/jobcore.c:51-135 and its job_core, :137-640): sky
+ * positions (mm[i], nn[i]) of hemisphere pm, each searched over its own list of noss[i] spindowns
+ * (the lists concatenated in ss, in the order of the spotlight file read at
+ * search/spotlight/src/init.c:392-442).  Records are appended per sky position in list order, as
+ * the spotlight driver writes them; one FNum per template.  Sky points outside the grid disk are
+ * skipped (the spotlight job_core does not test for them).  No line veto is applied unless the
+ * handle's settings carry lines. */
+int pgw_job_spotlight(pgw_handle *h, int pm, const int *mm, const int *nn, const int *noss, const int *ss,
+                      int nsky, const pgw_opts *opts, int *sgnlc, double *sgnlv, int sgnlv_cap, int *FNum);
+
 /* Several segments (time frames) of one network in one handle: sett[nseg], ifo[nseg*nifo]
  * (detector d of segment k at ifo[k*nifo + d]).  Segments may have different N; those with the
  * same nfft and fftpad share kernel launches.  pgw_init is the nseg = 1 case.  No counterpart

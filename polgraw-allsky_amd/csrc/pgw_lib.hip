@@ -220,8 +220,10 @@ void lin2ast(double be1, double be2, int pm, double sepsm, double cepsm, double*
 // ---------------------------------------------------------------- handle --
 struct pgw_handle {
   virtual ~pgw_handle() {}
+  // ssoff/sslist (optional): work item i searches spindowns sslist[ssoff[i] .. ssoff[i+1]) instead
+  // of the range's
   virtual int run(const int* work, int n, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap,
-                  int* rec_work, int* FNum) = 0;
+                  int* rec_work, int* FNum, const int* ssoff = nullptr, const int* sslist = nullptr) = 0;
   virtual int debug_fstat(int pm, int mm, int nn, int ss, const pgw_opts* o, double* F) = 0;
   virtual int debug_resampled(int pm, int mm, int nn, int det, double* a, double* b) = 0;
   virtual int time_stages(int seg, int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps,
@@ -253,6 +255,10 @@ template <typename T> struct Engine final : pgw_handle {
   struct Item {
     int work, seg, pm, mm, nn, slot, smin, smax, t0;
     SkyState sk;
+    const int* ssl = nullptr;  // explicit spindown list (spotlight), else smin..smax
+    int nlist = 0;
+    int ns() const { return ssl ? nlist : smax - smin + 1; }
+    int ss_at(int j) const { return ssl ? ssl[j] : smin + j; }
   };
   struct Group {
     std::vector<Item> items;
@@ -535,8 +541,9 @@ template <typename T> struct Engine final : pgw_handle {
   // descriptors of item `it`'s templates into host buffer `h` (index t0..)
   void fill_desc(const Item& it, const pgw_opts* o, TmplDesc* h) {
     const Seg& g = *segs[it.seg];
-    for (int ss = it.smin; ss <= it.smax; ss++) {
-      TmplDesc& d = h[it.t0 + (ss - it.smin)];
+    for (int j = 0; j < it.ns(); j++) {
+      const int ss = it.ss_at(j);
+      TmplDesc& d = h[it.t0 + j];
       d.slot = it.slot;
       tmpl_params(g.sett.M, it.mm, it.nn, ss, o->s0_flag, d.het1, d.sgnlt1);
       d.N = g.sett.N;
@@ -546,8 +553,8 @@ template <typename T> struct Engine final : pgw_handle {
   }
 
   // template stage for descriptors [t0, t0 + nt) of buffer buf; ev (optional) = 4 events around the kernels
-  // uni: all nt templates are consecutive spindowns of one item (from spindown ss0), else nullptr
-  int launch_templates(const Seg& g, int buf, int t0, int nt, const Item* uni, int ss0, const pgw_opts* o,
+  // uni: all nt templates belong to one item (its templates j0, j0 + 1, ...), else nullptr
+  int launch_templates(const Seg& g, int buf, int t0, int nt, const Item* uni, int j0, const pgw_opts* o,
                        hipEvent_t* ev = nullptr) {
     const TmplDesc* d = desc.p + (size_t)buf * kDescCap + t0;
     if (ev) HIPCHK(hipEventRecord(ev[0], stream));
@@ -556,8 +563,8 @@ template <typename T> struct Engine final : pgw_handle {
     if (uni && nt <= kSpinTab) {
       sb.uslot = uni->slot;
       sb.uN = segs[uni->seg]->sett.N;
-      const SpinGrid q = spin_grid(segs[uni->seg]->sett.M, uni->mm, uni->nn, ss0, o->s0_flag);
-      for (int i = 0; i < nt; i++) spin_coeffs(q, ss0 + i, tab.het1[i], tab.sgnlt1[i]);
+      const SpinGrid q = spin_grid(segs[uni->seg]->sett.M, uni->mm, uni->nn, 0, o->s0_flag);
+      for (int i = 0; i < nt; i++) spin_coeffs(q, uni->ss_at(j0 + i), tab.het1[i], tab.sgnlt1[i]);
     }
     launch_spin_col<T>(*g.ps, sb, tab, d, Z.p, nt, hi.p, lo.p, col_seq, stream);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));
@@ -595,8 +602,8 @@ template <typename T> struct Engine final : pgw_handle {
       const int nt = std::min(TB, gr.ntempl - t0);
       while (k + 1 < gr.items.size() && gr.items[k + 1].t0 <= t0) k++;
       const Item& it = gr.items[k];
-      const bool one = t0 + nt <= it.t0 + (it.smax - it.smin + 1);
-      if ((st = launch_templates(g0, gr.buf, t0, nt, one ? &it : nullptr, it.smin + (t0 - it.t0), o))) return st;
+      const bool one = t0 + nt <= it.t0 + it.ns();
+      if ((st = launch_templates(g0, gr.buf, t0, nt, one ? &it : nullptr, t0 - it.t0, o))) return st;
     }
     HIPCHK(hipMemcpyAsync(hn_pin.p + gr.buf, ncand.p + gr.buf, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(hc_pin.p + (size_t)gr.buf * kCandChunk, cand.p + (size_t)gr.buf * kCandCap,
@@ -627,7 +634,7 @@ template <typename T> struct Engine final : pgw_handle {
       const Seg& g = *segs[it.seg];
       const pgw_settings& S = g.sett;
       const double* M = S.M;
-      const int ss = it.smin + (hcand[c].t - it.t0), ii = hcand[c].ii;
+      const int ss = it.ss_at(hcand[c].t - it.t0), ii = hcand[c].ii;
       double het1, sgnlt1;
       tmpl_params(M, it.mm, it.nn, ss, o->s0_flag, het1, sgnlt1);
       const double sgnl0 = it.sk.het0 + het1;
@@ -656,7 +663,7 @@ template <typename T> struct Engine final : pgw_handle {
   // Work items (seg, pm, mm, nn) in order, grouped; the host turns group g-1 into records while
   // the device works on group g.
   int run(const int* work, int n, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap, int* rec_work,
-          int* FNum) override {
+          int* FNum, const int* ssoff, const int* sslist) override {
     HIPCHK(hipEventRecord(ev0, stream));
     Group prev;
     bool have_prev = false;
@@ -677,8 +684,13 @@ template <typename T> struct Engine final : pgw_handle {
         it.nn = work[4 * i + 3];
         double a, b, c, d;
         if (!sky_coords(g, it.pm, it.mm, it.nn, it.sk, a, b, c, d)) { i++; continue; }  // outside the disk
-        spindown_range(g, it.mm, it.nn, r, o, it.smin, it.smax);
-        const int ns = std::max(0, it.smax - it.smin + 1);
+        if (sslist) {
+          it.ssl = sslist + ssoff[i];
+          it.nlist = ssoff[i + 1] - ssoff[i];
+        } else {
+          spindown_range(g, it.mm, it.nn, r, o, it.smin, it.smax);
+        }
+        const int ns = std::max(0, it.ns());
         if (ns > kDescCap) { st = PGW_ERR_UNSUPPORTED; break; }
         if (g0 && (!compatible(*g0, g) || gr.ntempl + ns > kDescCap)) break;
         if (!g0) g0 = &g;
@@ -834,7 +846,7 @@ template <typename T> struct Engine final : pgw_handle {
       acc[0] = t;
       for (int rr = 0; rr < reps && !st; rr++) {
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p, 1);
-        st = launch_templates(g, 0, 0, nt, &it, it.smin, o, ev);
+        st = launch_templates(g, 0, 0, nt, &it, 0, o, ev);
         HIPCHK(hipStreamSynchronize(stream));
         for (int k = 0; k < 3; k++) {
           HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
@@ -1024,6 +1036,23 @@ int pgw_time_stages(pgw_handle* h, int pm, int mm, int nn, int s0, int nt, const
                     double ms[5]) {
   if (!h || !opts || !ms) return PGW_ERR_ARG;
   return h->time_stages(0, pm, mm, nn, s0, nt, opts, reps, ms);
+}
+
+int pgw_job_spotlight(pgw_handle* h, int pm, const int* mm, const int* nn, const int* noss, const int* ss, int nsky,
+                      const pgw_opts* opts, int* sgnlc, double* sgnlv, int sgnlv_cap, int* FNum) {
+  if (!h || !opts || !sgnlc || !FNum || nsky < 0 || (nsky && (!mm || !nn || !noss)) || (sgnlv_cap > 0 && !sgnlv))
+    return PGW_ERR_ARG;
+  std::vector<int> work(4 * (size_t)nsky), off((size_t)nsky + 1, 0);
+  for (int i = 0; i < nsky; i++) {
+    if (noss[i] < 0 || (noss[i] > 0 && !ss)) return PGW_ERR_ARG;
+    work[4 * i] = 0;
+    work[4 * i + 1] = pm;
+    work[4 * i + 2] = mm[i];
+    work[4 * i + 3] = nn[i];
+    off[i + 1] = off[i] + noss[i];
+  }
+  pgw_range r{};  // unused: every sky point brings its spindown list
+  return h->run(work.data(), nsky, &r, opts, sgnlc, sgnlv, sgnlv_cap, nullptr, FNum, off.data(), ss);
 }
 
 int pgw_fstatnet(pgw_handle* h, int seg, const double* pts, int npts, double* out, double* grad) {

@@ -9,6 +9,7 @@ Inputs of a frame (one time segment) as init_arrays()/read_grid() read them
                                                               solar-system barycentre, units of c dt),
                                                               then phir, then epsm
   range file                                                 8 ints: spndr0 spndr1 nr0 nr1 mr0 mr1 pmr0 pmr1
+  spotlight file                                             pm nsky, then per sky: mm nn noss ss_1..ss_noss
 
 Output: triggers_<ident %03d>_<band %04d><label>_<pm>.bin, records of 5 doubles
 (jobcore.c:92-93,141), written by pgw_search.
@@ -80,6 +81,26 @@ def read_range(path) -> dict:
     if len(v) != 8:
         raise ValueError(f"{path}: expected 8 integers")
     return dict(spndr=v[0:2], nr=v[2:4], mr=v[4:6], pmr=v[6:8])
+
+
+def read_spotlight(path):
+    """Spotlight range file (search/spotlight/src/init.c:392-442): "pm nskypos", then per sky
+    position "mm nn noss ss_1 .. ss_noss".  Returns (pm, [(mm, nn, [ss, ...]), ...])."""
+    v = [int(x) for x in Path(path).read_text().split()]
+    if len(v) < 2:
+        raise ValueError(f"{path}: missing header")
+    pm, nsky, k, sky = v[0], v[1], 2, []
+    while k < len(v):
+        if k + 3 > len(v):
+            raise ValueError(f"{path}: truncated sky position {len(sky)}")
+        mm, nn, noss = v[k:k + 3]
+        if noss < 0 or k + 3 + noss > len(v):
+            raise ValueError(f"{path}: bad spindown count at sky position {len(sky)}")
+        sky.append((mm, nn, v[k + 3:k + 3 + noss]))
+        k += 3 + noss
+    if len(sky) != nsky:
+        raise ValueError(f"{path}: {len(sky)} sky positions, header says {nsky}")
+    return pm, sky
 
 
 def read_triggers(path) -> np.ndarray:

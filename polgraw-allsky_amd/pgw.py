@@ -49,6 +49,8 @@ def lib():
         L.pgw_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts),
                                    _ip, _ip, _ip]
         L.pgw_fstatnet.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp]
+        L.pgw_job_spotlight.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, C.c_int, C.POINTER(abi.Opts),
+                                        _ip, _dp, C.c_int, _ip]
         L.pgw_get_settings.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.Settings)]
         L.pgw_search.argtypes = [C.c_void_p, C.POINTER(abi.Range), C.POINTER(abi.Opts), C.POINTER(abi.Output),
                                  _ip]
@@ -67,7 +69,7 @@ def lib():
 
 EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_job_core", "pgw_job_batch",
             "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_fstatnet", "pgw_get_settings",
-            "pgw_search", "pgw_read_checkpoint",
+            "pgw_search", "pgw_read_checkpoint", "pgw_job_spotlight",
             "pgw_debug_fstat", "pgw_debug_resampled", "pgw_time_stages", "pgw_copy_bench", "pgw_last_gpu_ms",
             "pgw_destroy", "pgw_strerror"]
 
@@ -163,6 +165,21 @@ class Handle:
                                  C.byref(sgnlc), sgnlv.ctypes.data_as(_dp), cap, C.byref(fnum))
         if st != 0:
             raise PGWError(st, "pgw_job_batch")
+        return sgnlv[:abi.NPAR * sgnlc.value].reshape(-1, abi.NPAR).copy(), fnum.value
+
+    def job_spotlight(self, pm, sky, opts, cap=1 << 18):
+        """sky: [(mm, nn, [ss, ...]), ...] as a spotlight file lists them; returns (records, FNum)."""
+        mm = np.ascontiguousarray([x[0] for x in sky], dtype=np.int32)
+        nn = np.ascontiguousarray([x[1] for x in sky], dtype=np.int32)
+        noss = np.ascontiguousarray([len(x[2]) for x in sky], dtype=np.int32)
+        ss = np.ascontiguousarray([v for x in sky for v in x[2]] or [0], dtype=np.int32)
+        sgnlv = np.empty(abi.NPAR * cap)
+        sgnlc, fnum = C.c_int(0), C.c_int(0)
+        st = lib().pgw_job_spotlight(self.h, pm, mm.ctypes.data_as(_ip), nn.ctypes.data_as(_ip),
+                                     noss.ctypes.data_as(_ip), ss.ctypes.data_as(_ip), len(sky), C.byref(opts),
+                                     C.byref(sgnlc), sgnlv.ctypes.data_as(_dp), cap, C.byref(fnum))
+        if st != 0:
+            raise PGWError(st, "pgw_job_spotlight")
         return sgnlv[:abi.NPAR * sgnlc.value].reshape(-1, abi.NPAR).copy(), fnum.value
 
     def job_batch_multi(self, work, rng, opts, cap=1 << 18):
