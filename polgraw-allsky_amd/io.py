@@ -10,6 +10,7 @@ Inputs of a frame (one time segment) as init_arrays()/read_grid() read them
                                                               then phir, then epsm
   range file                                                 8 ints: spndr0 spndr1 nr0 nr1 mr0 mr1 pmr0 pmr1
   spotlight file                                             pm nsky, then per sky: mm nn noss ss_1..ss_noss
+  <dtaprefix>/O1LinesToBeCleaned_<IFO>_v1.txt               known lines (7 columns, '%' comments)
 
 Output: triggers_<ident %03d>_<band %04d><label>_<pm>.bin, records of 5 doubles
 (jobcore.c:92-93,141), written by pgw_search.
@@ -101,6 +102,104 @@ def read_spotlight(path):
     if len(sky) != nsky:
         raise ValueError(f"{path}: {len(sky)} sky positions, header says {nsky}")
     return pm, sky
+
+
+FDOT_MAX = 0.5e-8  # line-broadening spindown limit of read_lines (settings.c:434)
+
+
+def read_lines(path, N: int, dt: float, DetSSB) -> list:
+    """Known lines of one detector (read_lines, settings.c:326-533) as [left, right] in Hz.
+
+    File columns: spacing or frequency (Hz), type (0 singlet, 1 comb with fixed width, 2 comb with
+    scaling width), offset of the first harmonic (Hz), first and last harmonic index, left and
+    right width (Hz); lines starting with '%' are comments.  Every width is replaced by the
+    broadening of the barycentring, 2 fdotmax (N dt + |max d(t E)/dt|) + f |max dE|, from the
+    detector's ephemeris (scaled by the harmonic index for type 2).  A missing file gives no lines;
+    blank lines are skipped."""
+    try:
+        text = Path(path).read_text()
+    except FileNotFoundError:
+        return []
+    rows = [[float(t) for t in ln.replace("\t", " ").split()] for ln in text.splitlines()
+            if ln.strip() and not ln.startswith("%")]
+    E = np.asarray(DetSSB, dtype=np.float64)[:3 * N].reshape(N, 3)
+    dE = np.abs(np.diff(E, axis=0)).max(axis=0)
+    i = np.arange(N, dtype=np.float64)[:, None]
+    dtE = (np.abs(np.diff(E * i, axis=0)) * dt).max(axis=0)
+    norm_dtE, norm_dE = math.sqrt(float(np.sum(dtE ** 2))), math.sqrt(float(np.sum(dE ** 2)))
+
+    def width(f):
+        return 2.0 * FDOT_MAX * (N * dt + norm_dtE) + f * norm_dE
+
+    out = []
+    for r in rows:
+        kind = int(r[1])
+        if kind == 0:
+            out.append([r[0] - width(r[0]), r[0] + width(r[0])])
+        elif kind in (1, 2):
+            for k in range(int(r[3]), int(r[4]) + 1):
+                f = r[2] + k * r[0]
+                w = width(f) * (k if kind == 2 else 1)
+                out.append([f - w, f + w])
+    return out
+
+
+def narrow_down_lines(narrowdown: float) -> list:
+    """Band edges excluded by -narrowdown (narrow_down_band, settings.c:535-550), in radians."""
+    return [[0.0, math.pi / 2 - narrowdown], [math.pi / 2 + narrowdown, math.pi]]
+
+
+def lines_in_band(fpo: float, B: float, det_lines, start=()) -> list:
+    """Known lines overlapping [fpo, fpo + B], clipped and mapped to [0, pi] (lines_in_band,
+    settings.c:552-597), after the entries of `start` (the narrow-down ranges)."""
+    out = [list(x) for x in start]
+    bs, be = fpo, fpo + B
+    for lines in det_lines:
+        for l, r in lines:
+            if not (r < bs or l > be):
+                ll, lr = max(l, bs), min(r, be)
+                out.append([(ll - bs) / B * math.pi, (lr - bs) / B * math.pi])
+    return out
+
+
+def veto_fraction(lines) -> float:
+    """Fraction of the band [0, pi] covered by the lines (fraction_of_band_vetoed,
+    settings.c:625-650: sorted, overlaps with the previous range counted once)."""
+    ls = sorted((float(a), float(b)) for a, b in lines)
+    tot = 0.0
+    for i, (a, b) in enumerate(ls):
+        tot += (b - ls[i - 1][1]) if i > 0 and a < ls[i - 1][1] else (b - a)
+    return tot / math.pi
+
+
+def band_fully_vetoed(lines) -> bool:
+    """True when the sorted ranges leave no gap in [0, pi] (check_if_band_is_fully_vetoed,
+    settings.c:599-623; the reference then ends the run successfully)."""
+    ll = 0.0
+    for a, b in sorted((float(a), float(b)) for a, b in lines):
+        if a > ll:
+            return False
+        ll = b
+    return not ll < math.pi
+
+
+MAXL = 2048  # search/network/src-cpu/struct.h:13
+
+
+def band_lines(frame: "Frame", dtaprefix, narrowdown: float = 0.5 * math.pi, veto: bool = True) -> list:
+    """The vetoed ranges of a frame's band, in band radians, the way main() assembles them
+    (search/network/src-cpu/main.c:83-111): the narrow-down edges when narrowdown < pi/2, then,
+    with veto, every detector's known lines (<dtaprefix>/O1LinesToBeCleaned_<IFO>_v1.txt) inside
+    the band.
+    The result goes to Handle(..., lines=...).  More than MAXL ranges raise ValueError."""
+    s = frame.sett
+    start = narrow_down_lines(narrowdown) if narrowdown < 0.5 * math.pi else []
+    det_lines = [] if not veto else [read_lines(Path(dtaprefix) / f"O1LinesToBeCleaned_{d}_v1.txt", s["N"], s["dt"], e)
+                 for d, e in zip(frame.dets, frame.DetSSB)]
+    out = lines_in_band(s["fpo"], s["B"], det_lines, start) if veto else start
+    if len(out) > MAXL:
+        raise ValueError(f"{len(out)} vetoed ranges, at most {MAXL}")
+    return out
 
 
 def read_triggers(path) -> np.ndarray:

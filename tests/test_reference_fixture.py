@@ -98,3 +98,77 @@ def test_fixture_search_fp32(frame, abi, oracle_records):
     assert len(hit) >= 0.999 * len(strong)
     rel = max(abs(F(kg[tuple(r[:4])]) - F(r[4])) / F(r[4]) for r in strong if tuple(r[:4]) in kg)
     assert rel <= 1e-4
+
+
+def test_known_lines(frame, tmp_path):
+    """Known-line veto inputs: widths from the ephemeris, band mapping and narrow-down, on the
+    reference frame's DetSSB."""
+    s = frame.sett
+    f0 = s["fpo"] + 0.3 * s["B"]
+    p = tmp_path / "O1LinesToBeCleaned_H1_v1.txt"
+    p.write_text("% comment\n"
+                 f"{f0}\t0\t0\t0\t0\t0.001\t0.001\n"          # singlet inside the band
+                 f"0.05\t1\t{s['fpo'] - 0.02}\t0\t3\t0\t0\n"    # comb: harmonics at fpo-0.02 .. fpo+0.13
+                 f"0.05\t2\t{s['fpo'] + 0.4}\t1\t2\t0\t0\n")   # scaling comb, beyond the band
+    lines = io.read_lines(p, s["N"], s["dt"], frame.DetSSB[0])
+    assert len(lines) == 1 + 4 + 2
+    w0 = (lines[0][1] - lines[0][0]) / 2
+    # the broadening is the orbital Doppler spread ~1e-4 f plus the spindown term
+    assert 0.5e-4 * f0 < w0 < 2e-4 * f0
+    assert lines[0][0] + w0 == pytest.approx(f0)
+    # scaling comb: harmonic k has k times the width of its frequency
+    for k, (a, b) in zip((1, 2), lines[5:]):
+        f = s["fpo"] + 0.4 + k * 0.05
+        assert (a + b) / 2 == pytest.approx(f)
+    band = io.lines_in_band(s["fpo"], s["B"], [lines])
+    # the singlet maps to 0.3 pi; combs below fpo are clipped at 0; the scaling comb is outside
+    assert any(a < 0.3 * math.pi < b for a, b in band)
+    assert all(0 <= a <= b <= math.pi for a, b in band)
+    assert not any(a > 0.9 * math.pi for a, b in band)
+    nd = io.narrow_down_lines(0.4)
+    assert nd == [[0.0, math.pi / 2 - 0.4], [math.pi / 2 + 0.4, math.pi]]
+    both = io.lines_in_band(s["fpo"], s["B"], [lines], start=nd)
+    assert both[:2] == nd
+    assert io.veto_fraction(nd) == pytest.approx(1 - 0.8 / math.pi)
+    assert io.veto_fraction([[0, 1], [0.5, 2], [3, 3.1]]) == pytest.approx(2.1 / math.pi)
+    assert io.read_lines(tmp_path / "missing.txt", s["N"], s["dt"], frame.DetSSB[0]) == []
+
+
+def test_band_veto_assembly(frame, tmp_path):
+    """band_lines: narrow-down first, then the detectors' lines in the band; fully vetoed bands."""
+    s = frame.sett
+    assert io.band_lines(frame, tmp_path) == []                      # no lines files, no narrow-down
+    assert io.band_lines(frame, tmp_path, narrowdown=0.2) == io.narrow_down_lines(0.2)
+    f0 = s["fpo"] + 0.7 * s["B"]
+    (tmp_path / "O1LinesToBeCleaned_L1_v1.txt").write_text(f"{f0} 0 0 0 0 0 0\n")
+    got = io.band_lines(frame, tmp_path, narrowdown=0.2)
+    assert len(got) == 3 and got[:2] == io.narrow_down_lines(0.2)
+    assert got[2][0] < 0.7 * math.pi < got[2][1]
+    assert io.band_lines(frame, tmp_path, narrowdown=0.2, veto=False) == io.narrow_down_lines(0.2)
+    assert not io.band_fully_vetoed(got)
+    assert io.band_fully_vetoed(io.narrow_down_lines(0.0))           # [0, pi/2] and [pi/2, pi]
+    assert io.band_fully_vetoed([[1.0, math.pi], [0.0, 1.5]])         # out of order, overlapping
+    assert not io.band_fully_vetoed([[0.0, 1.0], [1.1, math.pi]])     # a gap at 1..1.1
+    assert not io.band_fully_vetoed([[0.0, 3.0]])                     # open at the top
+
+
+@pytest.mark.gpu
+def test_fixture_veto_fp64(frame, abi, orc, tmp_path):
+    """Known-line veto and narrow-down from the reference's inputs: the device's records equal the
+    oracle's with the same ranges, none of them inside a vetoed range, and fewer than without."""
+    from polgraw_allsky_amd import pgw
+    s = frame.sett
+    (tmp_path / "O1LinesToBeCleaned_H1_v1.txt").write_text(
+        f"% singlet and a comb\n{s['fpo'] + 0.6 * s['B']} 0 0 0 0 0 0\n0.01 1 {s['fpo']} 30 36 0 0\n")
+    lines = io.band_lines(frame, tmp_path, narrowdown=1.2)
+    assert len(lines) == 2 + 1 + 7
+    r, opts = abi.make_range(frame), abi.make_opts(trl=12.0)
+    got, fg = pgw.Handle(frame, abi.PREC_FP64, lines=lines).job_core(1, 0, 0, r, opts)
+    ref, fr = orc.Search(frame, nthreads=8, lines=lines).job_core(1, 0, 0, r, opts)
+    full, _ = pgw.Handle(frame, abi.PREC_FP64).job_core(1, 0, 0, r, opts)
+    print("records", len(got), "unvetoed", len(full), flush=True)
+    assert fg == fr and got.shape == ref.shape and 0 < len(got) < len(full)
+    assert np.array_equal(got[:, :4], ref[:, :4])
+    assert np.max(np.abs(got[:, 4] - ref[:, 4]) / ref[:, 4]) <= 1e-9
+    for lo, hi in lines:
+        assert not np.any((got[:, 0] >= lo) & (got[:, 0] <= hi))
