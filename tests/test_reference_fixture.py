@@ -159,7 +159,7 @@ def test_fixture_veto_fp64(frame, abi, orc, tmp_path):
     from polgraw_allsky_amd import pgw
     s = frame.sett
     (tmp_path / "O1LinesToBeCleaned_H1_v1.txt").write_text(
-        f"% singlet and a comb\n{s['fpo'] + 0.6 * s['B']} 0 0 0 0 0 0\n0.01 1 {s['fpo']} 30 36 0 0\n")
+        f"% singlet and a comb\n{s['fpo'] + 0.6 * s['B']} 0 0 0 0 0 0\n0.01 1 {s['fpo']} 10 16 0 0\n")
     lines = io.band_lines(frame, tmp_path, narrowdown=1.2)
     assert len(lines) == 2 + 1 + 7
     r, opts = abi.make_range(frame), abi.make_opts(trl=12.0)
