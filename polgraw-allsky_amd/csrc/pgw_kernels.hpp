@@ -34,6 +34,21 @@ struct SkyParams {
   int N, nfft, nifo;
 };
 
+// One sky point of a batched sky stage: its coefficients, its segment's inputs and the sky slot
+// that receives its resampled series xm, shifts shft and aa/bb.  Sky points of one batch share
+// nfft, Ninterp and the detector count; N may differ.
+struct SkyItem {
+  SkyParams p;
+  const double* xDat;     // [nifo][N]
+  const double* DetSSB;   // [nifo][3N]
+  const double* cosmodf;  // [N]
+  const double* sinmodf;  // [N]
+  void* xm;               // slot series cplx<T>[nifo][{a,b}][N]
+  double* shft;           // slot shifts [nifo][N]
+  double* aabb;           // slot aa, bb
+  int nspl, pad;          // spline knots interpftpad N
+};
+
 // Expressions evaluated in the reference's operation order without fused multiply-add
 // (HIP compiles with -ffp-contract=fast-honor-pragmas, so the pragma holds after inlining).
 __device__ __forceinline__ double dot3_nc(double a0, double b0, double a1, double b1, double a2, double b2) {
@@ -47,19 +62,28 @@ __device__ __forceinline__ double lin2_nc(double a0, double b0, double a1, doubl
 
 // ------------------------------------------------------------- sky stage --
 
-// grid (ceil(nfft/NT), nifo).  R[2d][i] = xDat aa e^{-i phase}, R[2d+1][i] = xDat bb e^{-i phase}
-// for i < N, zero for N <= i < nfft; shft, shftf per detector; block partial sums of aa^2, bb^2.
+// grid (ceil(nfft/NT), nifo, sky points).  For sky point j: R[j][2d][i] = xDat aa e^{-i phase},
+// R[j][2d+1][i] = xDat bb e^{-i phase} for i < N, zero for N <= i < nfft; shft (slot), shftf
+// [j][d][Nstride]; block partial sums of aa^2, bb^2 at paa/pbb [j][d][block].
 template <typename T>
-__global__ __launch_bounds__(NT) void k_sky_prologue(SkyParams p, const double* __restrict__ xDat,
-                                                     const double* __restrict__ DetSSB,
-                                                     const double* __restrict__ cosmodf,
-                                                     const double* __restrict__ sinmodf, cplx<T>* __restrict__ R,
-                                                     double* __restrict__ shft, double* __restrict__ shftf,
+__global__ __launch_bounds__(NT) void k_sky_prologue(const SkyItem* __restrict__ items, cplx<T>* __restrict__ R,
+                                                     double* __restrict__ shftf, size_t Nstride,
                                                      double* __restrict__ paa, double* __restrict__ pbb) {
   using T2 = cplx<T>;
-  const int d = blockIdx.y;
+  const int d = blockIdx.y, j = blockIdx.z;
   const int i = blockIdx.x * NT + threadIdx.x;
+  const SkyItem& it = items[j];
+  const SkyParams& p = it.p;
   const SkyDet& q = p.det[d];
+  const double* __restrict__ xDat = it.xDat;
+  const double* __restrict__ DetSSB = it.DetSSB;
+  const double* __restrict__ cosmodf = it.cosmodf;
+  const double* __restrict__ sinmodf = it.sinmodf;
+  double* __restrict__ shft = it.shft;
+  shftf += (size_t)j * p.nifo * Nstride;
+  R += (size_t)j * 2 * p.nifo * p.nfft;
+  paa += (size_t)j * p.nifo * gridDim.x;
+  pbb += (size_t)j * p.nifo * gridDim.x;
   double a2 = 0.0, b2 = 0.0;
   T2 va = mk<T2>(0, 0), vb = mk<T2>(0, 0);
   if (i < p.N) {
@@ -77,7 +101,7 @@ __global__ __launch_bounds__(NT) void k_sky_prologue(SkyParams p, const double* 
     // shftf[0] must come out exactly 0 (the spline returns 0 for x < 0)
     const double sh = dot3_nc(q.nS0, D[0], q.nS1, D[1], q.nS2, D[2]);
     shft[(size_t)d * p.N + i] = sh;
-    shftf[(size_t)d * p.N + i] = sh - q.shft1;
+    shftf[(size_t)d * Nstride + i] = sh - q.shft1;
     double sp, cp;
     sincos(lin2_nc(p.het0, (double)i, p.oms, sh), &sp, &cp);
     const double x = xDat[(size_t)d * p.N + i];
@@ -191,37 +215,66 @@ struct SplineTab {
   double invp_inf, c_inf;
 };
 
-// Both recursions of the spline for SPF_OUT consecutive outputs of one series per workgroup:
-// the input window is read into LDS with coalesced loads, u (forward) and then y2 (backward) run
-// in chunks of SPL_K per thread from LDS, each chunk starting SPL_W samples early from a zero
-// state, and y2 leaves through LDS with coalesced stores.  grid (ceil(n / SPF_OUT), series).
-// LDS arrays are padded by one element per 16 so the lanes' chunk starts fall on different banks.
-constexpr int SPF_OUT = NT * SPL_K;              // y2 outputs per workgroup
-constexpr int SPF_UW = SPF_OUT + SPL_W;          // u values per workgroup (outputs + warm-up above)
-constexpr int SPF_YW = SPF_OUT + 2 * SPL_W + 2;  // y samples per workgroup
+// Both recursions of the spline for spf_out(K) = NT K consecutive outputs of one series per
+// workgroup: the input window is read into LDS with coalesced loads, u (forward) and then y2
+// (backward) run in chunks of K per thread from LDS, each chunk starting SPL_W samples early from
+// a zero state, and y2 leaves through LDS with coalesced stores.  grid (ceil(n / spf_out(K)),
+// series).  LDS arrays are padded by one element per 16 so the lanes' chunk starts fall on
+// different banks.
+__host__ __device__ constexpr int spf_out(int K) { return NT * K; }            // y2 outputs per workgroup
+__host__ __device__ constexpr int spf_uw(int K) { return NT * K + SPL_W; }     // u values (outputs + warm-up above)
+__host__ __device__ constexpr int spf_yw(int K) { return NT * K + 2 * SPL_W + 2; }  // y samples
 __device__ __forceinline__ constexpr int spad(int i) { return i + (i >> 4); }
 
-template <typename T>
-__global__ __launch_bounds__(NT) void k_spline(const cplx<T>* __restrict__ y, size_t ystride, int n,
+template <typename T, int K>
+__global__ __launch_bounds__(NT) void k_spline(const cplx<T>* __restrict__ y, size_t ystride,
                                                const SplineTab* __restrict__ tab, double2* __restrict__ y2,
-                                               size_t stride, SkyParams p, const double* __restrict__ paa,
-                                               const double* __restrict__ pbb, int npro, double* __restrict__ aabb) {
+                                               size_t stride, const SkyItem* __restrict__ items, int nb,
+                                               const double* __restrict__ paa, const double* __restrict__ pbb,
+                                               int npro) {
+  constexpr int SPF_OUT = spf_out(K), SPF_UW = spf_uw(K), SPF_YW = spf_yw(K);
   __shared__ cplx<T> Ys[spad(SPF_YW) + 1];
   __shared__ double2 Us[spad(SPF_UW) + 1];
   const int b = blockIdx.y, tid = threadIdx.x;
+  const SkyItem& it = items[b / nb];
+  // the sky point's aa, bb, by the first workgroup of its first series (saves a launch)
+  if (blockIdx.x == 0 && b % nb == 0) {
+    __shared__ double red[2][NT];
+    const size_t o = (size_t)(b / nb) * it.p.nifo * npro;
+    aabb_reduce(it.p, paa + o, pbb + o, npro, it.aabb, red);
+  }
+  const int n = it.nspl;
   const int A0 = blockIdx.x * SPF_OUT;
+  if (A0 >= n) return;  // beyond a shorter sky point's knots (uniform across the workgroup)
   const int A1 = min(A0 + SPF_OUT, n);         // y2 outputs [A0, A1)
   const int U1 = min(A1 + SPL_W - 1, n - 2);   // u needed on [A0, U1]
   const int i0 = max(1, A0 - SPL_W);           // first recursion index of any chunk
   const int Ylo = i0 - 1, Yhi = U1 + 1;        // y window [Ylo, Yhi] (Yhi <= n - 1)
   const cplx<T>* Y = y + (size_t)b * ystride;
   double2* Y2 = y2 + (size_t)b * stride;
-  for (int j = tid; j <= Yhi - Ylo; j += NT) Ys[spad(j)] = Y[Ylo + j];
+  // the window's loads all in flight before the LDS stores (addresses clamped to the window)
+  constexpr int NYL = (SPF_YW + NT - 1) / NT;
+  cplx<T> yv[NYL];
+#pragma unroll
+  for (int r = 0; r < NYL; r++) yv[r] = Y[Ylo + min(tid + r * NT, Yhi - Ylo)];
+#pragma unroll
+  for (int r = 0; r < NYL; r++)
+    if (tid + r * NT <= Yhi - Ylo) Ys[spad(tid + r * NT)] = yv[r];
+  // recursion coefficients: the first SPL_NTAB from LDS, the limit from registers, so the
+  // recursions read no global memory
+  __shared__ double tinv[SPL_NTAB], tcf[SPL_NTAB];
+  if (tid < SPL_NTAB) {
+    tinv[tid] = tab->invp[tid];
+    tcf[tid] = tab->c[tid];
+  }
+  const double invp_inf = tab->invp_inf, c_inf = tab->c_inf;
   if (A0 == 0 && tid == 0) Us[spad(0)] = make_double2(0.0, 0.0);  // u_0 = 0 (auxi.c:62)
   __syncthreads();
   // u_i = (-u_{i-1}/2 + 3 (y_{i-1} - 2 y_i + y_{i+1})) invp_i  (auxi.c:64-72)
-  for (int ch = tid; A0 + ch * SPL_K <= U1; ch += NT) {
-    const int ua = A0 + ch * SPL_K, ue = min(ua + SPL_K, U1 + 1);
+  // thread t: u on [A0 + t K, A0 + (t + 1) K); the thread of the last output chunk continues to U1
+  const int nch = (A1 - A0 + K - 1) / K;
+  if (tid < nch) {
+    const int ua = A0 + tid * K, ue = (tid == nch - 1) ? U1 + 1 : min(ua + K, U1 + 1);
     int i = max(1, ua - SPL_W);
     double ur = 0.0, ui = 0.0;
     double ym_r = (double)Ys[spad(i - 1 - Ylo)].x, ym_i = (double)Ys[spad(i - 1 - Ylo)].y;
@@ -229,7 +282,7 @@ __global__ __launch_bounds__(NT) void k_spline(const cplx<T>* __restrict__ y, si
     for (; i < ue; i++) {
       const cplx<T> yp = Ys[spad(i + 1 - Ylo)];
       const double yp_r = (double)yp.x, yp_i = (double)yp.y;
-      const double invp = i < SPL_NTAB ? tab->invp[i] : tab->invp_inf;
+      const double invp = i < SPL_NTAB ? tinv[i] : invp_inf;
       const double dr = ym_r - 2. * y0_r + yp_r, di = ym_i - 2. * y0_i + yp_i;
       ur = (-.5 * ur + 3. * dr) * invp;
       ui = (-.5 * ui + 3. * di) * invp;
@@ -239,22 +292,22 @@ __global__ __launch_bounds__(NT) void k_spline(const cplx<T>* __restrict__ y, si
   }
   __syncthreads();
   // y2_k = c_k y2_{k+1} + u_k, y2_{n-1} = 0  (auxi.c:73-80), outputs [a, e) of this thread
-  const int a = A0 + tid * SPL_K, e = min(a + SPL_K, A1);
-  double2 out[SPL_K];
+  const int a = A0 + tid * K, e = min(a + K, A1);
+  double2 out[K];
   if (a < A1) {
     int k = (e + SPL_W >= n - 1) ? n - 2 : e + SPL_W - 1;
     double zr = 0.0, zi = 0.0;
-    for (; k >= a + SPL_K; k--) {  // warm-up above the chunk
-      const double c = k < SPL_NTAB ? tab->c[k] : tab->c_inf;
+    for (; k >= a + K; k--) {  // warm-up above the chunk
+      const double c = k < SPL_NTAB ? tcf[k] : c_inf;
       const double2 uk = Us[spad(k - A0)];
       zr = c * zr + uk.x;
       zi = c * zi + uk.y;
     }
 #pragma unroll
-    for (int j = SPL_K - 1; j >= 0; j--) {
+    for (int j = K - 1; j >= 0; j--) {
       const int kk = a + j;
       if (kk <= k) {
-        const double c = kk < SPL_NTAB ? tab->c[kk] : tab->c_inf;
+        const double c = kk < SPL_NTAB ? tcf[kk] : c_inf;
         const double2 uk = Us[spad(kk - A0)];
         zr = c * zr + uk.x;
         zi = c * zi + uk.y;
@@ -265,7 +318,7 @@ __global__ __launch_bounds__(NT) void k_spline(const cplx<T>* __restrict__ y, si
   __syncthreads();  // every u read is done: Us now stages y2
   if (a < A1) {
 #pragma unroll
-    for (int j = 0; j < SPL_K; j++)
+    for (int j = 0; j < K; j++)
       if (a + j < e) Us[spad(a + j - A0)] = out[j];
   }
   __syncthreads();
@@ -274,25 +327,22 @@ __global__ __launch_bounds__(NT) void k_spline(const cplx<T>* __restrict__ y, si
     Y2[n - 1] = make_double2(0.0, 0.0);
     Y2[n] = make_double2(0.0, 0.0);
   }
-  // the sky point's aa, bb, by the first workgroup (saves a launch)
-  if (blockIdx.x == 0 && blockIdx.y == 0) {
-    __shared__ double red[2][NT];
-    aabb_reduce(p, paa, pbb, npro, aabb, red);
-  }
 }
 
 // out[b][i] = splint(y, y2, n, interpftpad (i - shftf[i])) / sig2   (auxi.c:83-120, jobcore.c:386-389)
-// batch b = 2d + {0: a, 1: b}; output stored as xm[d][{a,b}][i].
+// batch b = nb j + 2d + {0: a, 1: b} for sky point j; output stored as the slot's xm[d][{a,b}][i].
 template <typename T, typename TO>
 __global__ __launch_bounds__(NT) void k_spline_eval(const cplx<T>* __restrict__ y, size_t ystride,
                                                     const double2* __restrict__ y2, size_t y2stride,
-                                                    const double* __restrict__ shftf, int N, int interpftpad,
-                                                    SkyParams p, cplx<TO>* __restrict__ xm) {
-  const int b = blockIdx.y, d = b >> 1;
+                                                    const double* __restrict__ shftf, size_t Nstride,
+                                                    int interpftpad, const SkyItem* __restrict__ items, int nb) {
+  const int b = blockIdx.y, j = b / nb, bs = b % nb, d = bs >> 1;
+  const SkyItem& it = items[j];
+  const int N = it.p.N;
   const int i = blockIdx.x * NT + threadIdx.x;
   if (i >= N) return;
   const int n = interpftpad * N;
-  const double x = interpftpad * (i - shftf[(size_t)d * N + i]);
+  const double x = interpftpad * (i - shftf[((size_t)j * it.p.nifo + d) * Nstride + i]);
   double vr = 0.0, vi = 0.0;
   if (!(x < 0 || x > n - 1)) {
     const int klo = (int)floor(x), khi = klo + 1;
@@ -307,8 +357,8 @@ __global__ __launch_bounds__(NT) void k_spline_eval(const cplx<T>* __restrict__ 
     vr = A * ylo.x + B * yhi.x + (ca * zlo.x + cb * zhi.x) / 6.0;
     vi = A * ylo.y + B * yhi.y + (ca * zlo.y + cb * zhi.y) / 6.0;
   }
-  const double sig2 = p.det[d].sig2;
-  xm[(size_t)b * N + i] = mk<cplx<TO>>((TO)(vr / sig2), (TO)(vi / sig2));
+  const double sig2 = it.p.det[d].sig2;
+  static_cast<cplx<TO>*>(it.xm)[(size_t)bs * N + i] = mk<cplx<TO>>((TO)(vr / sig2), (TO)(vi / sig2));
 }
 
 // ---------------------------------------------------------- template stage --
@@ -390,7 +440,7 @@ struct Cand {
 };
 
 struct ThrParams {
-  int L1, L2, nmin, nmax, nblk, nparts, white, t0, cap;
+  int L1, L2, nmin, nmax, nblk, nparts, white, t0, cap, lgL2;  // L2 = 1 << lgL2
   double trl;
 };
 
@@ -425,32 +475,62 @@ __global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restri
   const double m = part_mean(sum, part, p);
   if (lane == 0) mu_out[(size_t)t * p.nparts + part] = m;
   if (!(mx / m > p.trl)) return;  // uniform across the wave
-  // flagged: means of all parts (same fixed-order reduction as above)
-  for (int q = 0; q < p.nparts; q++) {
-    double sq = 0.0;
-    for (int k2 = lane; k2 < p.L2; k2 += 64) sq += (double)ps0[(size_t)q * p.L2 + k2];
-    sq = wave_sum(sq);
-    if (lane == 0) mu[q] = part_mean(sq, q, p);
-  }
-  __syncthreads();
+  // flagged: this part's bins k = lok + lane + 64 j, all loads in flight at once (addresses
+  // clamped, no branches), pre-selected with F > trl m (1 - 1e-14), which every bin with
+  // F / m > trl passes; the exact test follows per selected bin
+  constexpr int NJ = PGW_NAV_DEV / 64;
   const int lok = p.nmin + PGW_NAV_DEV * part;
   const int hik = (part < p.nblk) ? lok + PGW_NAV_DEV : p.nmax;
-  for (int k = lok + lane; k < hik; k += 64) {
-    const double fn = (double)F[(size_t)(k % p.L2) * p.L1 + k / p.L2] / mu[part];
+  const int mL2 = p.L2 - 1;
+  auto at = [&](int k) { return (double)F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)]; };
+  T f[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; j++) {
+    const int k = min(lok + lane + 64 * j, hik - 1);
+    f[j] = F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)];
+  }
+  const double pre = p.trl * m * (1. - 1e-14);
+  uint64_t hits = 0;
+#pragma unroll
+  for (int j = 0; j < NJ; j++)
+    hits |= (uint64_t)(lok + lane + 64 * j < hik && (double)f[j] > pre) << j;
+  // means of all parts with the unflagged waves' arithmetic, 8 parts' loads in flight at a time
+  for (int q0 = 0; q0 < p.nparts; q0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = 0.0;
+    for (int k2 = lane; k2 < p.L2; k2 += 64) {
+      T x[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) x[j] = ps0[(size_t)min(q0 + j, p.nparts - 1) * p.L2 + k2];
+#pragma unroll
+      for (int j = 0; j < 8; j++) v[j] += (double)x[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const double sq = wave_sum(v[j]);
+      if (lane == 0 && q0 + j < p.nparts) mu[q0 + j] = part_mean(sq, q0 + j, p);
+    }
+  }
+  __syncthreads();
+  while (hits) {
+    const int j = __builtin_ctzll(hits);
+    hits &= hits - 1;
+    const int k = lok + lane + 64 * j;
+    const double fn = at(k) / mu[part];
     if (!(fn > p.trl)) continue;
     bool start = (k == p.nmin);
     if (!start) {
       const int kp = k - 1;
       const int pp = min((kp - p.nmin) / PGW_NAV_DEV, p.nparts - 1);
-      const double fp = (double)F[(size_t)(kp % p.L2) * p.L1 + kp / p.L2] / mu[pp];
-      start = !(fp > p.trl);
+      start = !(at(kp) / mu[pp] > p.trl);
     }
     if (!start) continue;
     int ii = k;
     double Fc = fn;
     for (int kk = k + 1; kk < p.nmax; kk++) {
       const int pk = min((kk - p.nmin) / PGW_NAV_DEV, p.nparts - 1);
-      const double v = (double)F[(size_t)(kk % p.L2) * p.L1 + kk / p.L2] / mu[pk];
+      const double v = at(kk) / mu[pk];
       if (!(v > p.trl)) break;
       if (v >= Fc) { ii = kk; Fc = v; }
     }

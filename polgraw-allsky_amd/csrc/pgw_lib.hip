@@ -275,6 +275,8 @@ template <typename T> struct Engine final : pgw_handle {
   DevBuf<double2> y2;
   DevBuf<double> shftf, paa, pbb;
   DevBuf<SplineTab> spl;
+  DevBuf<SkyItem> sky_dev;  // 2 x kSlots
+  PinBuf<SkyItem> sky_pin;  // 2 x kSlots
   // sky slots
   DevBuf<T2> xm_slots;
   DevBuf<double> shft_slots, aabb_slots;
@@ -296,6 +298,7 @@ template <typename T> struct Engine final : pgw_handle {
   int group = kSlots;  // sky points per group
   int row_na = 1;
   bool col_seq = false;
+  int spl_k = 8;  // spline outputs per thread (4, 8 or 16)
 
   ~Engine() override {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -403,6 +406,7 @@ template <typename T> struct Engine final : pgw_handle {
       if (const char* q = strstr(tune, "col_seq=")) if (sscanf(q, "col_seq=%d", &v) == 1) col_seq = v != 0;
       if (const char* q = strstr(tune, "tb=")) if (sscanf(q, "tb=%d", &v) == 1 && v >= 1 && v <= kTBMax) TB = v;
       if (const char* q = strstr(tune, "group=")) if (sscanf(q, "group=%d", &v) == 1 && v >= 1 && v <= kSlots) group = v;
+      if (const char* q = strstr(tune, "spl_k=")) if (sscanf(q, "spl_k=%d", &v) == 1 && (v == 4 || v == 8 || v == 16)) spl_k = v;
     }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
@@ -413,11 +417,14 @@ template <typename T> struct Engine final : pgw_handle {
     if ((st = upload_tw<T>(hi, 1 << 20, 10, 1024)) || (st = upload_tw<T>(lo, 1 << 20, 0, 1024))) return st;
     slot_xm = (size_t)2 * nifo * Nmax;
     slot_shft = (size_t)nifo * Nmax;
-    if ((st = R.alloc((size_t)2 * nifo * nfftmax)) || (st = Zf.alloc((size_t)2 * nifo * nfftmax)) ||
-        (st = Zi.alloc((size_t)2 * nifo * Nimax)) || (st = Y.alloc((size_t)2 * nifo * Nimax)) ||
-        (st = y2.alloc((size_t)2 * nifo * (nsplmax + 1))) ||
-        (st = shftf.alloc((size_t)nifo * Nmax)) || (st = paa.alloc((size_t)nifo * npromax)) ||
-        (st = pbb.alloc((size_t)nifo * npromax)) || (st = spl.alloc(1)) ||
+    // sky-stage scratch for a group of up to kSlots sky points
+    const size_t K = kSlots;
+    if ((st = R.alloc(K * 2 * nifo * nfftmax)) || (st = Zf.alloc(K * 2 * nifo * nfftmax)) ||
+        (st = Zi.alloc(K * 2 * nifo * Nimax)) || (st = Y.alloc(K * 2 * nifo * Nimax)) ||
+        (st = y2.alloc(K * 2 * nifo * (nsplmax + 1))) ||
+        (st = shftf.alloc(K * nifo * Nmax)) || (st = paa.alloc(K * nifo * npromax)) ||
+        (st = pbb.alloc(K * nifo * npromax)) || (st = spl.alloc(1)) || (st = sky_dev.alloc(2 * K)) ||
+        (st = sky_pin.alloc(2 * K)) ||
         (st = xm_slots.alloc((size_t)kSlots * slot_xm)) || (st = shft_slots.alloc((size_t)kSlots * slot_shft)) ||
         (st = aabb_slots.alloc((size_t)kSlots * 2)) || (st = desc.alloc((size_t)2 * kDescCap)) ||
         (st = desc_pin.alloc((size_t)2 * kDescCap)) || (st = cand.alloc((size_t)2 * kCandCap)) ||
@@ -462,15 +469,14 @@ template <typename T> struct Engine final : pgw_handle {
     return 1;
   }
 
-  // Sky-point stage, jobcore.c:270-393, into slot `slot`.  Returns 1 when inside the disk.
-  int sky_stage(const Seg& g, int pm, int mm, int nn, int slot, SkyState& ss, int& status) {
+  // Coefficients of one sky point's stage (jobcore.c:270-295); 0 when outside the disk.
+  int sky_item(const Seg& g, int pm, int mm, int nn, int slot, SkyState& ss, SkyItem& it) {
 #pragma clang fp contract(off)
-    status = PGW_OK;
     double sinalt, cosalt, sindelt, cosdelt;
     if (!sky_coords(g, pm, mm, nn, ss, sinalt, cosalt, sindelt, cosdelt)) return 0;
     const pgw_settings& S = g.sett;
-    SkyParams p;
-    memset(&p, 0, sizeof(p));
+    memset(&it, 0, sizeof(it));
+    SkyParams& p = it.p;
     p.het0 = ss.het0;
     p.oms = S.oms;
     p.N = S.N;
@@ -491,30 +497,68 @@ template <typename T> struct Engine final : pgw_handle {
       q.shft1 = q.nS0 * g.ssb0[3 * d] + q.nS1 * g.ssb0[3 * d + 1] + q.nS2 * g.ssb0[3 * d + 2];
       q.sig2 = g.sig2[d];
     }
+    it.xDat = g.xDat.p;
+    it.DetSSB = g.DetSSB.p;
+    it.cosmodf = g.cosmodf.p;
+    it.sinmodf = g.sinmodf.p;
+    it.xm = slot_xm_ptr(slot);
+    it.shft = slot_shft_ptr(slot);
+    it.aabb = slot_aabb_ptr(slot);
+    it.nspl = g.nspl;
+    return 1;
+  }
+
+  // Sky-point stage (jobcore.c:270-393) of G sky points at once: items[j] (host, prepared by
+  // sky_item) are staged through pinned buffer `buf`; every kernel covers all G sky points.
+  // Segments of one batch share nfft and Ninterp (compatible()).
+  int sky_batch(const Seg& g0, const SkyItem* items, int G, int buf) {
+    const pgw_settings& S = g0.sett;
+    SkyItem* hp = sky_pin.p + (size_t)buf * kSlots;
+    memcpy(hp, items, sizeof(SkyItem) * G);
+    const SkyItem* dp = sky_dev.p + (size_t)buf * kSlots;
+    HIPCHK(hipMemcpyAsync((void*)dp, hp, sizeof(SkyItem) * G, hipMemcpyHostToDevice, stream));
+    int Ng = 0, nsplg = 0;
+    for (int j = 0; j < G; j++) {
+      Ng = std::max(Ng, items[j].p.N);
+      nsplg = std::max(nsplg, items[j].nspl);
+    }
     const int nb = 2 * nifo;
-    double* shft = slot_shft_ptr(slot);
-    hipLaunchKernelGGL((k_sky_prologue<T>), dim3(g.npro, nifo), dim3(NT), 0, stream, p, g.xDat.p, g.DetSSB.p,
-                       g.cosmodf.p, g.sinmodf.p, R.p, shft, shftf.p, paa.p, pbb.p);
+    hipLaunchKernelGGL((k_sky_prologue<T>), dim3(g0.npro, nifo, G), dim3(NT), 0, stream, dp, R.p, shftf.p, Nmax,
+                       paa.p, pbb.p);
     // resampling: forward FFT (nfft), zero-insert, backward FFT (Ninterp), scale (jobcore.c:319-362)
     LoadPlain<T2> lp{R.p, S.nfft, S.nfft};
-    launch_col<T, -1>(*g.pf, lp, Zf.p, nb, hi.p, lo.p, stream);
-    launch_row<T, -1>(*g.pf, Zf.p, nb, stream);
-    LoadZeroInsert<T2> lz{Zf.p, S.nfft, S.Ninterp, S.nfft / 2 + 1, g.pf->L1, g.pf->L2};
-    launch_col<T, +1>(*g.pi, lz, Zi.p, nb, hi.p, lo.p, stream);
-    launch_row<T, +1>(*g.pi, Zi.p, nb, stream);
-    const int count = std::min(S.Ninterp, g.nspl);
-    const int rows_k1 = std::min(g.pi->L1, (count + g.pi->L2 - 1) / g.pi->L2);
-    dim3 gt((rows_k1 + 31) / 32, (g.pi->L2 + 31) / 32, nb);
+    launch_col<T, -1>(*g0.pf, lp, Zf.p, nb * G, hi.p, lo.p, stream);
+    launch_row<T, -1>(*g0.pf, Zf.p, nb * G, stream);
+    LoadZeroInsert<T2> lz{Zf.p, S.nfft, S.Ninterp, S.nfft / 2 + 1, g0.pf->L1, g0.pf->L2};
+    launch_col<T, +1>(*g0.pi, lz, Zi.p, nb * G, hi.p, lo.p, stream);
+    launch_row<T, +1>(*g0.pi, Zi.p, nb * G, stream);
+    const int count = std::min(S.Ninterp, nsplg);
+    const int rows_k1 = std::min(g0.pi->L1, (count + g0.pi->L2 - 1) / g0.pi->L2);
+    dim3 gt((rows_k1 + 31) / 32, (g0.pi->L2 + 31) / 32, nb * G);
     const T ft = (T)((double)S.interpftpad / S.Ninterp);
-    hipLaunchKernelGGL((k_untranspose<T>), gt, dim3(NT), 0, stream, Zi.p, Y.p, g.pi->L1, g.pi->L2, count, ft);
+    hipLaunchKernelGGL((k_untranspose<T>), gt, dim3(NT), 0, stream, Zi.p, Y.p, g0.pi->L1, g0.pi->L2, count, ft);
     // spline on the first interpftpad*N samples, evaluated at interpftpad*(i - shftf[i]) (auxi.c:98-122)
-    hipLaunchKernelGGL((k_spline<T>), dim3((g.nspl + SPF_OUT - 1) / SPF_OUT, nb), dim3(NT), 0, stream, Y.p,
-                       (size_t)S.Ninterp, g.nspl, spl.p, y2.p, (size_t)(g.nspl + 1), p, paa.p, pbb.p, g.npro,
-                       slot_aabb_ptr(slot));
-    dim3 ge((S.N + NT - 1) / NT, nb);
+#define PGW_SPLINE(K)                                                                                           \
+  hipLaunchKernelGGL((k_spline<T, K>), dim3((nsplg + spf_out(K) - 1) / spf_out(K), nb * G), dim3(NT), 0, stream, \
+                     Y.p, (size_t)S.Ninterp, spl.p, y2.p, (size_t)(nsplmax + 1), dp, nb, paa.p, pbb.p, g0.npro)
+    if (spl_k == 4) PGW_SPLINE(4);
+    else if (spl_k == 16) PGW_SPLINE(16);
+    else PGW_SPLINE(8);
+#undef PGW_SPLINE
+    dim3 ge((Ng + NT - 1) / NT, nb * G);
     hipLaunchKernelGGL((k_spline_eval<T, T>), ge, dim3(NT), 0, stream, Y.p, (size_t)S.Ninterp, y2.p,
-                       (size_t)(g.nspl + 1), shftf.p, S.N, S.interpftpad, p, slot_xm_ptr(slot));
-    if (hipGetLastError() != hipSuccess) status = PGW_ERR_DEVICE;
+                       (size_t)(nsplmax + 1), shftf.p, Nmax, S.interpftpad, dp, nb);
+    return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
+  }
+
+  // One sky point's stage into slot `slot` (diagnostics and stage timing).  Returns 1 when
+  // inside the disk; status receives errors.
+  int sky_stage(const Seg& g, int pm, int mm, int nn, int slot, SkyState& ss, int& status) {
+    SkyItem it;
+    status = PGW_OK;
+    if (!sky_item(g, pm, mm, nn, slot, ss, it)) return 0;
+    HIPCHK(hipStreamSynchronize(stream));  // the pinned staging buffer may still feed a copy
+    status = sky_batch(g, &it, 1, 0);
     return 1;
   }
 
@@ -571,7 +615,7 @@ template <typename T> struct Engine final : pgw_handle {
     RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2)};
     launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, stream);
     if (ev) HIPCHK(hipEventRecord(ev[2], stream));
-    ThrParams tp{g.ps->L1, g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, o->white_flag, t0, kCandCap, o->trl};
+    ThrParams tp{g.ps->L1, g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, o->white_flag, t0, kCandCap, ilog2(g.ps->L2), o->trl};
     hipLaunchKernelGGL((k_threshold<T>), dim3(g.nparts, nt), dim3(64), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p,
                        cand.p + (size_t)buf * kCandCap, ncand.p + buf);
     if (ev) HIPCHK(hipEventRecord(ev[3], stream));
@@ -586,11 +630,10 @@ template <typename T> struct Engine final : pgw_handle {
   // Queues a group: sky stages into slots, descriptors, template launches, async candidate copy.
   int launch_group(Group& gr, const pgw_opts* o) {
     int st = PGW_OK;
-    for (Item& it : gr.items) {
-      SkyState sk;
-      if (!sky_stage(*segs[it.seg], it.pm, it.mm, it.nn, it.slot, sk, st) || st) return st ? st : PGW_ERR_DEVICE;
-      it.sk = sk;
-    }
+    SkyItem si[kSlots];
+    for (Item& it : gr.items)
+      if (!sky_item(*segs[it.seg], it.pm, it.mm, it.nn, it.slot, it.sk, si[it.slot])) return PGW_ERR_DEVICE;
+    if ((st = sky_batch(*segs[gr.items[0].seg], si, (int)gr.items.size(), gr.buf))) return st;
     TmplDesc* hd = desc_pin.p + (size_t)gr.buf * kDescCap;
     for (const Item& it : gr.items) fill_desc(it, o, hd);
     HIPCHK(hipMemcpyAsync(desc.p + (size_t)gr.buf * kDescCap, hd, sizeof(TmplDesc) * gr.ntempl, hipMemcpyHostToDevice,
