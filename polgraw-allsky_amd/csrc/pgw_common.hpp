@@ -104,12 +104,64 @@ template <int DIR, typename T2> __device__ __forceinline__ void dft16(T2* v) {
   for (int k = 0; k < 8; k++) { v[k] = cadd(e[k], o[k]); v[k + 8] = csub(e[k], o[k]); }
 }
 
+// a * exp(DIR 2 pi i k / R), R in {4, 8, 16}, 0 < k < R/2 (k a compile-time constant after
+// unrolling), as the twiddles of dft4/dft8/dft16
+template <int R, int DIR, typename T2> __device__ __forceinline__ T2 rot_r(T2 a, int k) {
+  const int K = k * (16 / R);
+  switch (K) {
+    case 1: return rot16<DIR, 1>(a);
+    case 2: return rot16<DIR, 2>(a);
+    case 3: return rot16<DIR, 3>(a);
+    case 4: return mul_w4<DIR>(a);
+    case 5: return rot16<DIR, 5>(a);
+    case 6: return rot16<DIR, 6>(a);
+    case 7: return rot16<DIR, 7>(a);
+    default: return a;
+  }
+}
+
 template <int R, int DIR, typename T2> __device__ __forceinline__ void dftR(T2* v) {
   if constexpr (R == 2) dft2<DIR>(v);
   else if constexpr (R == 4) dft4<DIR>(v);
   else if constexpr (R == 8) dft8<DIR>(v);
   else if constexpr (R == 16) dft16<DIR>(v);
   else static_assert(R == 2, "unsupported radix");
+}
+
+// R-point DFT of v[0..R-1] whose inputs v[r], r >= NZ, are zero (input pruning): the same
+// even/odd split as dft4/dft8/dft16, with sub-transforms of one nonzero input reduced to copies
+// and of none to zeros, so the result equals dftR's on such inputs.
+template <int R, int NZ, int DIR, typename T2> __device__ __forceinline__ void dftP(T2* v) {
+  if constexpr (NZ >= R) {
+    dftR<R, DIR>(v);
+  } else if constexpr (NZ <= 0) {
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = mk<T2>(0, 0);
+  } else if constexpr (NZ == 1) {
+#pragma unroll
+    for (int r = 1; r < R; r++) v[r] = v[0];
+  } else {
+    constexpr int H = R / 2;
+    T2 e[H], o[H];
+#pragma unroll
+    for (int i = 0; i < H; i++) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+    dftP<H, (NZ + 1) / 2, DIR>(e);
+    dftP<H, NZ / 2, DIR>(o);
+    if constexpr (NZ / 2 > 0) {
+#pragma unroll
+      for (int k = 1; k < H; k++) o[k] = rot_r<R, DIR>(o[k], k);
+    }
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+      if constexpr (NZ / 2 > 0) {
+        v[k] = cadd(e[k], o[k]);
+        v[k + H] = csub(e[k], o[k]);
+      } else {
+        v[k] = e[k];
+        v[k + H] = e[k];
+      }
+    }
+  }
 }
 
 // ------------------------------------------------------------ Stockham stage

@@ -73,14 +73,15 @@ template <int P2, int E> struct ColShape {
 // NA arrays x[a][r] (registers) -> X in registers: out[a][q*G + kg] = X[kr + E kg], kr = grp*Q + q.
 // lds must hold NA * P2 * C elements.  twc[m] = W_P2^m (forward sign).  One barrier inside; the
 // caller must barrier before reusing lds.
-template <typename T2, int P2, int E, int NA, int DIR>
+// NZ < E: the inputs x[a][r], r >= NZ, are zero (pruned first DFT).
+template <typename T2, int P2, int E, int NA, int DIR, int NZ = E>
 __device__ __forceinline__ void col_fft(T2 (&x)[NA][E], T2* lds, const T2* __restrict__ twc) {
   using S = ColShape<P2, E>;
   const int c = threadIdx.x % S::C, g = threadIdx.x / S::C;
   const T2 wg = twc[g];
 #pragma unroll
   for (int a = 0; a < NA; a++) {
-    dftR<E, DIR>(x[a]);
+    dftP<E, NZ, DIR>(x[a]);
     twiddle_run<E, DIR>(x[a], wg);  // W_P2^{g kr}
 #pragma unroll
     for (int kr = 0; kr < E; kr++) lds[(size_t)a * P2 * S::C + (g * E + kr) * S::C + c] = x[a][kr];

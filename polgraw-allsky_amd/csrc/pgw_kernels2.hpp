@@ -19,7 +19,9 @@ template <> struct ColE<double, 256> { static constexpr int E = 16; static const
 // grid (L1/C, ntemplates), 256 threads.  SEQ: the two spectra share one LDS buffer in turn.  For template t (ss = ss0 + t) and detector d:
 //   x_{a,b}[n] += xm[d][{a,b}][n] exp(-i phi_d(n)),  phi_d(n) = het1 n + sgnlt1 (n^2 + 2 n shft_d[n]),
 // zero for n >= N; the column FFT and the W_L^{n1 k2} twiddle follow; Z[t][{a,b}][k2][n1].
-template <typename T, int P2, bool SEQ, bool UNI>
+// NZ: register rows r >= NZ hold n >= L1 P2 NZ / E >= N for every template of the launch
+// (zero-padded input), so they are neither demodulated nor transformed.
+template <typename T, int P2, bool SEQ, bool UNI, int NZ = ColE<T, P2>::E>
 __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, const SpinTab tab,
                                                    const TmplDesc* __restrict__ desc,
                                                    cplx<T>* __restrict__ Z, const cplx<T>* __restrict__ twc,
@@ -55,7 +57,7 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
   for (int r = 0; r < E; r++) {
     const int n = n1 + L1 * (g + S::G * r);
     T2 xa = mk<T2>(0, 0), xb = mk<T2>(0, 0);
-    if (n < N) {
+    if (r < NZ && n < N) {
       const double dn = (double)n;
       for (int d = 0; d < nifo; d++) {
         const double sh = ld_global(shft, (size_t)d * N + n);
@@ -73,7 +75,7 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
   T2 w[E];
   col_out_twiddles<P2, E>(tw_big(hi, lo, (uint32_t)n1 << p.twshift), w);
   if constexpr (!SEQ) {
-    col_fft<T2, P2, E, 2, -1>(x, lds, twc);
+    col_fft<T2, P2, E, 2, -1, NZ>(x, lds, twc);
 #pragma unroll
     for (int s = 0; s < E; s++) {
       const int k2 = col_out_k2<P2, E>(s);
@@ -85,7 +87,7 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
     for (int a = 0; a < 2; a++) {
       T2 (&xa)[1][E] = *reinterpret_cast<T2(*)[1][E]>(&x[a]);
       if (a) __syncthreads();
-      col_fft<T2, P2, E, 1, -1>(xa, lds, twc);
+      col_fft<T2, P2, E, 1, -1, NZ>(xa, lds, twc);
 #pragma unroll
       for (int s = 0; s < E; s++) {
         const int k2 = col_out_k2<P2, E>(s);

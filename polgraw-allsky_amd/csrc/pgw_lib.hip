@@ -159,13 +159,24 @@ void launch_row(const Plan<T>& pl, cplx<T>* Z, int nbatch, hipStream_t st) {
 #undef PGW_ROW_CASE
 }
 
+// Nl: the largest N of the launch's templates; register rows of the column pass that only see
+// zero padding for that N are pruned (three variants: all E rows, E/2, 3E/8).
 template <typename T>
 void launch_spin_col(const Plan<T>& pl, const SlotBases& sb, const SpinTab& tab, const TmplDesc* desc, cplx<T>* Z,
-                     int nt, const cplx<T>* hi, const cplx<T>* lo, bool seq, hipStream_t st) {
+                     int nt, int Nl, const cplx<T>* hi, const cplx<T>* lo, bool seq, hipStream_t st) {
   const SpinParams sp{pl.L1, pl.twshift};
+#define PGW_SCOL_NZ(P2, SQ, U, NZ)                                                                        \
+  hipLaunchKernelGGL((k_spin_col2<T, P2, SQ, U, NZ>), dim3(pl.L1 / col_tile<T, P2>(), nt), dim3(256), 0, st, sp, \
+                     sb, tab, desc, Z, pl.twc.p, hi, lo)
 #define PGW_SCOL_GO(P2, SQ, U)                                                                           \
-  hipLaunchKernelGGL((k_spin_col2<T, P2, SQ, U>), dim3(pl.L1 / col_tile<T, P2>(), nt), dim3(256), 0, st, sp, sb, \
-                     tab, desc, Z, pl.twc.p, hi, lo)
+  do {                                                                                                   \
+    constexpr int E_ = ColE<T, P2>::E;                                                                   \
+    const long span = (long)pl.L1 * (P2 / E_); /* n of register row r starts at r * span */              \
+    const long rows = (Nl + span - 1) / span;                                                            \
+    if (rows <= 3 * E_ / 8) PGW_SCOL_NZ(P2, SQ, U, 3 * E_ / 8);                                          \
+    else if (rows <= E_ / 2) PGW_SCOL_NZ(P2, SQ, U, E_ / 2);                                             \
+    else PGW_SCOL_NZ(P2, SQ, U, E_);                                                                     \
+  } while (0)
 #define PGW_SCOL_CASE(P2)                                                                                \
   case P2:                                                                                               \
     if (seq || ColE<T, P2>::SEQ) {                                                                       \
@@ -183,6 +194,7 @@ void launch_spin_col(const Plan<T>& pl, const SlotBases& sb, const SpinTab& tab,
   }
 #undef PGW_SCOL_CASE
 #undef PGW_SCOL_GO
+#undef PGW_SCOL_NZ
 }
 
 template <typename T>
@@ -598,7 +610,7 @@ template <typename T> struct Engine final : pgw_handle {
 
   // template stage for descriptors [t0, t0 + nt) of buffer buf; ev (optional) = 4 events around the kernels
   // uni: all nt templates belong to one item (its templates j0, j0 + 1, ...), else nullptr
-  int launch_templates(const Seg& g, int buf, int t0, int nt, const Item* uni, int j0, const pgw_opts* o,
+  int launch_templates(const Seg& g, int buf, int t0, int nt, const Item* uni, int j0, int Nl, const pgw_opts* o,
                        hipEvent_t* ev = nullptr) {
     const TmplDesc* d = desc.p + (size_t)buf * kDescCap + t0;
     if (ev) HIPCHK(hipEventRecord(ev[0], stream));
@@ -610,7 +622,7 @@ template <typename T> struct Engine final : pgw_handle {
       const SpinGrid q = spin_grid(segs[uni->seg]->sett.M, uni->mm, uni->nn, 0, o->s0_flag);
       for (int i = 0; i < nt; i++) spin_coeffs(q, uni->ss_at(j0 + i), tab.het1[i], tab.sgnlt1[i]);
     }
-    launch_spin_col<T>(*g.ps, sb, tab, d, Z.p, nt, hi.p, lo.p, col_seq, stream);
+    launch_spin_col<T>(*g.ps, sb, tab, d, Z.p, nt, uni ? segs[uni->seg]->sett.N : Nl, hi.p, lo.p, col_seq, stream);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));
     RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2)};
     launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, stream);
@@ -640,13 +652,15 @@ template <typename T> struct Engine final : pgw_handle {
                           stream));
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p + gr.buf, 1);
     const Seg& g0 = *segs[gr.items[0].seg];
+    int Ng = 0;  // largest N of the group
+    for (const Item& it : gr.items) Ng = std::max(Ng, segs[it.seg]->sett.N);
     size_t k = 0;
     for (int t0 = 0; t0 < gr.ntempl; t0 += TB) {
       const int nt = std::min(TB, gr.ntempl - t0);
       while (k + 1 < gr.items.size() && gr.items[k + 1].t0 <= t0) k++;
       const Item& it = gr.items[k];
       const bool one = t0 + nt <= it.t0 + it.ns();
-      if ((st = launch_templates(g0, gr.buf, t0, nt, one ? &it : nullptr, t0 - it.t0, o))) return st;
+      if ((st = launch_templates(g0, gr.buf, t0, nt, one ? &it : nullptr, t0 - it.t0, Ng, o))) return st;
     }
     HIPCHK(hipMemcpyAsync(hn_pin.p + gr.buf, ncand.p + gr.buf, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(hc_pin.p + (size_t)gr.buf * kCandChunk, cand.p + (size_t)gr.buf * kCandCap,
@@ -889,7 +903,7 @@ template <typename T> struct Engine final : pgw_handle {
       acc[0] = t;
       for (int rr = 0; rr < reps && !st; rr++) {
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p, 1);
-        st = launch_templates(g, 0, 0, nt, &it, 0, o, ev);
+        st = launch_templates(g, 0, 0, nt, &it, 0, g.sett.N, o, ev);
         HIPCHK(hipStreamSynchronize(stream));
         for (int k = 0; k < 3; k++) {
           HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
