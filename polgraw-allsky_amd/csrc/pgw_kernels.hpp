@@ -483,18 +483,24 @@ __global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restri
   const int hik = (part < p.nblk) ? lok + PGW_NAV_DEV : p.nmax;
   const int mL2 = p.L2 - 1;
   auto at = [&](int k) { return (double)F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)]; };
-  T f[NJ];
-#pragma unroll
-  for (int j = 0; j < NJ; j++) {
-    const int k = min(lok + lane + 64 * j, hik - 1);
-    f[j] = F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)];
-  }
+  // (16 loads per batch keep the kernel's register count, and so the occupancy of the far more
+  // common unflagged waves, low)
   const double pre = p.trl * m * (1. - 1e-14);
   uint64_t hits = 0;
+#pragma unroll 1
+  for (int j0 = 0; j0 < NJ; j0 += 16) {
+    T f[16];
 #pragma unroll
-  for (int j = 0; j < NJ; j++)
-    hits |= (uint64_t)(lok + lane + 64 * j < hik && (double)f[j] > pre) << j;
+    for (int j = 0; j < 16; j++) {
+      const int k = min(lok + lane + 64 * (j0 + j), hik - 1);
+      f[j] = F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)];
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+      hits |= (uint64_t)(lok + lane + 64 * (j0 + j) < hik && (double)f[j] > pre) << (j0 + j);
+  }
   // means of all parts with the unflagged waves' arithmetic, 8 parts' loads in flight at a time
+#pragma unroll 1
   for (int q0 = 0; q0 < p.nparts; q0 += 8) {
     double v[8];
 #pragma unroll

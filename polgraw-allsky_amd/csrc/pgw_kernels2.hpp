@@ -6,6 +6,13 @@
 
 namespace pgw {
 
+#ifndef PGW_ROW64_W
+#define PGW_ROW64_W 1
+#endif
+#ifndef PGW_ROW64_PREF
+#define PGW_ROW64_PREF true
+#endif
+
 // column-pass register depth per precision and column length (LDS per array = 256*E elements)
 template <typename T, int P2> struct ColE;
 template <int P2> struct ColE<float, P2> { static constexpr int E = (P2 <= 64 ? 8 : 16); static constexpr bool SEQ = false; };
@@ -106,7 +113,20 @@ template <typename T, int P1, int NA> struct RowOcc {
   static constexpr int W = 1;
   static constexpr bool PREF = true;
 };
-template <typename T, int P1, int NA>
+#ifndef PGW_ROW32_W
+#define PGW_ROW32_W 1
+#endif
+template <int P1> struct RowOcc<float, P1, 1> {
+  static constexpr int W = PGW_ROW32_W;
+  static constexpr bool PREF = true;
+};
+template <int P1> struct RowOcc<double, P1, 1> {
+  static constexpr int W = PGW_ROW64_W;
+  static constexpr bool PREF = PGW_ROW64_PREF;
+};
+// HALF: nmax <= L/2, so only k1 < P1/2 (the thread's outputs m < 8) can hold a bin of
+// [nmin, nmax); the other half of the last FFT stage is never formed.
+template <typename T, int P1, int NA, bool HALF>
 __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(RowParams p, const double* __restrict__ aabb_slots,
                                                       const TmplDesc* __restrict__ desc,
                                                       const cplx<T>* __restrict__ Z,
@@ -124,14 +144,15 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
   // F = |xa|^2/aa + |xb|^2/bb (jobcore.c:635-638) with the two reciprocals taken once: each
   // product differs from the quotient by at most one rounding
   const T raa = (T)(1.0 / ld_global(aabb, 0)), rbb = (T)(1.0 / ld_global(aabb, 1));
-  T F[16];
+  constexpr int MO = HALF ? 8 : 16;  // output slots that can hold a bin of [nmin, nmax)
+  T F[MO];
   if constexpr (NA == 2) {
     T2 x[2][16];
 #pragma unroll
     for (int m = 0; m < 16; m++) { x[0][m] = rowa[tid + NT * m]; x[1][m] = rowb[tid + NT * m]; }
     row_fft<T2, P1, 2, -1>(x, lds, twr);
 #pragma unroll
-    for (int m = 0; m < 16; m++)
+    for (int m = 0; m < MO; m++)
       F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * raa + (x[1][m].x * x[1][m].x + x[1][m].y * x[1][m].y) * rbb;
   } else {
     constexpr bool PREF = RowOcc<T, P1, NA>::PREF;
@@ -144,13 +165,13 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
     }
     row_fft<T2, P1, 1, -1>(x, lds, twr);
 #pragma unroll
-    for (int m = 0; m < 16; m++) F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * raa;
+    for (int m = 0; m < MO; m++) F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * raa;
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 16; m++) x[0][m] = PREF ? y[0][PREF ? m : 0] : rowb[tid + NT * m];
     row_fft<T2, P1, 1, -1>(x, lds, twr);
 #pragma unroll
-    for (int m = 0; m < 16; m++) F[m] = F[m] + (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * rbb;
+    for (int m = 0; m < MO; m++) F[m] = F[m] + (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * rbb;
   }
   __syncthreads();
   T* Fs = reinterpret_cast<T*>(lds);
@@ -158,10 +179,14 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const int k1 = tid + NT * m;
-    const int k = p.L2 * k1 + k2;
-    const bool ok = (k >= p.nmin && k < p.nmax);
-    if (ok) Frow[k1] = F[m];
-    Fs[k1] = ok ? F[m] : (T)0;
+    if (m < MO) {
+      const int k = p.L2 * k1 + k2;
+      const bool ok = (k >= p.nmin && k < p.nmax);
+      if (ok) Frow[k1] = F[m < MO ? m : 0];
+      Fs[k1] = ok ? F[m < MO ? m : 0] : (T)0;
+    } else {
+      Fs[k1] = (T)0;
+    }
   }
   __syncthreads();
   // FStat block partials of this row: block b holds bins k in [nmin + 4096 b, nmin + 4096 (b+1))

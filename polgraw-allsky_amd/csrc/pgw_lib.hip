@@ -199,15 +199,20 @@ void launch_spin_col(const Plan<T>& pl, const SlotBases& sb, const SpinTab& tab,
 
 template <typename T>
 void launch_spin_row(const Plan<T>& pl, const RowParams& rp, const double* aabb, const TmplDesc* desc, const cplx<T>* Z, T* Ft, T* psum,
-                     T* pmax, int nt, int na, hipStream_t st) {
+                     T* pmax, int nt, int na, bool allow_half, hipStream_t st) {
+  // nmax <= L/2 (always so with the reference's settings): the upper half of each row's outputs
+  // holds no bin of [nmin, nmax)
+  const bool half = allow_half && 2L * rp.nmax <= (long)pl.L1 * pl.L2;
+#define PGW_SROW_GO(P1, NA, H)                                                                            \
+  hipLaunchKernelGGL((k_spin_row2<T, P1, NA, H>), dim3(pl.L2, nt), dim3(P1 / 16), 0, st, rp, aabb, desc, Z, pl.twr.p, \
+                     Ft, psum, pmax)
 #define PGW_SROW_CASE(P1)                                                                                 \
   case P1:                                                                                                \
-    if (na == 2)                                                                                          \
-      hipLaunchKernelGGL((k_spin_row2<T, P1, 2>), dim3(pl.L2, nt), dim3(P1 / 16), 0, st, rp, aabb, desc, Z, pl.twr.p, Ft, \
-                         psum, pmax);                                                                     \
-    else                                                                                                  \
-      hipLaunchKernelGGL((k_spin_row2<T, P1, 1>), dim3(pl.L2, nt), dim3(P1 / 16), 0, st, rp, aabb, desc, Z, pl.twr.p, Ft, \
-                         psum, pmax);                                                                     \
+    if (na == 2) {                                                                                        \
+      if (half) PGW_SROW_GO(P1, 2, true); else PGW_SROW_GO(P1, 2, false);                                 \
+    } else {                                                                                              \
+      if (half) PGW_SROW_GO(P1, 1, true); else PGW_SROW_GO(P1, 1, false);                                 \
+    }                                                                                                     \
     break;
   switch (pl.L1) {
     PGW_SROW_CASE(1024)
@@ -215,6 +220,7 @@ void launch_spin_row(const Plan<T>& pl, const RowParams& rp, const double* aabb,
     PGW_SROW_CASE(4096)
   }
 #undef PGW_SROW_CASE
+#undef PGW_SROW_GO
 }
 
 // lin2ast, search/network/src-cpu/auxi.c:15-24 (host; same expression order)
@@ -311,6 +317,7 @@ template <typename T> struct Engine final : pgw_handle {
   int row_na = 1;
   bool col_seq = false;
   int spl_k = 8;  // spline outputs per thread (4, 8 or 16)
+  bool row_half = true;  // row pass forms only the outputs that can fall in [nmin, nmax)
 
   ~Engine() override {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -411,7 +418,8 @@ template <typename T> struct Engine final : pgw_handle {
     int st;
     for (int k = 0; k < nseg; k++)
       if ((st = add_segment(&s[k], ifo + (size_t)k * ndet))) return st;
-    // kernel variants; PGW_TUNE="row_na=2,col_seq=1,tb=64,group=4" overrides (used when measuring variants)
+    // kernel variants; PGW_TUNE="row_na=2,col_seq=1,tb=64,group=4,spl_k=8,half=1" overrides (used when
+    // measuring variants)
     if (const char* tune = getenv("PGW_TUNE")) {
       int v;
       if (const char* q = strstr(tune, "row_na=")) if (sscanf(q, "row_na=%d", &v) == 1 && (v == 1 || v == 2)) row_na = v;
@@ -419,6 +427,7 @@ template <typename T> struct Engine final : pgw_handle {
       if (const char* q = strstr(tune, "tb=")) if (sscanf(q, "tb=%d", &v) == 1 && v >= 1 && v <= kTBMax) TB = v;
       if (const char* q = strstr(tune, "group=")) if (sscanf(q, "group=%d", &v) == 1 && v >= 1 && v <= kSlots) group = v;
       if (const char* q = strstr(tune, "spl_k=")) if (sscanf(q, "spl_k=%d", &v) == 1 && (v == 4 || v == 8 || v == 16)) spl_k = v;
+      if (const char* q = strstr(tune, "half=")) if (sscanf(q, "half=%d", &v) == 1) row_half = v != 0;
     }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
@@ -625,7 +634,7 @@ template <typename T> struct Engine final : pgw_handle {
     launch_spin_col<T>(*g.ps, sb, tab, d, Z.p, nt, uni ? segs[uni->seg]->sett.N : Nl, hi.p, lo.p, col_seq, stream);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));
     RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2)};
-    launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, stream);
+    launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, row_half, stream);
     if (ev) HIPCHK(hipEventRecord(ev[2], stream));
     ThrParams tp{g.ps->L1, g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, o->white_flag, t0, kCandCap, ilog2(g.ps->L2), o->trl};
     hipLaunchKernelGGL((k_threshold<T>), dim3(g.nparts, nt), dim3(64), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p,
