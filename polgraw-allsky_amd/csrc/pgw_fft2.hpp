@@ -202,4 +202,61 @@ __device__ __forceinline__ void row_fft(T2 (&x)[NA][16], T2* lds, const T2* __re
   }
 }
 
+// Row FFT with 8 elements per thread (NT = P1/8 threads): radix-8 Stockham stages with Ns = 1, 8,
+// 64, then radix R = P1/512 with Ns = 512, through one padded LDS array of P1 elements (three
+// exchanges).  Half the registers of row_fft per thread, for FP64.  x[m] = row element
+// (t + NT m) on entry, X[t + NT m] on exit.  Ends after a barrier-free read (caller must barrier
+// before reusing lds).
+template <int P1> struct Row8Shape {
+  static constexpr int NT = P1 / 8;
+  static constexpr int RL = P1 / 512;           // last radix
+  static constexpr int PADN = P1 + P1 / 16;     // padded elements
+  static_assert(RL == 2 || RL == 4 || RL == 8, "row length 1024, 2048 or 4096");
+};
+
+template <typename T2, int P1, int DIR>
+__device__ __forceinline__ void row_fft8(T2 (&x)[8], T2* lds, const T2* __restrict__ twr) {
+  using S = Row8Shape<P1>;
+  constexpr int NT = S::NT, RL = S::RL;
+  const int t = threadIdx.x;
+  const T2 wB = twr[(t & 7) * (P1 / 64)];     // W_64^k, k = t mod 8
+  const T2 wC = twr[(t & 63) * (P1 / 512)];   // W_512^k, k = t mod 64
+  T2 wD[8 / RL];
+#pragma unroll
+  for (int i = 0; i < 8 / RL; i++) wD[i] = twr[t + NT * i];  // W_P1^b, b = t + NT i
+  // stage A: radix 8, Ns = 1: outputs at 8 t + r
+  dftR<8, DIR>(x);
+#pragma unroll
+  for (int r = 0; r < 8; r++) lds[rpad(8 * t + r)] = x[r];
+  __syncthreads();
+  // stages B (Ns = 8) and C (Ns = 64): butterfly j = t reads j + NT r, writes (j - k) 8 + k + Ns r
+#pragma unroll
+  for (int st = 0; st < 2; st++) {
+    const int Ns = st == 0 ? 8 : 64;
+    const int k = t & (Ns - 1);
+#pragma unroll
+    for (int r = 0; r < 8; r++) x[r] = lds[rpad(t + NT * r)];
+    twiddle_run<8, DIR>(x, st == 0 ? wB : wC);
+    dftR<8, DIR>(x);
+    __syncthreads();
+    const int base = (t - k) * 8 + k;
+#pragma unroll
+    for (int r = 0; r < 8; r++) lds[rpad(base + Ns * r)] = x[r];
+    __syncthreads();
+  }
+  // stage D: radix RL, Ns = 512: butterflies b = t + NT i (i < 8/RL) read b + 512 r (k = b);
+  // outputs b + 512 r = t + NT (i + (8/RL) r)
+#pragma unroll
+  for (int i = 0; i < 8 / RL; i++) {
+    const int b = t + NT * i;
+    T2 v[RL];
+#pragma unroll
+    for (int r = 0; r < RL; r++) v[r] = lds[rpad(b + 512 * r)];
+    twiddle_run<RL, DIR>(v, wD[i]);
+    dftR<RL, DIR>(v);
+#pragma unroll
+    for (int r = 0; r < RL; r++) x[i + (8 / RL) * r] = v[r];
+  }
+}
+
 }  // namespace pgw

@@ -236,6 +236,102 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
   }
 }
 
+// Row pass with 8 elements per thread (row_fft8; P1/8 threads, two workgroups per CU within 128
+// VGPRs): the FP64 form of k_spin_row2, same outputs.  The spectra are transformed in turn through
+// one LDS array, the second row's loads in flight during the first transform (PREF).
+template <typename T, int P1, bool HALF, bool PREF>
+__global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const double* __restrict__ aabb_slots,
+                                                         const TmplDesc* __restrict__ desc,
+                                                         const cplx<T>* __restrict__ Z,
+                                                         const cplx<T>* __restrict__ twr, T* __restrict__ Ft,
+                                                         T* __restrict__ psum, T* __restrict__ pmax) {
+  using T2 = cplx<T>;
+  using S = Row8Shape<P1>;
+  constexpr int NT = S::NT;
+  __shared__ T2 lds[S::PADN];
+  const int k2 = blockIdx.x, t = blockIdx.y, tid = threadIdx.x;
+  const size_t L = (size_t)P1 * p.L2;
+  const T2* rowa = Z + (size_t)t * 2 * L + (size_t)k2 * P1;
+  const T2* rowb = rowa + L;
+  const double* aabb = aabb_slots + 2 * desc[t].slot;
+  const T raa = (T)(1.0 / ld_global(aabb, 0)), rbb = (T)(1.0 / ld_global(aabb, 1));
+  constexpr int MO = HALF ? 4 : 8;  // output slots that can hold a bin of [nmin, nmax)
+  T F[MO];
+  T2 x[8], y[PREF ? 8 : 1];
+#pragma unroll
+  for (int m = 0; m < 8; m++) x[m] = rowa[tid + NT * m];
+  if constexpr (PREF) {
+#pragma unroll
+    for (int m = 0; m < 8; m++) y[m] = rowb[tid + NT * m];
+  }
+  row_fft8<T2, P1, -1>(x, lds, twr);
+#pragma unroll
+  for (int m = 0; m < MO; m++) F[m] = (x[m].x * x[m].x + x[m].y * x[m].y) * raa;
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 8; m++) x[m] = PREF ? y[PREF ? m : 0] : rowb[tid + NT * m];
+  row_fft8<T2, P1, -1>(x, lds, twr);
+#pragma unroll
+  for (int m = 0; m < MO; m++) F[m] = F[m] + (x[m].x * x[m].x + x[m].y * x[m].y) * rbb;
+  __syncthreads();
+  T* Fs = reinterpret_cast<T*>(lds);
+  T* Frow = Ft + (size_t)t * L + (size_t)k2 * P1;
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    const int k1 = tid + NT * m;
+    if (m < MO) {
+      const int k = p.L2 * k1 + k2;
+      const bool ok = (k >= p.nmin && k < p.nmax);
+      if (ok) Frow[k1] = F[m < MO ? m : 0];
+      Fs[k1] = ok ? F[m < MO ? m : 0] : (T)0;
+    } else {
+      Fs[k1] = (T)0;
+    }
+  }
+  __syncthreads();
+  // FStat block partials of this row as in k_spin_row2, with runs of 8 consecutive bins per thread
+  // (a block spans 4096 / L2 >= 16 consecutive k1, so a run falls into at most two blocks).
+  T* rs = Fs + P1;   // [2][NT] sums, then [2][NT] maxima
+  {
+    const int j0 = 8 * tid;
+    const int nb = p.nparts - 1;
+    const int b0 = min(max((p.L2 * j0 + k2 - p.nmin) >> 12, 0), nb);
+    const int jsplit = b0 == nb ? P1 : (PGW_NAV_DEV * (b0 + 1) + p.nmin - k2 + p.L2 - 1) >> p.lgL2;
+    T s0 = 0, s1 = 0, m0 = 0, m1 = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int j = j0 + ((i + tid) & 7);  // rotated start: lanes spread over the LDS banks
+      const T f = Fs[j];
+      if (j < jsplit) { s0 += f; m0 = f > m0 ? f : m0; }
+      else { s1 += f; m1 = f > m1 ? f : m1; }
+    }
+    rs[tid] = s0;
+    rs[NT + tid] = s1;
+    rs[2 * NT + tid] = m0;
+    rs[3 * NT + tid] = m1;
+  }
+  __syncthreads();
+  for (int part = tid; part < p.nparts; part += NT) {
+    const int lok = p.nmin + PGW_NAV_DEV * part;
+    const int hik = (part < p.nblk) ? lok + PGW_NAV_DEV : p.nmax;
+    const int jlo = max(0, (lok - k2 + p.L2 - 1) >> p.lgL2);
+    const int jhi = min((hik - k2 + p.L2 - 1) >> p.lgL2, P1);
+    T sum = 0, mx = 0;
+    if (jlo < jhi) {
+      const int nb = p.nparts - 1;
+      for (int u = jlo >> 3; u <= (jhi - 1) >> 3; u++) {
+        const int b0 = min(max((p.L2 * (8 * u) + k2 - p.nmin) >> 12, 0), nb);
+        const int h = b0 == part ? 0 : 1;
+        sum += rs[h * NT + u];
+        const T v = rs[(2 + h) * NT + u];
+        mx = v > mx ? v : mx;
+      }
+    }
+    psum[((size_t)t * p.nparts + part) * p.L2 + k2] = sum;
+    pmax[((size_t)t * p.nparts + part) * p.L2 + k2] = mx;
+  }
+}
+
 // Generic column pass (one array per block): grid (L1/C, nbatch).
 template <typename T, int P2, int DIR, typename Loader>
 __global__ __launch_bounds__(256) void k_fft_col2(Loader ld, cplx<T>* __restrict__ Z, int L1,

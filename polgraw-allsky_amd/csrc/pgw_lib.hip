@@ -20,6 +20,7 @@
 #include <map>
 #include <memory>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pgw.h"
@@ -199,7 +200,7 @@ void launch_spin_col(const Plan<T>& pl, const SlotBases& sb, const SpinTab& tab,
 
 template <typename T>
 void launch_spin_row(const Plan<T>& pl, const RowParams& rp, const double* aabb, const TmplDesc* desc, const cplx<T>* Z, T* Ft, T* psum,
-                     T* pmax, int nt, int na, bool allow_half, hipStream_t st) {
+                     T* pmax, int nt, int na, bool allow_half, int row8, hipStream_t st) {
   // nmax <= L/2 (always so with the reference's settings): the upper half of each row's outputs
   // holds no bin of [nmin, nmax)
   const bool half = allow_half && 2L * rp.nmax <= (long)pl.L1 * pl.L2;
@@ -214,11 +215,32 @@ void launch_spin_row(const Plan<T>& pl, const RowParams& rp, const double* aabb,
       if (half) PGW_SROW_GO(P1, 1, true); else PGW_SROW_GO(P1, 1, false);                                 \
     }                                                                                                     \
     break;
-  switch (pl.L1) {
-    PGW_SROW_CASE(1024)
-    PGW_SROW_CASE(2048)
-    PGW_SROW_CASE(4096)
+#define PGW_SROW8_GO(P1, H, PF)                                                                          \
+  hipLaunchKernelGGL((k_spin_row8<T, P1, H, PF>), dim3(pl.L2, nt), dim3(P1 / 8), 0, st, rp, aabb, desc, Z, pl.twr.p, \
+                     Ft, psum, pmax)
+#define PGW_SROW8_CASE(P1)                                                                                \
+  case P1:                                                                                                \
+    if (row8 == 1) {                                                                                      \
+      if (half) PGW_SROW8_GO(P1, true, true); else PGW_SROW8_GO(P1, false, true);                         \
+    } else {                                                                                              \
+      if (half) PGW_SROW8_GO(P1, true, false); else PGW_SROW8_GO(P1, false, false);                       \
+    }                                                                                                     \
+    break;
+  if (row8) {  // 8 elements per thread (1: second row prefetched, 2: not)
+    switch (pl.L1) {
+      PGW_SROW8_CASE(1024)
+      PGW_SROW8_CASE(2048)
+      PGW_SROW8_CASE(4096)
+    }
+  } else {
+    switch (pl.L1) {
+      PGW_SROW_CASE(1024)
+      PGW_SROW_CASE(2048)
+      PGW_SROW_CASE(4096)
+    }
   }
+#undef PGW_SROW8_CASE
+#undef PGW_SROW8_GO
 #undef PGW_SROW_CASE
 #undef PGW_SROW_GO
 }
@@ -318,6 +340,9 @@ template <typename T> struct Engine final : pgw_handle {
   bool col_seq = false;
   int spl_k = 8;  // spline outputs per thread (4, 8 or 16)
   bool row_half = true;  // row pass forms only the outputs that can fall in [nmin, nmax)
+  // row pass with 8 elements per thread (1 with the second row prefetched, 2 without): the FP64
+  // default (two workgroups per CU instead of one); FP32 keeps 16 elements per thread
+  int row8 = std::is_same<T, double>::value ? 2 : 0;
 
   ~Engine() override {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -428,6 +453,7 @@ template <typename T> struct Engine final : pgw_handle {
       if (const char* q = strstr(tune, "group=")) if (sscanf(q, "group=%d", &v) == 1 && v >= 1 && v <= kSlots) group = v;
       if (const char* q = strstr(tune, "spl_k=")) if (sscanf(q, "spl_k=%d", &v) == 1 && (v == 4 || v == 8 || v == 16)) spl_k = v;
       if (const char* q = strstr(tune, "half=")) if (sscanf(q, "half=%d", &v) == 1) row_half = v != 0;
+      if (const char* q = strstr(tune, "row8=")) if (sscanf(q, "row8=%d", &v) == 1 && v >= 0 && v <= 2) row8 = v;
     }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
@@ -634,7 +660,7 @@ template <typename T> struct Engine final : pgw_handle {
     launch_spin_col<T>(*g.ps, sb, tab, d, Z.p, nt, uni ? segs[uni->seg]->sett.N : Nl, hi.p, lo.p, col_seq, stream);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));
     RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2)};
-    launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, row_half, stream);
+    launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, row_half, row8, stream);
     if (ev) HIPCHK(hipEventRecord(ev[2], stream));
     ThrParams tp{g.ps->L1, g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, o->white_flag, t0, kCandCap, ilog2(g.ps->L2), o->trl};
     hipLaunchKernelGGL((k_threshold<T>), dim3(g.nparts, nt), dim3(64), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p,
@@ -931,16 +957,20 @@ template <typename T> struct Engine final : pgw_handle {
   // given sky point and keeps the fastest.
   int autotune(int sgi, int pm, int mm, int nn, const pgw_opts* o, int* tb_out, int* na_out, int* seq_out) override {
     const int tbs[3] = {64, 128, 256};
+    // row pass variants: 16 elements per thread with the spectra in turn (na 1) or together (na 2),
+    // 8 elements per thread without / with the second row prefetched (row8 2 / 1)
+    const int rv_na[4] = {1, 2, 1, 1}, rv_r8[4] = {0, 0, 2, 1};
     double best = 1e30;
-    int btb = TB, bna = row_na;
+    int btb = TB, bna = row_na, br8 = row8;
     bool bseq = col_seq;
     for (int tb : tbs) {
       int st = ensure_tb(tb);
       if (st) return st;
-      for (int na = 1; na <= 2; na++)
+      for (int rv = 0; rv < 4; rv++)
         for (int seq = 0; seq <= 1; seq++) {
           TB = tb;
-          row_na = na;
+          row_na = rv_na[rv];
+          row8 = rv_r8[rv];
           col_seq = seq != 0;
           double ms[5];
           if ((st = time_stages(sgi, pm, mm, nn, 0, tb, o, 3, ms))) return st;
@@ -948,16 +978,18 @@ template <typename T> struct Engine final : pgw_handle {
           if (per < best) {
             best = per;
             btb = tb;
-            bna = na;
+            bna = rv_na[rv];
+            br8 = rv_r8[rv];
             bseq = seq != 0;
           }
         }
     }
     TB = btb;
     row_na = bna;
+    row8 = br8;
     col_seq = bseq;
     if (tb_out) *tb_out = TB;
-    if (na_out) *na_out = row_na;
+    if (na_out) *na_out = row8 ? 8 : row_na;
     if (seq_out) *seq_out = col_seq;
     return PGW_OK;
   }

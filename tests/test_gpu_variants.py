@@ -1,0 +1,83 @@
+"""Kernel variants (chosen per process with PGW_TUNE, or by the autotuner) give the oracle's
+records: row pass with 16 or 8 elements per thread, with and without the half-spectrum pruning,
+both spectra together, column pass with shared LDS, spline chunk lengths, launch and group
+sizes.  FP64 to 1e-9 relative in F; FP32 against the FP64 oracle as the precision form is
+specified (every record away from the threshold found, F within 1e-4)."""
+import os
+
+import numpy as np
+import pytest
+
+from cases import injected_sky, segment
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = ["", "row8=0", "row8=1", "row8=2", "half=0", "row8=0,half=0", "row8=2,half=0", "row_na=2,row8=0",
+            "col_seq=1", "spl_k=4", "spl_k=16", "tb=32,group=3"]
+CASES = ["plain-1det-32k", "net-2det-32k", "net-2det-86k"]
+
+
+@pytest.fixture(scope="module")
+def pgw():
+    from polgraw_allsky_amd import pgw as p
+    p.lib()
+    return p
+
+
+@pytest.fixture(scope="module")
+def oracle_runs(orc, abi):
+    out = {}
+    for name in CASES:
+        seg = segment(name)
+        pm, mm, nn = injected_sky(seg)
+        sky = [(pm, mm, nn), (pm, mm, nn + 1)]
+        rng, opts = abi.make_range(seg), abi.make_opts(trl=12.0)
+        S = orc.Search(seg, nthreads=8)
+        recs, fn = [], 0
+        for s in sky:
+            r, f = S.job_core(*s, rng, opts)
+            recs.append(r)
+            fn += f
+        out[name] = (sky, np.concatenate(recs), fn)
+    return out
+
+
+def _run(pgw, abi, seg, prec, tune, sky):
+    old = os.environ.get("PGW_TUNE")
+    os.environ["PGW_TUNE"] = tune
+    try:
+        h = pgw.Handle(seg, prec)
+    finally:
+        if old is None:
+            del os.environ["PGW_TUNE"]
+        else:
+            os.environ["PGW_TUNE"] = old
+    return h.job_batch(sky, abi.make_range(seg), abi.make_opts(trl=12.0))
+
+
+@pytest.mark.parametrize("tune", VARIANTS)
+@pytest.mark.parametrize("name", CASES)
+def test_variant_fp64(pgw, abi, oracle_runs, name, tune):
+    sky, ref, fr = oracle_runs[name]
+    got, fg = _run(pgw, abi, segment(name), abi.PREC_FP64, tune, sky)
+    print(name, repr(tune), "records", len(got), len(ref), flush=True)
+    assert fg == fr and got.shape == ref.shape and len(ref) > 0
+    assert np.array_equal(got[:, :4], ref[:, :4])
+    Fg, Fr = got[:, 4] ** 2 / 2 + 2, ref[:, 4] ** 2 / 2 + 2
+    assert np.max(np.abs(Fg - Fr) / Fr) <= 1e-9
+
+
+@pytest.mark.parametrize("tune", ["", "row8=1", "row8=2", "half=0", "row_na=2", "spl_k=4"])
+@pytest.mark.parametrize("name", CASES)
+def test_variant_fp32(pgw, abi, oracle_runs, name, tune):
+    sky, ref, fr = oracle_runs[name]
+    got, fg = _run(pgw, abi, segment(name), abi.PREC_FP32, tune, sky)
+    print(name, repr(tune), "records", len(got), len(ref), flush=True)
+    assert fg == fr
+    F = lambda snr: snr * snr / 2 + 2  # noqa: E731
+    kg = {tuple(r[:4]): F(r[4]) for r in got}
+    strong = [r for r in ref if F(r[4]) > 12.05]
+    assert strong
+    hit = [r for r in strong if tuple(r[:4]) in kg]
+    assert len(hit) == len(strong)
+    assert max(abs(kg[tuple(r[:4])] - F(r[4])) / F(r[4]) for r in hit) <= 1e-4
