@@ -139,7 +139,9 @@ def measured_flops(kernel):
     return None
 
 
-def roofline(h, seg, opts, prec_bytes, nt=128, reps=10):
+def roofline(h, seg, opts, prec_bytes, tuned=None, nt=256, reps=10):
+    """Template-stage kernels of one launch (the handle's templates per launch, at most nt) timed
+    with HIP events; the dominant one against the HBM roofline."""
     s = seg.sett
     pm, mm, nn = seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2])
     ms = h.time_stages(pm, mm, nn, seg.range["spndr"][0], nt, opts, reps=reps)
@@ -150,7 +152,9 @@ def roofline(h, seg, opts, prec_bytes, nt=128, reps=10):
     b_col = nifo * N * (4 * p + 8) + 4 * L * p     # read xDatma/b + shft, write two L-point spectra
     b_row = 4 * L * p + M * p                      # read the two spectra, write F once
     k = "col" if ms["col"] >= ms["row"] else "row"
-    kname = "k_spin_col2" if k == "col" else "k_spin_row2"
+    # row pass kernel in use: 8 elements per thread when tuned so, or by default for FP64
+    row8 = tuned["row_na"] == 8 if tuned else prec_bytes == 8
+    kname = "k_spin_col2" if k == "col" else ("k_spin_row8" if row8 else "k_spin_row2")
     b = (b_col if k == "col" else b_row) * nt
     achieved = b / (ms[k] * 1e-3) / 1e9
     stage_ms = ms["col"] + ms["row"] + ms["thr"]
@@ -197,6 +201,17 @@ def cpu_baseline(seg, rng, opts, sky=None):
                        f"OpenMP over spindowns; {dt:.1f} s wall)", cpu_model=cpu_model())
 
 
+def tune_handle(args, h, segi, seg, opts):
+    """The library's autotuner on the injected sky point (setup, before any timed step): kernel
+    variants and templates per launch for this handle.  Returns the choice (None with --no-tune)."""
+    if args.no_tune:
+        return None
+    pm, mm, nn = seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2])
+    choice = h.autotune(segi, pm, mm, nn, opts)
+    print(f"autotune: {choice}", file=sys.stderr, flush=True)
+    return choice
+
+
 def base_line(args, value, t, cfg, dtype="fp32", unit="templates/s", metric=METRIC):
     return {"metric": metric, "value": round(value, 1), "unit": unit, "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(1e3 * t / args.steps, 3), "higher_is_better": True,
@@ -217,6 +232,7 @@ def bench_templates(args, name):
     print(f"workload {name}: N={s['N']} nfft={s['nfft']} fftpad={s['fftpad']} L={s['nfftf']} "
           f"sky={len(sky)} spindowns={nspin}", file=sys.stderr, flush=True)
     h32 = pgw.Handle(seg, abi.PREC_FP32)
+    tune = {"fp32": tune_handle(args, h32, 0, seg, opts)}
     n32, t32, g32, r32 = run_path(h32, sky, rng, opts, args.steps, args.warmup, per_step)
     v32 = n32 / t32
     print(f"fp32: {n32} templates in {t32:.3f} s -> {v32:.0f} templates/s (gpu {g32:.1f} ms)",
@@ -224,24 +240,25 @@ def bench_templates(args, name):
     out_fp64, acc = None, None
     if not args.no_fp64:
         h64 = pgw.Handle(seg, abi.PREC_FP64)
+        tune["fp64"] = tune_handle(args, h64, 0, seg, opts)
         n64, t64, g64, r64 = run_path(h64, sky, rng, opts, args.steps, args.warmup, per_step)
         v64 = n64 / t64
         print(f"fp64: {n64} templates in {t64:.3f} s -> {v64:.0f} templates/s (gpu {g64:.1f} ms)",
               file=sys.stderr, flush=True)
         out_fp64 = dict(value=round(v64, 1), ms_per_step=round(1e3 * t64 / args.steps, 3),
                         gpu_ms_per_step=round(g64 / args.steps, 3), templates=n64,
-                        roofline=roofline(h64, seg, opts, 8))
+                        roofline=roofline(h64, seg, opts, 8, tune.get("fp64")))
         acc = cand_compare(r32, r64)
         mabs, mrel = fstat_compare(h32, h64, seg, opts)
         acc.update(max_abs_dF=mabs, max_rel_dF_F_ge_1=mrel)
         h64.close()
-    rl = roofline(h32, seg, opts, 4)
+    rl = roofline(h32, seg, opts, 4, tune.get("fp32"))
     cpu = None if args.no_cpu else cpu_baseline(seg, rng, opts)
     if cpu:
         print(f"cpu: {cpu}", file=sys.stderr, flush=True)
     cfg = {"workload": TEMPLATE_WORKLOADS[name][1], "N": s["N"], "nfft": s["nfft"], "fftpad": s["fftpad"],
            "L": s["nfftf"], "detectors": len(seg.dets), "sky_points": len(sky), "spindowns": nspin,
-           "sky_per_step": per_step}
+           "sky_per_step": per_step, "tuning": tune}
     line = base_line(args, v32, t32, cfg)
     line.update({"templates": n32, "gpu_ms_per_step": round(g32 / args.steps, 3), "fp64": out_fp64,
                  "speedup_fp32_vs_fp64": (round(v32 / out_fp64["value"], 3) if out_fp64 else None),
@@ -273,8 +290,14 @@ def bench_batching(args):
           file=sys.stderr, flush=True)
     opts = abi.make_opts(trl=20.0)
 
+    tune = {}
+
+    # the autotuner runs on a segment of the longer FFT length (most of the templates)
+    kt = max(range(len(segs)), key=lambda k: (segs[k].sett["nfftf"], -k))
+
     def run(prec):
         h = pgw.Handle(segs, prec)
+        tune["fp32" if prec == abi.PREC_FP32 else "fp64"] = tune_handle(args, h, kt, segs[kt], opts)
         chunks = [work[(i * per_step) % len(work):(i * per_step) % len(work) + per_step]
                   for i in range(args.warmup + args.steps)]
         # each segment's spindowns come from its own range; the call takes one range, so the
@@ -300,14 +323,15 @@ def bench_batching(args):
         v64 = n64 / t64
         print(f"fp64: {n64} templates in {t64:.3f} s -> {v64:.0f} templates/s", file=sys.stderr, flush=True)
         out_fp64 = dict(value=round(v64, 1), ms_per_step=round(1e3 * t64 / args.steps, 3), templates=n64,
-                        roofline=roofline(h64, segs[0], opts, 8))
+                        roofline=roofline(h64, segs[0], opts, 8, tune.get("fp64")))
         acc = cand_compare(r32, r64)
         h64.close()
-    rl = roofline(h32, segs[0], opts, 4)
+    rl = roofline(h32, segs[0], opts, 4, tune.get("fp32"))
     cpu = None if args.no_cpu else cpu_baseline(segs[0], abi.make_range(segs[0]), opts)
     cfg = {"workload": "batching: 3 detectors (H1, L1, V1), 64 segments of N in [60000, 86400] with 5-30 % gaps, "
                        "nfft 2^16 or 2^17, fftpad=1, 32 sky x 512 spindowns per segment, trl=20",
-           "segments": len(segs), "segments_nfft_2^16": n16, "work_items": len(work), "items_per_step": per_step}
+           "segments": len(segs), "segments_nfft_2^16": n16, "work_items": len(work), "items_per_step": per_step,
+           "tuning": tune}
     line = base_line(args, v32, t32, cfg)
     line.update({"templates": n32, "fp64": out_fp64,
                  "speedup_fp32_vs_fp64": (round(v32 / out_fp64["value"], 3) if out_fp64 else None),
@@ -393,6 +417,8 @@ def main():
                     choices=["network", "plain", "precision", "batching", "differentiation"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (kernel variants and templates per launch) before timing")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a plain 256 MiB device copy (scale for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()

@@ -177,8 +177,8 @@ int pgw_fstatnet(pgw_handle *h, int seg, const double *pts, int npts, double *ou
 
 /* Autotuner (no counterpart in the reference): times the template stage of sky point
  * (pm, mm, nn) of segment `seg` for every kernel variant (templates per launch 64/128/256;
- * row pass with 16 elements per thread and both spectra together or in turn, or with 8
- * elements per thread; column pass LDS sharing) and keeps the fastest for this handle.  The
+ * row pass with 16 elements per thread and both spectra together or in turn, or, in the FP64
+ * form, with 8 elements per thread; column pass LDS sharing) and keeps the fastest for this handle.  The
  * chosen values are returned (pointers may be NULL): *row_na is 1 or 2 (16 elements per thread,
  * spectra in turn or together) or 8 (8 elements per thread). */
 int pgw_autotune(pgw_handle *h, int seg, int pm, int mm, int nn, const pgw_opts *opts, int *tb,
