@@ -41,6 +41,9 @@ from polgraw_allsky_amd import abi, pgw, synth  # noqa: E402
 
 METRIC = "F-stat templates/s on 1 MI355X (FP32-batched vs FP64 plain); max rel ΔF vs FP64"
 HBM_PEAK_GBS = 8000.0
+# the template kernels of every KTIME_EVERY-th launch of the timed steps are bracketed by HIP
+# events (each event record costs the following kernels a little; sampling keeps that < 1 %)
+KTIME_EVERY = 4
 
 
 # template-search workloads: make_segment arguments and the config text
@@ -61,12 +64,13 @@ def workload(args, name="network"):
 
 
 def run_path(h, sky, rng, opts, steps, warmup, per_step):
-    """Runs warmup + timed steps; returns (templates, seconds, gpu_ms, records)."""
+    """Runs warmup + timed steps; returns (templates, seconds, gpu_ms, records, kernel times)."""
     nsteps_total = warmup + steps
     chunks = [sky[(i * per_step) % len(sky):(i * per_step) % len(sky) + per_step] for i in range(nsteps_total)]
     for c in chunks[:warmup]:
         h.job_batch(c, rng, opts)
     recs, fnum, gpu_ms = [], 0, 0.0
+    h.kernel_times(KTIME_EVERY)  # HIP events around the template kernels of the timed steps
     t0 = time.perf_counter()
     for c in chunks[warmup:]:
         r, f = h.job_batch(c, rng, opts)
@@ -74,7 +78,8 @@ def run_path(h, sky, rng, opts, steps, warmup, per_step):
         fnum += f
         gpu_ms += h.last_gpu_ms
     dt = time.perf_counter() - t0
-    return fnum, dt, gpu_ms, (np.concatenate(recs) if recs else np.zeros((0, 5)))
+    kt = h.kernel_times(0)
+    return fnum, dt, gpu_ms, (np.concatenate(recs) if recs else np.zeros((0, 5))), kt
 
 
 def cand_compare(r32, r64):
@@ -139,36 +144,45 @@ def measured_flops(kernel):
     return None
 
 
-def roofline(h, seg, opts, prec_bytes, tuned=None, nt=256, reps=10):
-    """Template-stage kernels of one launch (the handle's templates per launch, at most nt) timed
-    with HIP events; the dominant one against the HBM roofline."""
+def alg_bytes(seg, p):
+    """Algorithmic bytes per template of the column and row passes (both stated in DESIGN.md);
+    p = bytes per real of the series and spectra."""
     s = seg.sett
-    pm, mm, nn = seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2])
-    ms = h.time_stages(pm, mm, nn, seg.range["spndr"][0], nt, opts, reps=reps)
-    nt = ms.pop("nt")
-    nifo, N, L, p = len(seg.dets), s["N"], s["nfftf"], prec_bytes
+    nifo, N, L = len(seg.dets), s["N"], s["nfftf"]
     M = s["nmax"] - s["nmin"]
-    # algorithmic bytes per template and kernel (both stated in DESIGN.md)
     b_col = nifo * N * (4 * p + 8) + 4 * L * p     # read xDatma/b + shft, write two L-point spectra
     b_row = 4 * L * p + M * p                      # read the two spectra, write F once
-    k = "col" if ms["col"] >= ms["row"] else "row"
+    return b_col, b_row
+
+
+def roofline(kt, segw, prec_bytes, tuned=None):
+    """The dominant template kernel against the HBM roofline.  kt: the library's HIP-event kernel
+    timing over the timed steps (pgw_kernel_times); segw: [(segment, templates)] of those steps
+    for the algorithmic bytes."""
+    tot = sum(w for _, w in segw)
+    b_col = sum(w * alg_bytes(sg, prec_bytes)[0] for sg, w in segw) / tot
+    b_row = sum(w * alg_bytes(sg, prec_bytes)[1] for sg, w in segw) / tot
+    n, nl = kt["templates"], kt["launches"]
+    k = "col" if kt["col_ms"] >= kt["row_ms"] else "row"
     # row pass kernel in use: 8 elements per thread when tuned so, or by default for FP64
     row8 = tuned["row_na"] == 8 if tuned else prec_bytes == 8
     kname = "k_spin_col2" if k == "col" else ("k_spin_row8" if row8 else "k_spin_row2")
-    b = (b_col if k == "col" else b_row) * nt
-    achieved = b / (ms[k] * 1e-3) / 1e9
-    stage_ms = ms["col"] + ms["row"] + ms["thr"]
-    b_t = (b_col + b_row) * nt
+    achieved = (b_col if k == "col" else b_row) * n / (kt[k + "_ms"] * 1e-3) / 1e9
+    stage_ms = kt["col_ms"] + kt["row_ms"] + kt["thr_ms"]
     prec = "fp32" if prec_bytes == 4 else "fp64"
     tr = measured_traffic(kname, prec)
+    ntl = n / nl
     # the kernels also issue close to the VALU limit: counter-measured fraction of VALU issue
     # cycles (SQ_INSTS_VALU x 4 / SIMD cycles) from the committed profile
     return dict(bound="hbm", kernel=kname, achieved=round(achieved, 1),
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
-                traffic=(int(tr * nt) if tr else None), traffic_unit="bytes per launch",
+                traffic=(int(tr * ntl) if tr else None), traffic_unit="bytes per launch",
                 valu_busy=measured_valu(kname, prec),
-                templates_per_launch=nt, ms_per_launch=ms, bytes_per_template=dict(col=b_col, row=b_row),
-                template_stage_GBps=round(b_t / (stage_ms * 1e-3) / 1e9, 1))
+                timing=f"HIP events around every {KTIME_EVERY}th template launch of the timed steps",
+                launches=nl, templates_per_launch=round(ntl, 2),
+                ms_per_launch={x: round(kt[x + "_ms"] / nl, 4) for x in ("col", "row", "thr")},
+                bytes_per_template=dict(col=round(b_col), row=round(b_row)),
+                template_stage_GBps=round((b_col + b_row) * n / (stage_ms * 1e-3) / 1e9, 1))
 
 
 def cpu_model():
@@ -187,18 +201,25 @@ def oracle():
     return orc
 
 
-def cpu_baseline(seg, rng, opts, sky=None):
-    """The FP64 oracle on one sky point of the workload (all its spindowns), OpenMP threads."""
+def cpu_baseline(seg, rng, opts, sky=None, budget_s=10.0):
+    """The FP64 oracle on sky points of the workload (all spindowns of each), OpenMP threads,
+    until about budget_s seconds of wall time have been spent (at least one sky point)."""
     orc = oracle()
     threads = orc.default_threads()
     o = orc.Search(seg, nthreads=threads)
-    pm, mm, nn = sky or (seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2]))
-    t0 = time.perf_counter()
-    _, fn = o.job_core(pm, mm, nn, rng, opts)
-    dt = time.perf_counter() - t0
+    pts = sky or [(seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2]))]
+    fn, nsky, dt = 0, 0, 0.0
+    for pm, mm, nn in pts:
+        t0 = time.perf_counter()
+        _, f = o.job_core(pm, mm, nn, rng, opts)
+        dt += time.perf_counter() - t0
+        fn += f
+        nsky += 1
+        if dt >= budget_s:
+            break
     return dict(value=round(fn / dt, 2), unit="templates/s", cores=threads, kind="port",
-                sample=f"1 sky point x {fn} spindowns of the same workload (FP64 oracle, own Stockham FFT, "
-                       f"OpenMP over spindowns; {dt:.1f} s wall)", cpu_model=cpu_model())
+                sample=f"{nsky} sky points x all spindowns ({fn} templates) of the same workload (FP64 oracle, "
+                       f"own Stockham FFT, OpenMP over spindowns; {dt:.1f} s wall)", cpu_model=cpu_model())
 
 
 def tune_handle(args, h, segi, seg, opts):
@@ -233,7 +254,7 @@ def bench_templates(args, name):
           f"sky={len(sky)} spindowns={nspin}", file=sys.stderr, flush=True)
     h32 = pgw.Handle(seg, abi.PREC_FP32)
     tune = {"fp32": tune_handle(args, h32, 0, seg, opts)}
-    n32, t32, g32, r32 = run_path(h32, sky, rng, opts, args.steps, args.warmup, per_step)
+    n32, t32, g32, r32, kt32 = run_path(h32, sky, rng, opts, args.steps, args.warmup, per_step)
     v32 = n32 / t32
     print(f"fp32: {n32} templates in {t32:.3f} s -> {v32:.0f} templates/s (gpu {g32:.1f} ms)",
           file=sys.stderr, flush=True)
@@ -241,19 +262,19 @@ def bench_templates(args, name):
     if not args.no_fp64:
         h64 = pgw.Handle(seg, abi.PREC_FP64)
         tune["fp64"] = tune_handle(args, h64, 0, seg, opts)
-        n64, t64, g64, r64 = run_path(h64, sky, rng, opts, args.steps, args.warmup, per_step)
+        n64, t64, g64, r64, kt64 = run_path(h64, sky, rng, opts, args.steps, args.warmup, per_step)
         v64 = n64 / t64
         print(f"fp64: {n64} templates in {t64:.3f} s -> {v64:.0f} templates/s (gpu {g64:.1f} ms)",
               file=sys.stderr, flush=True)
         out_fp64 = dict(value=round(v64, 1), ms_per_step=round(1e3 * t64 / args.steps, 3),
                         gpu_ms_per_step=round(g64 / args.steps, 3), templates=n64,
-                        roofline=roofline(h64, seg, opts, 8, tune.get("fp64")))
+                        roofline=roofline(kt64, [(seg, 1)], 8, tune.get("fp64")))
         acc = cand_compare(r32, r64)
         mabs, mrel = fstat_compare(h32, h64, seg, opts)
         acc.update(max_abs_dF=mabs, max_rel_dF_F_ge_1=mrel)
         h64.close()
-    rl = roofline(h32, seg, opts, 4, tune.get("fp32"))
-    cpu = None if args.no_cpu else cpu_baseline(seg, rng, opts)
+    rl = roofline(kt32, [(seg, 1)], 4, tune.get("fp32"))
+    cpu = None if args.no_cpu else cpu_baseline(seg, rng, opts, sky)
     if cpu:
         print(f"cpu: {cpu}", file=sys.stderr, flush=True)
     cfg = {"workload": TEMPLATE_WORKLOADS[name][1], "N": s["N"], "nfft": s["nfft"], "fftpad": s["fftpad"],
@@ -293,11 +314,11 @@ def bench_batching(args):
     tune = {}
 
     # the autotuner runs on a segment of the longer FFT length (most of the templates)
-    kt = max(range(len(segs)), key=lambda k: (segs[k].sett["nfftf"], -k))
+    ktune = max(range(len(segs)), key=lambda k: (segs[k].sett["nfftf"], -k))
 
     def run(prec):
         h = pgw.Handle(segs, prec)
-        tune["fp32" if prec == abi.PREC_FP32 else "fp64"] = tune_handle(args, h, kt, segs[kt], opts)
+        tune["fp32" if prec == abi.PREC_FP32 else "fp64"] = tune_handle(args, h, ktune, segs[ktune], opts)
         chunks = [work[(i * per_step) % len(work):(i * per_step) % len(work) + per_step]
                   for i in range(args.warmup + args.steps)]
         # each segment's spindowns come from its own range; the call takes one range, so the
@@ -305,29 +326,34 @@ def bench_batching(args):
         for c in chunks[:args.warmup]:
             h.job_batch_multi(c, abi.make_range(segs[c[0][0]]), opts)
         recs, fn, gms = [], 0, 0.0
+        h.kernel_times(KTIME_EVERY)
         t0 = time.perf_counter()
         for c in chunks[args.warmup:]:
             r, _, f = h.job_batch_multi(c, abi.make_range(segs[c[0][0]]), opts)
             recs.append(r)
             fn += f
             gms += h.last_gpu_ms
-        return h, fn, time.perf_counter() - t0, gms, np.concatenate(recs)
+        dt = time.perf_counter() - t0
+        kt = h.kernel_times(0)
+        # algorithmic bytes: each timed work item's segment, weighted by its templates (equal)
+        segw = [(segs[k], 1) for c in chunks[args.warmup:] for k, _, _, _ in c]
+        return h, fn, dt, gms, np.concatenate(recs), kt, segw
 
-    h32, n32, t32, g32, r32 = run(abi.PREC_FP32)
+    h32, n32, t32, g32, r32, kt32, sw32 = run(abi.PREC_FP32)
     v32 = n32 / t32
     print(f"fp32: {n32} templates in {t32:.3f} s -> {v32:.0f} templates/s (gpu {g32:.1f} ms)",
           file=sys.stderr, flush=True)
     out_fp64, acc = None, None
     if not args.no_fp64:
-        h64, n64, t64, g64, r64 = run(abi.PREC_FP64)
+        h64, n64, t64, g64, r64, kt64, sw64 = run(abi.PREC_FP64)
         v64 = n64 / t64
         print(f"fp64: {n64} templates in {t64:.3f} s -> {v64:.0f} templates/s", file=sys.stderr, flush=True)
         out_fp64 = dict(value=round(v64, 1), ms_per_step=round(1e3 * t64 / args.steps, 3), templates=n64,
-                        roofline=roofline(h64, segs[0], opts, 8, tune.get("fp64")))
+                        roofline=roofline(kt64, sw64, 8, tune.get("fp64")))
         acc = cand_compare(r32, r64)
         h64.close()
-    rl = roofline(h32, segs[0], opts, 4, tune.get("fp32"))
-    cpu = None if args.no_cpu else cpu_baseline(segs[0], abi.make_range(segs[0]), opts)
+    rl = roofline(kt32, sw32, 4, tune.get("fp32"))
+    cpu = None if args.no_cpu else cpu_baseline(segs[0], abi.make_range(segs[0]), opts, synth.sky_list(segs[0]))
     cfg = {"workload": "batching: 3 detectors (H1, L1, V1), 64 segments of N in [60000, 86400] with 5-30 % gaps, "
                        "nfft 2^16 or 2^17, fftpad=1, 32 sky x 512 spindowns per segment, trl=20",
            "segments": len(segs), "segments_nfft_2^16": n16, "work_items": len(work), "items_per_step": per_step,

@@ -175,8 +175,9 @@ int pgw_job_batch_multi(pgw_handle *h, const int *work, int nwork, pgw_range *s_
  * Nelder-Mead and MADS, followup.c:537-1173).  FP64 on the device. */
 int pgw_fstatnet(pgw_handle *h, int seg, const double *pts, int npts, double *out, double *grad);
 
-/* Autotuner (no counterpart in the reference): times the template stage of sky point
- * (pm, mm, nn) of segment `seg` for every kernel variant (templates per launch 64/128/256;
+/* Autotuner (no counterpart in the reference): times a small search around sky point
+ * (pm, mm, nn) of segment `seg` (which must lie inside the sky disk; opts->trl is not used) for
+ * every kernel variant (templates per launch 64/128/256;
  * row pass with 16 elements per thread and both spectra together or in turn, or, in the FP64
  * form, with 8 elements per thread; column pass LDS sharing) and keeps the fastest for this handle.  The
  * chosen values are returned (pointers may be NULL): *row_na is 1 or 2 (16 elements per thread,
@@ -201,6 +202,15 @@ int pgw_debug_resampled(pgw_handle *h, int pm, int mm, int nn, int det, double *
  * number of templates per launch used. */
 int pgw_time_stages(pgw_handle *h, int pm, int mm, int nn, int s0, int nt, const pgw_opts *opts,
                     int reps, double ms[5]);
+
+/* Kernel timing over real searches (no counterpart in the reference): with enable = k >= 1,
+ * every k-th template launch of later pgw_job_core / pgw_job_batch / pgw_job_batch_multi /
+ * pgw_job_spotlight / pgw_search calls records HIP events (without system-scope fences) around
+ * its column pass, row pass and threshold kernels on the handle's stream; enable = 0 stops
+ * recording.  Each call returns in out[5] the totals since the previous call, then resets them:
+ * out[0..2] milliseconds of the column, row and threshold kernels summed over the recorded
+ * launches, out[3] those launches, out[4] their templates. */
+int pgw_kernel_times(pgw_handle *h, int enable, double out[5]);
 
 /* Calibration helper (no counterpart in the reference): a plain device copy kernel of `bytes`
  * bytes (read + write), run `reps` times; *ms = average milliseconds per launch.  Used to scale

@@ -269,6 +269,7 @@ struct pgw_handle {
   virtual int time_stages(int seg, int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps,
                           double* ms) = 0;
   virtual int autotune(int seg, int pm, int mm, int nn, const pgw_opts* o, int* tb, int* row_na, int* col_seq) = 0;
+  virtual int kernel_times(int enable, double* out) = 0;
   virtual int fstatnet(int seg, const double* pts, int n, double* out, double* grad) = 0;
   virtual int settings(int seg, pgw_settings* out) const = 0;
   double last_ms = 0.0;
@@ -344,8 +345,59 @@ template <typename T> struct Engine final : pgw_handle {
   // default (two workgroups per CU instead of one); FP32 keeps 16 elements per thread
   int row8 = std::is_same<T, double>::value ? 2 : 0;
 
+  // kernel timing over searches: 4 events per template launch (column, row, threshold)
+  int ktime = 0;         // record every ktime-th template launch (0: off)
+  long kseq = 0;         // template launches since recording was switched on
+  std::vector<hipEvent_t> kpool;
+  std::vector<int> knt;  // templates of each timed launch
+  size_t kused = 0;
+  double kacc[3] = {0, 0, 0};
+  double klaunch = 0, ktempl = 0;
+
+  hipEvent_t* kquad(int nt) {
+    if (4 * (kused + 1) > kpool.size()) {
+      for (int k = 0; k < 4; k++) {
+        hipEvent_t e = nullptr;
+        // timing only: no system-scope fence (its cache writeback and invalidation would slow
+        // the following kernels and enter the measurement)
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
+        kpool.push_back(e);
+      }
+      knt.push_back(0);
+    }
+    knt[kused] = nt;
+    return &kpool[4 * kused++];
+  }
+  // after the stream has drained: add the recorded launches to the totals
+  void kcollect() {
+    for (size_t q = 0; q < kused; q++) {
+      for (int k = 0; k < 3; k++) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, kpool[4 * q + k], kpool[4 * q + k + 1]) == hipSuccess) kacc[k] += t;
+      }
+      klaunch += 1;
+      ktempl += knt[q];
+    }
+    kused = 0;
+  }
+  int kernel_times(int enable, double* out) override {
+    HIPCHK(hipStreamSynchronize(stream));
+    kcollect();
+    if (out) {
+      for (int k = 0; k < 3; k++) out[k] = kacc[k];
+      out[3] = klaunch;
+      out[4] = ktempl;
+    }
+    kacc[0] = kacc[1] = kacc[2] = 0;
+    klaunch = ktempl = 0;
+    ktime = enable > 0 ? enable : 0;
+    kseq = 0;
+    return PGW_OK;
+  }
+
   ~Engine() override {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (hipEvent_t e : kpool) (void)hipEventDestroy(e);
     for (hipEvent_t e : {ev0, ev1, evd[0], evd[1]})
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -695,7 +747,10 @@ template <typename T> struct Engine final : pgw_handle {
       while (k + 1 < gr.items.size() && gr.items[k + 1].t0 <= t0) k++;
       const Item& it = gr.items[k];
       const bool one = t0 + nt <= it.t0 + it.ns();
-      if ((st = launch_templates(g0, gr.buf, t0, nt, one ? &it : nullptr, t0 - it.t0, Ng, o))) return st;
+      const bool timed = ktime && kseq++ % ktime == 0;
+      if ((st = launch_templates(g0, gr.buf, t0, nt, one ? &it : nullptr, t0 - it.t0, Ng, o,
+                                 timed ? kquad(nt) : nullptr)))
+        return st;
     }
     HIPCHK(hipMemcpyAsync(hn_pin.p + gr.buf, ncand.p + gr.buf, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(hc_pin.p + (size_t)gr.buf * kCandChunk, cand.p + (size_t)gr.buf * kCandCap,
@@ -810,6 +865,7 @@ template <typename T> struct Engine final : pgw_handle {
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, ev0, ev1));
     last_ms = t;
+    if (kused) kcollect();
     return st;
   }
 
@@ -922,7 +978,7 @@ template <typename T> struct Engine final : pgw_handle {
     Item it{0, sgi, pm, mm, nn, 0, s0, s0 + nt - 1, 0, SkyState{}};
     int st = PGW_OK;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (int k = 0; k < 4; k++) HIPCHK(hipEventCreate(&ev[k]));
+    for (int k = 0; k < 4; k++) HIPCHK(hipEventCreateWithFlags(&ev[k], hipEventDisableSystemFence));
     HIPCHK(hipEventRecord(ev0, stream));
     const int inside = sky_stage(g, pm, mm, nn, 0, it.sk, st);
     HIPCHK(hipEventRecord(ev1, stream));
@@ -955,30 +1011,56 @@ template <typename T> struct Engine final : pgw_handle {
 
   // Times the template stage per template for each (TB, row pass, column pass) variant on the
   // given sky point and keeps the fastest.
+  // Times every kernel variant on a small search of sky point (pm, mm, nn) of segment sgi (four
+  // copies of it with spindowns 0..511, so launches run back to back as in a search; threshold
+  // out of reach so no records) and keeps the fastest.
   int autotune(int sgi, int pm, int mm, int nn, const pgw_opts* o, int* tb_out, int* na_out, int* seq_out) override {
+    if (sgi < 0 || sgi >= (int)segs.size()) return PGW_ERR_ARG;
     const int tbs[3] = {64, 128, 256};
-    // row pass variants: 16 elements per thread with the spectra in turn (na 1) or together (na 2),
-    // 8 elements per thread without / with the second row prefetched (row8 2 / 1).  FP32 leaves
-    // out the 8-element kernels: as fast in a single launch, measurably slower over a search.
-    const int rv_na[4] = {1, 2, 1, 1}, rv_r8[4] = {0, 0, 2, 1};
-    const int nrv = std::is_same<T, double>::value ? 4 : 2;
+    // row pass variants: 16 elements per thread with the spectra in turn (na 1), 8 elements per
+    // thread without / with the second row prefetched (row8 2 / 1), 16 elements per thread with
+    // the spectra together (na 2).  FP32 keeps the first: the others are slower over a search.
+    const int rv_na[4] = {1, 1, 1, 2}, rv_r8[4] = {0, 2, 1, 0};
+    const int nrv = std::is_same<T, double>::value ? 4 : 1;
+    constexpr int NI = 4, NS = 512;
+    std::vector<int> work(4 * NI), ssoff(NI + 1), sslist((size_t)NI * NS);
+    for (int i = 0; i < NI; i++) {
+      work[4 * i] = sgi;
+      work[4 * i + 1] = pm;
+      work[4 * i + 2] = mm;
+      work[4 * i + 3] = nn;
+      ssoff[i] = i * NS;
+      for (int j = 0; j < NS; j++) sslist[(size_t)i * NS + j] = j;
+    }
+    ssoff[NI] = NI * NS;
+    pgw_opts oq = *o;
+    oq.trl = 1e300;
+    pgw_range rq;
+    memset(&rq, 0, sizeof(rq));
+    const int kt_saved = ktime;
+    ktime = 0;
+    auto per_template = [&](double& ms) {
+      int sgnlc = 0, fn = 0;
+      const int st = run(work.data(), NI, &rq, &oq, &sgnlc, nullptr, 0, nullptr, &fn, ssoff.data(), sslist.data());
+      if (st) return st;
+      if (fn == 0) return PGW_ERR_ARG;  // outside the sky disk
+      ms = last_ms / fn;
+      return PGW_OK;
+    };
     double best = 1e30;
-    int btb = TB, bna = row_na, br8 = row8;
+    int btb = TB, bna = row_na, br8 = row8, st = PGW_OK;
     bool bseq = col_seq;
     for (int tb : tbs) {
-      int st = ensure_tb(tb);
-      if (st) return st;
-      for (int rv = 0; rv < nrv; rv++)
-        for (int seq = 0; seq <= 1; seq++) {
+      if ((st = ensure_tb(tb))) break;
+      for (int rv = 0; rv < nrv && !st; rv++)
+        for (int seq = 0; seq <= 1 && !st; seq++) {
           TB = tb;
           row_na = rv_na[rv];
           row8 = rv_r8[rv];
           col_seq = seq != 0;
-          double ms[5];
-          // one untimed pass (first launch of a variant), then the mean of 5 launches
-          if ((st = time_stages(sgi, pm, mm, nn, 0, tb, o, 1, ms))) return st;
-          if ((st = time_stages(sgi, pm, mm, nn, 0, tb, o, 5, ms))) return st;
-          const double per = (ms[1] + ms[2] + ms[3]) / ms[4];
+          double w, t1, t2;  // one untimed search, then the faster of two
+          if ((st = per_template(w)) || (st = per_template(t1)) || (st = per_template(t2))) break;
+          const double per = std::min(t1, t2);
           if (per < best) {
             best = per;
             btb = tb;
@@ -987,11 +1069,14 @@ template <typename T> struct Engine final : pgw_handle {
             bseq = seq != 0;
           }
         }
+      if (st) break;
     }
+    ktime = kt_saved;
     TB = btb;
     row_na = bna;
     row8 = br8;
     col_seq = bseq;
+    if (st) return st;
     if (tb_out) *tb_out = TB;
     if (na_out) *na_out = row8 ? 8 : row_na;
     if (seq_out) *seq_out = col_seq;
@@ -1192,6 +1277,11 @@ int pgw_copy_bench(size_t bytes, int reps, double* ms) {
 }
 
 double pgw_last_gpu_ms(const pgw_handle* h) { return h ? h->last_ms : 0.0; }
+
+int pgw_kernel_times(pgw_handle* h, int enable, double out[5]) {
+  if (!h) return PGW_ERR_ARG;
+  return h->kernel_times(enable, out);
+}
 
 int pgw_get_settings(const pgw_handle* h, int seg, pgw_settings* sett) {
   if (!h || !sett) return PGW_ERR_ARG;

@@ -60,6 +60,7 @@ def lib():
         L.pgw_copy_bench.argtypes = [C.c_size_t, C.c_int, _dp]
         L.pgw_last_gpu_ms.restype = C.c_double
         L.pgw_last_gpu_ms.argtypes = [C.c_void_p]
+        L.pgw_kernel_times.argtypes = [C.c_void_p, C.c_int, _dp]
         L.pgw_destroy.argtypes = [C.c_void_p]
         L.pgw_strerror.restype = C.c_char_p
         L.pgw_strerror.argtypes = [C.c_int]
@@ -71,7 +72,7 @@ EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_j
             "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_fstatnet", "pgw_get_settings",
             "pgw_search", "pgw_read_checkpoint", "pgw_job_spotlight",
             "pgw_debug_fstat", "pgw_debug_resampled", "pgw_time_stages", "pgw_copy_bench", "pgw_last_gpu_ms",
-            "pgw_destroy", "pgw_strerror"]
+            "pgw_kernel_times", "pgw_destroy", "pgw_strerror"]
 
 
 def search_settings(fpo: float, dt: float, N: int, fftpad: int) -> abi.Settings:
@@ -248,6 +249,15 @@ class Handle:
         if st != 0:
             raise PGWError(st, "pgw_time_stages")
         return dict(sky=ms[0], col=ms[1], row=ms[2], thr=ms[3], nt=int(ms[4]))
+
+    def kernel_times(self, every: int) -> dict:
+        """Totals of the kernel timing since the previous call (then reset); later searches record
+        every `every`-th template launch (0: off) (pgw_kernel_times)."""
+        out = np.zeros(5)
+        st = lib().pgw_kernel_times(self.h, int(every), out.ctypes.data_as(_dp))
+        if st != 0:
+            raise PGWError(st, "pgw_kernel_times")
+        return dict(col_ms=out[0], row_ms=out[1], thr_ms=out[2], launches=int(out[3]), templates=int(out[4]))
 
     @property
     def last_gpu_ms(self) -> float:

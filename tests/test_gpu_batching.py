@@ -1,5 +1,7 @@
 """Batching form: segments of uneven length in one handle, templates of many sky points per
 launch, and the autotuner; every record checked against the CPU oracle run per segment."""
+import os
+
 import numpy as np
 import pytest
 
@@ -74,3 +76,28 @@ def test_autotune_keeps_results(pgw, abi, prec):
     assert np.array_equal(before[:, :4], after[:, :4])
     Fa, Fb = before[:, 4] ** 2 / 2 + 2, after[:, 4] ** 2 / 2 + 2
     assert np.max(np.abs(Fa - Fb) / Fa) <= 1e-6
+
+
+def test_kernel_times_cover_every_launch(pgw, abi):
+    """pgw_kernel_times: events around every template launch (or every k-th) of a search; totals
+    reset per call."""
+    from cases import segment
+    seg = segment("net-2det-32k")
+    os.environ["PGW_TUNE"] = "tb=16"
+    try:
+        h = pgw.Handle(seg, abi.PREC_FP32)
+    finally:
+        del os.environ["PGW_TUNE"]
+    sky = synth.sky_list(seg)
+    rng, opts = abi.make_range(seg), abi.make_opts(trl=12.0)
+    assert h.kernel_times(1)["launches"] == 0
+    _, fn = h.job_batch(sky, rng, opts)
+    kt = h.kernel_times(3)
+    print(kt, flush=True)
+    assert kt["templates"] == fn and kt["launches"] >= 3
+    assert kt["col_ms"] > 0 and kt["row_ms"] > 0 and kt["thr_ms"] > 0
+    h.job_batch(sky, rng, opts)  # every third launch
+    k3 = h.kernel_times(0)
+    assert k3["launches"] == (kt["launches"] + 2) // 3
+    h.job_batch(sky, rng, opts)  # recording off
+    assert h.kernel_times(0)["launches"] == 0
