@@ -24,6 +24,7 @@ Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import functools
 import json
 import os
 import sys
@@ -108,14 +109,15 @@ def fstat_compare(h32, h64, seg, opts, ntempl=4):
     return mabs, mrel
 
 
-def measured_traffic(kernel, prec):
-    """HBM bytes per template of a kernel from the committed rocprofv3 counter summary
-    (profiles/*.md lines 'bench-traffic: {json}'), or None."""
+@functools.lru_cache(maxsize=None)
+def measured_traffic(kernel, prec, workload, L):
+    """HBM bytes per template of a kernel at F-statistic FFT length L on a workload, from the
+    committed rocprofv3 counter summaries (profiles/*.md lines 'bench-traffic: {json}'), or None."""
     for f in sorted((ROOT / "profiles").glob("*.md")):
         for line in f.read_text().splitlines():
             if line.startswith("bench-traffic:"):
                 d = json.loads(line.split(":", 1)[1])
-                if d.get("kernel") == kernel and d.get("precision") == prec:
+                if (d.get("kernel"), d.get("precision"), d.get("workload"), d.get("L")) == (kernel, prec, workload, L):
                     return d["bytes_per_template"]
     return None
 
@@ -155,7 +157,7 @@ def alg_bytes(seg, p):
     return b_col, b_row
 
 
-def roofline(kt, segw, prec_bytes, tuned=None):
+def roofline(kt, segw, prec_bytes, workload, tuned=None):
     """The dominant template kernel against the HBM roofline.  kt: the library's HIP-event kernel
     timing over the timed steps (pgw_kernel_times); segw: [(segment, templates)] of those steps
     for the algorithmic bytes."""
@@ -170,7 +172,8 @@ def roofline(kt, segw, prec_bytes, tuned=None):
     achieved = (b_col if k == "col" else b_row) * n / (kt[k + "_ms"] * 1e-3) / 1e9
     stage_ms = kt["col_ms"] + kt["row_ms"] + kt["thr_ms"]
     prec = "fp32" if prec_bytes == 4 else "fp64"
-    tr = measured_traffic(kname, prec)
+    trs = [(measured_traffic(kname, prec, workload, sg.sett["nfftf"]), w) for sg, w in segw]
+    tr = None if any(t is None for t, _ in trs) else sum(t * w for t, w in trs) / tot
     ntl = n / nl
     # the kernels also issue close to the VALU limit: counter-measured fraction of VALU issue
     # cycles (SQ_INSTS_VALU x 4 / SIMD cycles) from the committed profile
@@ -202,23 +205,22 @@ def oracle():
 
 
 def cpu_baseline(seg, rng, opts, sky=None, budget_s=10.0):
-    """The FP64 oracle on sky points of the workload (all spindowns of each), OpenMP threads,
-    until about budget_s seconds of wall time have been spent (at least one sky point)."""
+    """The FP64 oracle on sky points of the workload (all spindowns of each, the list repeated if
+    it is short), OpenMP threads, until about budget_s seconds of wall time have been spent."""
     orc = oracle()
     threads = orc.default_threads()
     o = orc.Search(seg, nthreads=threads)
     pts = sky or [(seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2]))]
     fn, nsky, dt = 0, 0, 0.0
-    for pm, mm, nn in pts:
+    while dt < budget_s and nsky < 100 * len(pts):  # a small workload is repeated
+        pm, mm, nn = pts[nsky % len(pts)]
         t0 = time.perf_counter()
         _, f = o.job_core(pm, mm, nn, rng, opts)
         dt += time.perf_counter() - t0
         fn += f
         nsky += 1
-        if dt >= budget_s:
-            break
     return dict(value=round(fn / dt, 2), unit="templates/s", cores=threads, kind="port",
-                sample=f"{nsky} sky points x all spindowns ({fn} templates) of the same workload (FP64 oracle, "
+                sample=f"{nsky} sky-point searches x all spindowns ({fn} templates) of the same workload (FP64 oracle, "
                        f"own Stockham FFT, OpenMP over spindowns; {dt:.1f} s wall)", cpu_model=cpu_model())
 
 
@@ -268,12 +270,12 @@ def bench_templates(args, name):
               file=sys.stderr, flush=True)
         out_fp64 = dict(value=round(v64, 1), ms_per_step=round(1e3 * t64 / args.steps, 3),
                         gpu_ms_per_step=round(g64 / args.steps, 3), templates=n64,
-                        roofline=roofline(kt64, [(seg, 1)], 8, tune.get("fp64")))
+                        roofline=roofline(kt64, [(seg, 1)], 8, name, tune.get("fp64")))
         acc = cand_compare(r32, r64)
         mabs, mrel = fstat_compare(h32, h64, seg, opts)
         acc.update(max_abs_dF=mabs, max_rel_dF_F_ge_1=mrel)
         h64.close()
-    rl = roofline(kt32, [(seg, 1)], 4, tune.get("fp32"))
+    rl = roofline(kt32, [(seg, 1)], 4, name, tune.get("fp32"))
     cpu = None if args.no_cpu else cpu_baseline(seg, rng, opts, sky)
     if cpu:
         print(f"cpu: {cpu}", file=sys.stderr, flush=True)
@@ -349,10 +351,10 @@ def bench_batching(args):
         v64 = n64 / t64
         print(f"fp64: {n64} templates in {t64:.3f} s -> {v64:.0f} templates/s", file=sys.stderr, flush=True)
         out_fp64 = dict(value=round(v64, 1), ms_per_step=round(1e3 * t64 / args.steps, 3), templates=n64,
-                        roofline=roofline(kt64, sw64, 8, tune.get("fp64")))
+                        roofline=roofline(kt64, sw64, 8, "batching", tune.get("fp64")))
         acc = cand_compare(r32, r64)
         h64.close()
-    rl = roofline(kt32, sw32, 4, tune.get("fp32"))
+    rl = roofline(kt32, sw32, 4, "batching", tune.get("fp32"))
     cpu = None if args.no_cpu else cpu_baseline(segs[0], abi.make_range(segs[0]), opts, synth.sky_list(segs[0]))
     cfg = {"workload": "batching: 3 detectors (H1, L1, V1), 64 segments of N in [60000, 86400] with 5-30 % gaps, "
                        "nfft 2^16 or 2^17, fftpad=1, 32 sky x 512 spindowns per segment, trl=20",
@@ -403,14 +405,18 @@ def bench_differentiation(args, npts=1024):
         orc = oracle()
         threads = orc.default_threads()
         o = orc.Search(seg, nthreads=threads)
-        ns = 64
-        tc = time.perf_counter()
-        out_o, g_o = o.fstatnet(pts[:ns])
-        tcd = time.perf_counter() - tc
+        # all points with gradient, repeated until ~10 s of wall time
+        ns, tcd, reps = 0, 0.0, 0
+        while tcd < 10.0 and reps < 200:
+            tc = time.perf_counter()
+            out_o, g_o = o.fstatnet(pts)
+            tcd += time.perf_counter() - tc
+            ns += len(pts)
+            reps += 1
         cpu = dict(value=round(ns / tcd, 2), unit="points/s", cores=threads, kind="port",
-                   sample=f"{ns} of the points with gradient (FP64 oracle Fstatnet, OpenMP over points; "
+                   sample=f"{reps} x the {len(pts)} points with gradient (FP64 oracle Fstatnet, OpenMP over points; "
                           f"{tcd:.1f} s wall)", cpu_model=cpu_model(),
-                   max_rel_dF_vs_device=float(np.max(np.abs(out_o[:, 5] - out[:ns, 5]) / np.abs(out_o[:, 5]))))
+                   max_rel_dF_vs_device=float(np.max(np.abs(out_o[:, 5] - out[:, 5]) / np.abs(out_o[:, 5]))))
     nifo, N = len(seg.dets), seg.sett["N"]
     fps = measured_flops("k_fstatnet_part")
     rl = None
