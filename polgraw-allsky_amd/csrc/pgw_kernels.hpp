@@ -389,13 +389,15 @@ inline void spin_coeffs(const SpinGrid& q, int ss, double& het1, double& sgnlt1)
 }
 
 // uslot >= 0: the launch's templates all belong to the sky point in slot uslot with data length
-// uN, so the kernels take addresses from these arguments instead of waiting for the descriptor.
+// uN, so the kernels take addresses from these arguments instead of waiting for the descriptor;
+// tab = 1: their het1, sgnlt1 are in the kernel-argument table (launches of at most kSpinTab
+// templates), else in the descriptors (a load that no data load waits for).
 struct SlotBases {
   const void* xm;
   const double* shft;
   const double* aabb;
   unsigned long long xm_stride, shft_stride;  // elements per slot
-  int nifo, uslot, uN, pad;
+  int nifo, uslot, uN, tab;
 };
 
 // het1 and sgnlt[1] of the templates of a launch of at most kSpinTab templates, passed by value as

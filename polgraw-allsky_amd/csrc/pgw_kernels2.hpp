@@ -43,8 +43,13 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
   if constexpr (UNI) {
     slot = sb.uslot;
     N = sb.uN;
-    het1 = tab.het1[t];
-    sgnlt1 = tab.sgnlt1[t];
+    if (sb.tab) {
+      het1 = tab.het1[t];
+      sgnlt1 = tab.sgnlt1[t];
+    } else {
+      het1 = ld_global(&desc[t].het1, 0);
+      sgnlt1 = ld_global(&desc[t].sgnlt1, 0);
+    }
   } else {
     const TmplDesc& dsc = desc[t];
     slot = dsc.slot;

@@ -703,11 +703,14 @@ template <typename T> struct Engine final : pgw_handle {
     if (ev) HIPCHK(hipEventRecord(ev[0], stream));
     SlotBases sb{xm_slots.p, shft_slots.p, aabb_slots.p, slot_xm, slot_shft, nifo, -1, 0, 0};
     SpinTab tab;
-    if (uni && nt <= kSpinTab) {
+    if (uni) {
       sb.uslot = uni->slot;
       sb.uN = segs[uni->seg]->sett.N;
-      const SpinGrid q = spin_grid(segs[uni->seg]->sett.M, uni->mm, uni->nn, 0, o->s0_flag);
-      for (int i = 0; i < nt; i++) spin_coeffs(q, uni->ss_at(j0 + i), tab.het1[i], tab.sgnlt1[i]);
+      if (nt <= kSpinTab) {  // else the kernels read het1, sgnlt1 from the descriptors
+        sb.tab = 1;
+        const SpinGrid q = spin_grid(segs[uni->seg]->sett.M, uni->mm, uni->nn, 0, o->s0_flag);
+        for (int i = 0; i < nt; i++) spin_coeffs(q, uni->ss_at(j0 + i), tab.het1[i], tab.sgnlt1[i]);
+      }
     }
     launch_spin_col<T>(*g.ps, sb, tab, d, Z.p, nt, uni ? segs[uni->seg]->sett.N : Nl, hi.p, lo.p, col_seq, stream);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));

@@ -81,3 +81,29 @@ def test_variant_fp32(pgw, abi, oracle_runs, name, tune):
     hit = [r for r in strong if tuple(r[:4]) in kg]
     assert len(hit) == len(strong)
     assert max(abs(kg[tuple(r[:4])] - F(r[4])) / F(r[4]) for r in hit) <= 1e-4
+
+
+@pytest.mark.parametrize("tune", ["tb=256", "tb=200", "tb=128"])
+def test_long_spindown_range(pgw, abi, orc, tune):
+    """300 spindowns of one sky point: launches of more templates than the kernel-argument table
+    holds take het1, sgnlt1 from the descriptors; records equal the oracle's."""
+    seg = segment("net-2det-32k")
+    pm, mm, nn = injected_sky(seg)
+    s0 = seg.range["spndr"][0] - 150
+    rng, opts = abi.make_range(seg, spndr=[s0, s0 + 299]), abi.make_opts(trl=12.0)
+    ref, fr = orc.Search(seg, nthreads=8).job_core(pm, mm, nn, rng, opts)
+    old = os.environ.get("PGW_TUNE")
+    os.environ["PGW_TUNE"] = tune
+    try:
+        h = pgw.Handle(seg, abi.PREC_FP64)
+    finally:
+        if old is None:
+            del os.environ["PGW_TUNE"]
+        else:
+            os.environ["PGW_TUNE"] = old
+    got, fg = h.job_core(pm, mm, nn, rng, opts)
+    print(tune, "records", len(got), flush=True)
+    assert fg == fr == 300 and got.shape == ref.shape and len(ref) > 0
+    assert np.array_equal(got[:, :4], ref[:, :4])
+    Fg, Fr = got[:, 4] ** 2 / 2 + 2, ref[:, 4] ** 2 / 2 + 2
+    assert np.max(np.abs(Fg - Fr) / Fr) <= 1e-9
