@@ -485,42 +485,51 @@ __global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restri
   const int hik = (part < p.nblk) ? lok + PGW_NAV_DEV : p.nmax;
   const int mL2 = p.L2 - 1;
   auto at = [&](int k) { return (double)F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)]; };
-  // (16 loads per batch keep the kernel's register count, and so the occupancy of the far more
-  // common unflagged waves, low)
+  // (TJ loads per batch keep the kernel's register count, and so the occupancy of the far more
+  // common unflagged waves, moderate)
+  constexpr int TJ = sizeof(T) == 4 ? 32 : 16;
   const double pre = p.trl * m * (1. - 1e-14);
   uint64_t hits = 0;
 #pragma unroll 1
-  for (int j0 = 0; j0 < NJ; j0 += 16) {
-    T f[16];
+  for (int j0 = 0; j0 < NJ; j0 += TJ) {
+    T f[TJ];
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
+    for (int j = 0; j < TJ; j++) {
       const int k = min(lok + lane + 64 * (j0 + j), hik - 1);
       f[j] = F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)];
     }
 #pragma unroll
-    for (int j = 0; j < 16; j++)
+    for (int j = 0; j < TJ; j++)
       hits |= (uint64_t)(lok + lane + 64 * (j0 + j) < hik && (double)f[j] > pre) << (j0 + j);
   }
-  // means of all parts with the unflagged waves' arithmetic, 8 parts' loads in flight at a time
+  // means of all parts with the unflagged waves' arithmetic, QB parts' loads in flight at a time
+  constexpr int QB = 16;
 #pragma unroll 1
-  for (int q0 = 0; q0 < p.nparts; q0 += 8) {
-    double v[8];
+  for (int q0 = 0; q0 < p.nparts; q0 += QB) {
+    double v[QB];
 #pragma unroll
-    for (int j = 0; j < 8; j++) v[j] = 0.0;
+    for (int j = 0; j < QB; j++) v[j] = 0.0;
     for (int k2 = lane; k2 < p.L2; k2 += 64) {
-      T x[8];
+      T x[QB];
 #pragma unroll
-      for (int j = 0; j < 8; j++) x[j] = ps0[(size_t)min(q0 + j, p.nparts - 1) * p.L2 + k2];
+      for (int j = 0; j < QB; j++) x[j] = ps0[(size_t)min(q0 + j, p.nparts - 1) * p.L2 + k2];
 #pragma unroll
-      for (int j = 0; j < 8; j++) v[j] += (double)x[j];
+      for (int j = 0; j < QB; j++) v[j] += (double)x[j];
     }
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
+    for (int j = 0; j < QB; j++) {
       const double sq = wave_sum(v[j]);
       if (lane == 0 && q0 + j < p.nparts) mu[q0 + j] = part_mean(sq, q0 + j, p);
     }
   }
   __syncthreads();
+  // neighbours' pre-selection bits: bin k - 1 and k + 1 of this part are lane - 1 / lane + 1 (same
+  // j), or lane 63 (j - 1) / lane 0 (j + 1) across the wave's edge.  A neighbour without its bit is
+  // certainly not above trl, so its exact test (a dependent load) is skipped.
+  const uint64_t h_lo = __shfl(hits, lane == 0 ? 63 : lane - 1, 64);
+  const uint64_t h_hi = __shfl(hits, lane == 63 ? 0 : lane + 1, 64);
+  const uint64_t prevm = lane == 0 ? h_lo << 1 : h_lo;
+  const uint64_t nextm = lane == 63 ? h_hi >> 1 : h_hi;
   while (hits) {
     const int j = __builtin_ctzll(hits);
     hits &= hits - 1;
@@ -528,15 +537,18 @@ __global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restri
     const double fn = at(k) / mu[part];
     if (!(fn > p.trl)) continue;
     bool start = (k == p.nmin);
-    if (!start) {
+    if (!start && (k == lok || (prevm >> j) & 1)) {  // previous bin in another part, or pre-selected
       const int kp = k - 1;
       const int pp = min((kp - p.nmin) / PGW_NAV_DEV, p.nparts - 1);
       start = !(at(kp) / mu[pp] > p.trl);
+    } else if (!start) {
+      start = true;
     }
     if (!start) continue;
     int ii = k;
     double Fc = fn;
-    for (int kk = k + 1; kk < p.nmax; kk++) {
+    const bool ends = k + 1 < hik && !((nextm >> j) & 1);  // next bin of this part not pre-selected
+    for (int kk = k + 1; !ends && kk < p.nmax; kk++) {
       const int pk = min((kk - p.nmin) / PGW_NAV_DEV, p.nparts - 1);
       const double v = at(kk) / mu[pk];
       if (!(v > p.trl)) break;

@@ -107,3 +107,19 @@ def test_long_spindown_range(pgw, abi, orc, tune):
     assert np.array_equal(got[:, :4], ref[:, :4])
     Fg, Fr = got[:, 4] ** 2 / 2 + 2, ref[:, 4] ** 2 / 2 + 2
     assert np.max(np.abs(Fg - Fr) / Fr) <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["plain-1det-32k", "net-2det-32k"])
+def test_low_threshold_runs(pgw, abi, orc, name):
+    """trl = 5 on noise: several per cent of the bins exceed it, so runs of many bins cross lanes,
+    register rows and FStat blocks; every run-maximum record equals the oracle's."""
+    seg = segment(name)
+    pm, mm, nn = injected_sky(seg)
+    rng, opts = abi.make_range(seg), abi.make_opts(trl=5.0)
+    ref, fr = orc.Search(seg, nthreads=8).job_core(pm, mm, nn + 1, rng, opts)
+    got, fg = pgw.Handle(seg, abi.PREC_FP64).job_core(pm, mm, nn + 1, rng, opts, cap=1 << 21)
+    print(name, "records", len(got), flush=True)
+    assert fg == fr and got.shape == ref.shape and len(ref) > 1000
+    assert np.array_equal(got[:, :4], ref[:, :4])
+    Fg, Fr = got[:, 4] ** 2 / 2 + 2, ref[:, 4] ** 2 / 2 + 2
+    assert np.max(np.abs(Fg - Fr) / Fr) <= 1e-9
