@@ -148,8 +148,11 @@ template <int P1> struct RowShape {
 // x[a][m] = row element (t + NT m) on entry, X[t + NT m] on exit.  lds holds NA * PADN elements.
 // twr[m] = W_P1^m (forward sign).  Barriers inside; ends after a barrier-free read (caller must
 // barrier before reusing lds).
+// pc (optional): the stage-C twiddle powers of this thread, computed on the first call (pc_have =
+// false) and reused on later calls of the same thread (the second spectrum of a row).
 template <typename T2, int P1, int NA, int DIR>
-__device__ __forceinline__ void row_fft(T2 (&x)[NA][16], T2* lds, const T2* __restrict__ twr) {
+__device__ __forceinline__ void row_fft(T2 (&x)[NA][16], T2* lds, const T2* __restrict__ twr, T2* pc = nullptr,
+                                        bool pc_have = false) {
   using S = RowShape<P1>;
   constexpr int NT = S::NT, R3 = S::R3, PADN = S::PADN;
   const int t = threadIdx.x;
@@ -193,7 +196,12 @@ __device__ __forceinline__ void row_fft(T2 (&x)[NA][16], T2* lds, const T2* __re
       T2 v[R3];
 #pragma unroll
       for (int r = 0; r < R3; r++) v[r] = lds[a * PADN + rpad(b + 256 * r)];
-      twiddle_run<R3, DIR>(v, wC[i]);  // W_P1^{b r}
+      if (pc) {  // W_P1^{b r} from the cached powers
+        if (!pc_have && a == 0) tw_powers<R3>(tw_dir<DIR>(wC[i]), pc + R3 * i);
+        apply_tw_pow<R3, DIR>(v, pc + R3 * i);
+      } else {
+        twiddle_run<R3, DIR>(v, wC[i]);  // W_P1^{b r}
+      }
       dftR<R3, DIR>(v);
       // outputs b + 256 r = t + NT (i + (16/R3) r)
 #pragma unroll

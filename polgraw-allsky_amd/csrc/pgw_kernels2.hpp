@@ -168,13 +168,17 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
 #pragma unroll
       for (int m = 0; m < 16; m++) y[0][m] = rowb[tid + NT * m];  // in flight during the first FFT
     }
-    row_fft<T2, P1, 1, -1>(x, lds, twr);
+    // FP32: the stage-C twiddle powers of the first transform serve the second (no more
+    // registers at the peak: they live while the prefetched row no longer does)
+    constexpr bool PC = sizeof(T) == 4;
+    T2 pc[PC ? 16 : 1];
+    row_fft<T2, P1, 1, -1>(x, lds, twr, PC ? pc : nullptr, false);
 #pragma unroll
     for (int m = 0; m < MO; m++) F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * raa;
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 16; m++) x[0][m] = PREF ? y[0][PREF ? m : 0] : rowb[tid + NT * m];
-    row_fft<T2, P1, 1, -1>(x, lds, twr);
+    row_fft<T2, P1, 1, -1>(x, lds, twr, PC ? pc : nullptr, true);
 #pragma unroll
     for (int m = 0; m < MO; m++) F[m] = F[m] + (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * rbb;
   }
