@@ -141,7 +141,9 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
   using S = RowShape<P1>;
   constexpr int NT = S::NT;
   __shared__ T2 lds[NA * S::PADN];
-  const int k2 = blockIdx.x, t = blockIdx.y, tid = threadIdx.x;
+  // templates in reverse launch order: the column pass wrote the last templates' spectra last, so
+  // the first row workgroups find them still in the last-level cache (MALL)
+  const int k2 = blockIdx.x, t = gridDim.y - 1 - blockIdx.y, tid = threadIdx.x;
   const size_t L = (size_t)P1 * p.L2;
   const T2* rowa = Z + (size_t)t * 2 * L + (size_t)k2 * P1;
   const T2* rowb = rowa + L;
@@ -258,7 +260,9 @@ __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const doub
   using S = Row8Shape<P1>;
   constexpr int NT = S::NT;
   __shared__ T2 lds[S::PADN];
-  const int k2 = blockIdx.x, t = blockIdx.y, tid = threadIdx.x;
+  // templates in reverse launch order: the column pass wrote the last templates' spectra last, so
+  // the first row workgroups find them still in the last-level cache (MALL)
+  const int k2 = blockIdx.x, t = gridDim.y - 1 - blockIdx.y, tid = threadIdx.x;
   const size_t L = (size_t)P1 * p.L2;
   const T2* rowa = Z + (size_t)t * 2 * L + (size_t)k2 * P1;
   const T2* rowb = rowa + L;
