@@ -400,6 +400,11 @@ def bench_differentiation(args, npts=1024):
     t1 = time.perf_counter()
     x, F, nev = followup.ascend(followup.FStatNet(h), pts, G, steps=20)
     t_asc = time.perf_counter() - t1
+    # the followup's Nelder-Mead from the same points, all simplices batched per device call
+    t3 = time.perf_counter()
+    xs, outs, nev_s, conv_s = followup.amoeba(followup.FStatNet(h), pts, 0.5 * sc, tol=1e-9)
+    t_nm = time.perf_counter() - t3
+    print(f"simplex: {npts} points, {int(nev_s.sum())} evaluations in {t_nm:.3f} s", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         orc = oracle()
@@ -434,6 +439,10 @@ def bench_differentiation(args, npts=1024):
                  "values_only_points_per_s": round(v_nograd, 1),
                  "ascent": {"steps": 20, "evaluations_per_point": nev, "seconds": round(t_asc, 3),
                             "F_start_median": float(np.median(-out[:, 5])), "F_end_median": float(np.median(F))},
+                 "simplex": {"tol": 1e-9, "evaluations_per_point_mean": round(float(nev_s.mean()), 1),
+                             "converged": int(conv_s.sum()), "seconds": round(t_nm, 3),
+                             "evaluations_per_s": round(float(nev_s.sum()) / t_nm, 1),
+                             "F_end_median": float(np.median(-outs[:, 5]))},
                  "roofline": rl, "cpu_baseline": cpu})
     return line
 

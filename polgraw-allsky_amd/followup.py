@@ -127,3 +127,116 @@ def ascend(fs: FStatNet, start, metric, steps: int = 20, trust0: float = 1.0, tr
         F[ok], g[ok] = Ft[ok], g_t[ok]
         trust = np.where(ok, np.minimum(2 * trust, trust_max), trust / 4)
     return x, F, nev
+
+
+def amoeba(fs: FStatNet, starts, steps, tol: float = 1e-7, maxeval: int = 20000):
+    """The followup's Nelder-Mead on -F (followup/src/followup.c:1126-1173) for many starting
+    points at once: simplex k starts at starts[k] with vertex i (< 4) moved by steps[k][i] along
+    parameter i; every round's trial vertices of all unfinished simplices go to the device in one
+    batched Fstatnet call.  Per simplex the steps are the reference's: reflection; expansion when
+    the reflected vertex is the new best; contraction when it is still the worst, and a shrink
+    toward the best vertex when that fails; stop when |fmax - fmin| < (|fmax| + |fmin|) tol + 1e-14
+    (or after maxeval evaluations, which leaves that simplex's best vertex unconverged).
+    Returns (best points (n, 4), their 11 outputs (n, 11), evaluations (n,), converged (n,))."""
+    D = 4
+    x0 = np.asarray(starts, dtype=np.float64).reshape(-1, D)
+    st = np.broadcast_to(np.asarray(steps, dtype=np.float64), x0.shape)
+    n = len(x0)
+    sx = np.repeat(x0[:, None, :], D + 1, axis=1)          # (n, 5, 4)
+    for i in range(D):
+        sx[:, i, i] += st[:, i]
+    out = fs(sx.reshape(-1, D), grad=False)[0].reshape(n, D + 1, 11)
+    nev = np.full(n, D + 1)
+    done = np.zeros(n, bool)
+    conv = np.zeros(n, bool)
+    ar = np.arange(n)
+
+    def evaluate(idx, pts):
+        return fs(pts, grad=False)[0] if len(idx) else np.zeros((0, 11))
+
+    while True:
+        act = np.nonzero(~done)[0]
+        if not len(act):
+            break
+        f = out[act, :, 5]
+        # ordering with the reference's tie rules
+        first = f[:, 0] > f[:, 1]
+        ihi = np.where(first, 0, 1)
+        ilo = np.where(first, 1, 0)
+        inhi = ilo.copy()
+        for i in range(2, D + 1):
+            fi = f[:, i]
+            lo_ = fi <= f[np.arange(len(act)), ilo]
+            hi_ = ~lo_ & (fi > f[np.arange(len(act)), ihi])
+            nh_ = ~lo_ & ~hi_ & (fi > f[np.arange(len(act)), inhi])
+            inhi = np.where(hi_, ihi, np.where(nh_, i, inhi))
+            ihi = np.where(hi_, i, ihi)
+            ilo = np.where(lo_, i, ilo)
+        a = np.arange(len(act))
+        mid = np.zeros((len(act), D))
+        for i in range(D + 1):
+            mid += np.where((ihi != i)[:, None], sx[act, i, :], 0.0)
+        mid /= D
+        line = sx[act, ihi, :] - mid
+        fmax, fmin = f[a, ihi], f[a, ilo]
+        fin = np.abs(fmax - fmin) < (np.abs(fmax) + np.abs(fmin)) * tol + 1e-14
+        conv[act[fin]] = True
+        stop = fin | (nev[act] >= maxeval)
+        done[act[stop]] = True
+        keep = ~stop
+        act, ihi, ilo, inhi, mid, line = act[keep], ihi[keep], ilo[keep], inhi[keep], mid[keep], line[keep]
+        if not len(act):
+            continue
+
+        def trial(sel, scale):
+            """Try mid + scale line for simplices act[sel]; accept where it beats their worst."""
+            idx = act[sel]
+            pts = mid[sel] + scale * line[sel]
+            o = evaluate(idx, pts)
+            nev[idx] += 1
+            acc = o[:, 5] < out[idx, ihi[sel], 5]
+            out[idx[acc], ihi[sel][acc]] = o[acc]
+            sx[idx[acc], ihi[sel][acc]] = pts[acc]
+            return acc
+
+        allsel = np.arange(len(act))
+        trial(allsel, -1.0)                                     # reflection
+        fw = out[act, ihi, 5]
+        best = fw < out[act, ilo, 5]
+        if best.any():
+            trial(np.nonzero(best)[0], -2.0)                     # expansion
+        worst = ~best & (fw > out[act, inhi, 5])
+        if worst.any():
+            sel = np.nonzero(worst)[0]
+            acc = trial(sel, 0.5)                                # contraction
+            shr = sel[~acc]
+            if len(shr):                                         # shrink toward the best vertex
+                idx = act[shr]
+                pts, where = [], []
+                for k, s in zip(idx, shr):
+                    for i in range(D + 1):
+                        if i == ilo[s]:
+                            continue
+                        sx[k, i] = (sx[k, ilo[s]] + sx[k, i]) * 0.5
+                        pts.append(sx[k, i])
+                        where.append((k, i))
+                o = evaluate(idx, np.array(pts))
+                for (k, i), oo in zip(where, o):
+                    out[k, i] = oo
+                nev[idx] += D
+    f = out[:, :, 5]
+    # the reference returns the best vertex of its last ordering (ties resolved as there)
+    best = np.array([_last_ilo(f[k]) for k in range(n)])
+    return sx[ar, best], out[ar, best], nev, conv
+
+
+def _last_ilo(f):
+    """The best vertex as the reference's ordering loop names it (ties resolved as there)."""
+    if f[0] > f[1]:
+        ilo = 1
+    else:
+        ilo = 0
+    for i in range(2, len(f)):
+        if f[i] <= f[ilo]:
+            ilo = i
+    return ilo

@@ -124,3 +124,25 @@ def test_gradient_ascent_vs_simplex(pgw, abi, orc):
               flush=True)
         assert F[k] >= -out_n[5] * (1 - 1e-5)
         assert F[k] >= F0 * (1 - 1e-5)
+
+
+def test_batched_amoeba_on_device(pgw, abi, orc):
+    """followup.amoeba fed by the device's Fstatnet, 8 simplices in each batched call, against the
+    oracle's Nelder-Mead from the same starts: both converge, to F within 1e-8 relative (their F
+    differ by ~1e-12 relative, which can change a near-tie decision and so the path)."""
+    from polgraw_allsky_amd import followup
+    seg, truth, amp, rho2 = noise_free_signal(N=86400, seed=8, amp=[0.7, 0.0, -0.4, 0.0])
+    sc = phase_scales(seg, truth)
+    starts = truth + np.random.default_rng(5).uniform(-0.8, 0.8, (8, 4)) * sc
+    h = pgw.Handle(seg, abi.PREC_FP64)
+    x, out, nev, conv = followup.amoeba(followup.FStatNet(h), starts, 0.5 * sc, tol=1e-9)
+    assert conv.all()
+    S = orc.Search(seg, nthreads=8)
+    same = 0
+    for k in range(len(starts)):
+        xr, outr, nevr = S.amoeba(starts[k], 0.5 * sc, tol=1e-9)
+        print(f"start {k}: device F {-out[k, 5]:.10f} ({nev[k]} evals), oracle F {-outr[5]:.10f} ({nevr})",
+              flush=True)
+        assert abs(out[k, 5] - outr[5]) <= 1e-8 * abs(outr[5])
+        same += int(nev[k] == nevr)
+    assert same >= len(starts) // 2

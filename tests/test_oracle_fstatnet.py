@@ -115,3 +115,35 @@ def test_amoeba_reaches_a_maximum(orc):
     assert -out[5] >= -F0[0, 5] * (1 - 1e-9)
     _, g = S.fstatnet(x[None, :])
     assert np.max(np.abs(g[0] * sc)) < 1e-3 * -out[5]
+
+
+class _OracleFS:
+    """The followup's batched driver fed by the oracle's Fstatnet (CPU): the same arithmetic as the
+    oracle's own Nelder-Mead, so the two must walk identical simplices."""
+
+    def __init__(self, S):
+        self.S = S
+
+    def __call__(self, pts, grad=False):
+        out, _ = self.S.fstatnet(np.asarray(pts, dtype=np.float64).reshape(-1, 4), grad=False)
+        return out, None
+
+
+def test_batched_amoeba_equals_reference_steps(orc):
+    """followup.amoeba (batched over simplices) against the oracle's restatement of followup.c's
+    amoeba, one start at a time: identical best points, outputs and evaluation counts."""
+    from polgraw_allsky_amd import followup
+    seg, truth, amp, rho2 = noise_free_signal(seed=8, amp=[0.7, 0.0, -0.4, 0.0])
+    S = orc.Search(seg, nthreads=8)
+    sc = phase_scales(seg, truth)
+    starts = truth + np.random.default_rng(3).uniform(-0.8, 0.8, (5, 4)) * sc
+    x, out, nev, conv = followup.amoeba(_OracleFS(S), starts, 0.5 * sc, tol=1e-9)
+    assert conv.all()
+    for k in range(len(starts)):
+        xr, outr, nevr = S.amoeba(starts[k], 0.5 * sc, tol=1e-9)
+        assert nev[k] == nevr
+        assert np.array_equal(x[k], xr)
+        assert np.array_equal(out[k], outr)
+    # an evaluation budget ends a simplex unconverged
+    _, _, nev2, conv2 = followup.amoeba(_OracleFS(S), starts[:2], 0.5 * sc, tol=1e-14, maxeval=12)
+    assert not conv2.any() and np.all(nev2 >= 12)
