@@ -73,16 +73,23 @@ template <int P2, int E> struct ColShape {
 // NA arrays x[a][r] (registers) -> X in registers: out[a][q*G + kg] = X[kr + E kg], kr = grp*Q + q.
 // lds must hold NA * P2 * C elements.  twc[m] = W_P2^m (forward sign).  One barrier inside; the
 // caller must barrier before reusing lds.
-// NZ < E: the inputs x[a][r], r >= NZ, are zero (pruned first DFT).
+// NZ < E: the inputs x[a][r], r >= NZ, are zero (pruned first DFT).  pw (optional): the step-1
+// twiddle powers of this thread, computed on the first call (pw_have = false), reused after.
 template <typename T2, int P2, int E, int NA, int DIR, int NZ = E>
-__device__ __forceinline__ void col_fft(T2 (&x)[NA][E], T2* lds, const T2* __restrict__ twc) {
+__device__ __forceinline__ void col_fft(T2 (&x)[NA][E], T2* lds, const T2* __restrict__ twc, T2* pw = nullptr,
+                                        bool pw_have = false) {
   using S = ColShape<P2, E>;
   const int c = threadIdx.x % S::C, g = threadIdx.x / S::C;
   const T2 wg = twc[g];
 #pragma unroll
   for (int a = 0; a < NA; a++) {
     dftP<E, NZ, DIR>(x[a]);
-    twiddle_run<E, DIR>(x[a], wg);  // W_P2^{g kr}
+    if (pw) {  // W_P2^{g kr} from the cached powers
+      if (!pw_have && a == 0) tw_powers<E>(tw_dir<DIR>(wg), pw);
+      apply_tw_pow<E, DIR>(x[a], pw);
+    } else {
+      twiddle_run<E, DIR>(x[a], wg);  // W_P2^{g kr}
+    }
 #pragma unroll
     for (int kr = 0; kr < E; kr++) lds[(size_t)a * P2 * S::C + (g * E + kr) * S::C + c] = x[a][kr];
   }

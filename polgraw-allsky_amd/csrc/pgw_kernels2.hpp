@@ -95,11 +95,16 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
       Za[L + (size_t)k2 * L1 + n1] = cmul(x[1][s], w[s]);
     }
   } else {
+    // FP32 with 16 rows per thread: the step-1 twiddle powers of the first spectrum serve the
+    // second (4 % fewer instructions at the same occupancy; with 8 rows the extra registers cost
+    // a wave per SIMD and more time than they save)
+    constexpr bool PW = sizeof(T) == 4 && E == 16;
+    T2 pw[PW ? E : 1];
 #pragma unroll
     for (int a = 0; a < 2; a++) {
       T2 (&xa)[1][E] = *reinterpret_cast<T2(*)[1][E]>(&x[a]);
       if (a) __syncthreads();
-      col_fft<T2, P2, E, 1, -1, NZ>(xa, lds, twc);
+      col_fft<T2, P2, E, 1, -1, NZ>(xa, lds, twc, PW ? pw : nullptr, a > 0);
 #pragma unroll
       for (int s = 0; s < E; s++) {
         const int k2 = col_out_k2<P2, E>(s);
