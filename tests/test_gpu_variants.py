@@ -123,3 +123,40 @@ def test_low_threshold_runs(pgw, abi, orc, name):
     assert np.array_equal(got[:, :4], ref[:, :4])
     Fg, Fr = got[:, 4] ** 2 / 2 + 2, ref[:, 4] ** 2 / 2 + 2
     assert np.max(np.abs(Fg - Fr) / Fr) <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["tiny-1det-12k", "huge-2det-500k"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_extreme_fft_lengths(pgw, abi, orc, name, prec):
+    """The smallest and largest supported F-statistic FFT (L = 2^14 with a 16-point column pass,
+    L = 2^20 with a 256-point column pass and Ninterp = 2^20) against the oracle, with the row
+    pass variants of each precision."""
+    seg = segment(name)
+    pm, mm, nn = injected_sky(seg)
+    sky = [(pm, mm, nn), (pm, mm + 1, nn)]
+    rng, opts = abi.make_range(seg), abi.make_opts(trl=12.0)
+    S = orc.Search(seg, nthreads=8)
+    ref = np.concatenate([S.job_core(*s, rng, opts)[0] for s in sky])
+    for tune in (["row8=0", "row8=2"] if prec == "fp64" else ["", "row_na=2"]):
+        old = os.environ.get("PGW_TUNE")
+        os.environ["PGW_TUNE"] = tune
+        try:
+            h = pgw.Handle(seg, abi.PREC_FP64 if prec == "fp64" else abi.PREC_FP32)
+        finally:
+            if old is None:
+                del os.environ["PGW_TUNE"]
+            else:
+                os.environ["PGW_TUNE"] = old
+        got, fg = h.job_batch(sky, rng, opts)
+        h.close()
+        print(name, prec, repr(tune), "records", len(got), len(ref), flush=True)
+        F = lambda snr: snr * snr / 2 + 2  # noqa: E731
+        if prec == "fp64":
+            assert got.shape == ref.shape and len(ref) > 0
+            assert np.array_equal(got[:, :4], ref[:, :4])
+            assert np.max(np.abs(F(got[:, 4]) - F(ref[:, 4])) / F(ref[:, 4])) <= 1e-9
+        else:
+            kg = {tuple(r[:4]): F(r[4]) for r in got}
+            strong = [r for r in ref if F(r[4]) > 12.05]
+            assert strong and all(tuple(r[:4]) in kg for r in strong)
+            assert max(abs(kg[tuple(r[:4])] - F(r[4])) / F(r[4]) for r in strong) <= 1e-4
