@@ -1,15 +1,16 @@
 // pgw_kernels.hpp - HIP kernels of the coherent F-statistic search core (gfx950).
 //
-// Sky-point stage (once per sky point, amortised over its spindowns):
+// Sky-point stage (once per sky point, amortised over its spindowns; each launch covers all the
+// sky points of a group):
 //   k_sky_prologue    modvir + shft + first demodulation    (jobcore.c:270-313, settings.c:281-325)
-//   k_fft_col/k_fft_row + loaders   resampling FFTs          (jobcore.c:319-362)
+//   k_fft_col2/k_fft_row2 + loaders  resampling FFTs         (jobcore.c:319-362)
 //   k_untranspose     T-layout -> natural order, times interpftpad/Ninterp
-//   k_spline/k_spline_eval               natural cubic spline (auxi.c:55-122), /sig2 (jobcore.c:386-389)
-//   k_aabb            noise-weighted sums aa, bb              (jobcore.c:375-393)
-// Template stage (per spindown, the hot loop):
-//   k_spin_col        phase modulation fused into the column pass of the F-statistic FFT
+//   k_spline          natural cubic spline (auxi.c:55-81) and the noise-weighted sums aa, bb
+//                     (jobcore.c:375-393); k_spline_eval: splint (auxi.c:83-122), /sig2
+// Template stage (per spindown, the hot loop; pgw_kernels2.hpp):
+//   k_spin_col2       phase modulation fused into the column pass of the F-statistic FFT
 //                     (jobcore.c:456-627)
-//   k_spin_row        row pass + F = |xa|^2/aa + |xb|^2/bb + FStat block partials (jobcore.c:629-651)
+//   k_spin_row2/8     row pass + F = |xa|^2/aa + |xb|^2/bb + FStat block partials (jobcore.c:629-651)
 //   k_threshold       FStat means, normalisation, run-max threshold, candidates (jobcore.c:659-695)
 #pragma once
 #include "pgw_common.hpp"

@@ -27,6 +27,10 @@ CASES = {
     # the precision configuration's shape: N=259200, nfft=2^18, fftpad 2, L=2^19
     "prec-2det-259k": dict(N=259200, dets=("H1", "L1"), sigmas=(1.0, 1.3), fftpad=2, seed=17, h0=0.06,
                            nsky=(1, 1), nspin=4),
+    # the most detectors a handle takes (8; sites repeat with independent noise)
+    "net-8det-16k": dict(N=16384, dets=("H1", "L1", "V1", "H1", "L1", "V1", "H1", "L1"),
+                         sigmas=(1.0, 1.3, 1.1, 0.9, 1.2, 1.0, 1.4, 1.1), fftpad=2, seed=31, h0=0.4,
+                         nsky=(2, 2), nspin=6),
     # smallest supported F-statistic FFT: N=12000 -> nfft = L = 2^14 (1024 x 16 split)
     "tiny-1det-12k": dict(N=12000, dets=("H1",), sigmas=(1.0,), fftpad=1, seed=23, h0=0.8, nsky=(2, 2),
                           nspin=6),

@@ -160,3 +160,19 @@ def test_extreme_fft_lengths(pgw, abi, orc, name, prec):
             strong = [r for r in ref if F(r[4]) > 12.05]
             assert strong and all(tuple(r[:4]) in kg for r in strong)
             assert max(abs(kg[tuple(r[:4])] - F(r[4])) / F(r[4]) for r in strong) <= 1e-4
+
+
+@pytest.mark.parametrize("white", [False, True])
+def test_eight_detectors_and_whitenoise(pgw, abi, orc, white):
+    """Eight detectors (the handle's maximum), and --whitenoise (no FStat normalisation,
+    jobcore.c:650): FP64 records equal the oracle's."""
+    seg = segment("net-8det-16k")
+    pm, mm, nn = injected_sky(seg)
+    rng, opts = abi.make_range(seg), abi.make_opts(trl=12.0, white=white)
+    ref, fr = orc.Search(seg, nthreads=8).job_core(pm, mm, nn, rng, opts)
+    got, fg = pgw.Handle(seg, abi.PREC_FP64).job_core(pm, mm, nn, rng, opts)
+    print("white", white, "records", len(got), flush=True)
+    assert fg == fr and got.shape == ref.shape and len(ref) > 0
+    assert np.array_equal(got[:, :4], ref[:, :4])
+    Fg, Fr = got[:, 4] ** 2 / 2 + 2, ref[:, 4] ** 2 / 2 + 2
+    assert np.max(np.abs(Fg - Fr) / Fr) <= 1e-9
