@@ -71,7 +71,8 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
     T2 xa = mk<T2>(0, 0), xb = mk<T2>(0, 0);
     if (r < NZ && n < N) {
       const double dn = (double)n;
-      for (int d = 0; d < nifo; d++) {
+#pragma unroll 2
+      for (int d = 0; d < nifo; d++) {  // two detectors' loads in flight together
         const double sh = ld_global(shft, (size_t)d * N + n);
         // _tmp1 = i^2 + 2 i shft, phase = het1 i + sgnlt1 _tmp1 (jobcore.c:294,530), uncontracted
         const double tmp1 = lin2_nc(dn, dn, (double)(2 * n), sh);

@@ -29,13 +29,13 @@ CASES = {
                            nsky=(1, 1), nspin=4),
     # the most detectors a handle takes (8; sites repeat with independent noise)
     "net-8det-16k": dict(N=16384, dets=("H1", "L1", "V1", "H1", "L1", "V1", "H1", "L1"),
-                         sigmas=(1.0, 1.3, 1.1, 0.9, 1.2, 1.0, 1.4, 1.1), fftpad=2, seed=31, h0=0.4,
+                         sigmas=(1.0, 1.3, 1.1, 0.9, 1.2, 1.0, 1.4, 1.1), fftpad=2, seed=32, h0=0.4,
                          nsky=(2, 2), nspin=6),
     # smallest supported F-statistic FFT: N=12000 -> nfft = L = 2^14 (1024 x 16 split)
     "tiny-1det-12k": dict(N=12000, dets=("H1",), sigmas=(1.0,), fftpad=1, seed=23, h0=0.8, nsky=(2, 2),
                           nspin=6),
     # largest: N=500000 -> nfft = 2^19, fftpad 2, L = 2^20 (4096 x 256 split), Ninterp = 2^20
-    "huge-2det-500k": dict(N=500000, dets=("H1", "L1"), sigmas=(1.0, 1.3), fftpad=2, seed=29, h0=0.05,
+    "huge-2det-500k": dict(N=500000, dets=("H1", "L1"), sigmas=(1.0, 1.3), fftpad=2, seed=29, h0=0.1,
                            nsky=(1, 1), nspin=3),
     # the batching configuration's shorter segments: N=60000 -> nfft = L = 2^16, three detectors, gaps
     "batch-3det-60k": dict(N=60000, dets=("H1", "L1", "V1"), sigmas=(1.0, 1.3, 1.1), fftpad=1, seed=19, h0=0.12,
