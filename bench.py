@@ -405,6 +405,15 @@ def bench_differentiation(args, npts=1024):
     xs, outs, nev_s, conv_s = followup.amoeba(followup.FStatNet(h), pts, 0.5 * sc, tol=1e-9)
     t_nm = time.perf_counter() - t3
     print(f"simplex: {npts} points, {int(nev_s.sum())} evaluations in {t_nm:.3f} s", file=sys.stderr, flush=True)
+    # the followup's mesh search (invertedMADS), 80 mesh points of every candidate per device call,
+    # from the first 128 points (its ~470 rounds per candidate make it the slowest of the three)
+    nod = seg.sett["N"] * seg.sett["dt"] / 86400.0
+    nm = min(128, npts)
+    t4 = time.perf_counter()
+    xm, outm, rounds_m = followup.mads(followup.FStatNet(h), pts[:nm], 2.0 * sc, nod)
+    t_mads = time.perf_counter() - t4
+    print(f"mads: {nm} points, {int(rounds_m.sum()) * 80} evaluations in {t_mads:.3f} s", file=sys.stderr,
+          flush=True)
     cpu = None
     if not args.no_cpu:
         orc = oracle()
@@ -443,6 +452,10 @@ def bench_differentiation(args, npts=1024):
                              "converged": int(conv_s.sum()), "seconds": round(t_nm, 3),
                              "evaluations_per_s": round(float(nev_s.sum()) / t_nm, 1),
                              "F_end_median": float(np.median(-outs[:, 5]))},
+                 "mads": {"points": nm, "p2": "2 phase scales", "rounds_mean": round(float(rounds_m.mean()), 1),
+                          "seconds": round(t_mads, 3),
+                          "evaluations_per_s": round(float(rounds_m.sum()) * 80 / t_mads, 1),
+                          "F_end_median": float(np.median(-outm[:, 5]))},
                  "roofline": rl, "cpu_baseline": cpu})
     return line
 

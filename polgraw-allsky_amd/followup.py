@@ -7,6 +7,8 @@ followup.c:1126-1173 (`amoeba`) when the gradient is available.
 """
 from __future__ import annotations
 
+import itertools
+
 import numpy as np
 
 from . import pgw
@@ -240,3 +242,49 @@ def _last_ilo(f):
         if f[i] <= f[ilo]:
             ilo = i
     return ilo
+
+
+# The 80 points of the 3^4 mesh around the centre as multiples (-1, 0, +1) of the mesh sizes,
+# grouped as in the reference (followup/src/followup.c:576-683): vertices (all four parameters
+# moved), edges (three), faces (two), cells (one).  The order inside the list only decides exact
+# ties, which the reference leaves to its OpenMP schedule.
+def _mads_stencil() -> np.ndarray:
+    pts = [p for p in itertools.product((1.0, 0.0, -1.0), repeat=4) if any(p)]
+    return np.array(sorted(pts, key=lambda p: -sum(v != 0.0 for v in p)))
+
+
+MADS_STENCIL = _mads_stencil()
+
+
+def mads(fs: FStatNet, starts, p2, nod: float, limit: int = 700):
+    """The followup's mesh search (invertedMADS, followup/src/followup.c:537-754) for many
+    candidates at once: each round evaluates, for every unfinished candidate, the 80 points of the
+    3^4 mesh around its current best point (mesh sizes `param`) in one batched Fstatnet call.  A
+    better point becomes the centre and the mesh restarts at (1 - scale) p2; otherwise the mesh
+    grows by (1 - scale) p2; scale = 1 - 0.06 / nod.  A candidate finishes when every mesh size
+    exceeds p2, or after `limit` rounds.  Returns (points (n, 4), their 11 outputs (n, 11), rounds
+    (n,))."""
+    x = np.asarray(starts, dtype=np.float64).reshape(-1, 4).copy()
+    n = len(x)
+    p2 = np.broadcast_to(np.asarray(p2, dtype=np.float64), x.shape)
+    step = (1.0 - (1.0 - 0.01 * 6.0 / nod)) * p2
+    param = step.copy()
+    best = fs(x, grad=False)[0]
+    rounds = np.zeros(n, int)
+    act = np.ones(n, bool)
+    while act.any():
+        idx = np.nonzero(act)[0]
+        rounds[idx] += 1
+        pts = x[idx, None, :] + MADS_STENCIL[None, :, :] * param[idx, None, :]
+        out = fs(pts.reshape(-1, 4), grad=False)[0].reshape(len(idx), 80, 11)
+        j = np.argmin(out[:, :, 5], axis=1)              # first minimum in stencil order
+        cand = out[np.arange(len(idx)), j]
+        better = cand[:, 5] < best[idx, 5]
+        bi = idx[better]
+        x[bi] = pts[np.arange(len(idx))[better], j[better]]
+        best[bi] = cand[better]
+        param[bi] = step[bi]
+        gi = idx[~better]
+        param[gi] = param[gi] + step[gi]
+        act[idx] = np.any(param[idx] <= p2[idx], axis=1) & (rounds[idx] < limit)
+    return x, best, rounds

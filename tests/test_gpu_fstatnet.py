@@ -146,3 +146,28 @@ def test_batched_amoeba_on_device(pgw, abi, orc):
         assert abs(out[k, 5] - outr[5]) <= 1e-8 * abs(outr[5])
         same += int(nev[k] == nevr)
     assert same >= len(starts) // 2
+
+
+def test_batched_mads_on_device(pgw, abi, orc):
+    """followup.mads fed by the device's Fstatnet (80 mesh points of every unfinished candidate
+    per call) against the same search fed by the oracle: the same end points' F within 1e-8
+    relative and the same number of rounds for most candidates (F differ by ~1e-12 relative, which
+    can change a near-tie)."""
+    from polgraw_allsky_amd import followup
+    seg, truth, amp, rho2 = noise_free_signal(N=86400, seed=8, amp=[0.7, 0.0, -0.4, 0.0])
+    sc = phase_scales(seg, truth)
+    nod = seg.sett["N"] * seg.sett["dt"] / 86400.0
+    starts = truth + np.random.default_rng(7).uniform(-0.8, 0.8, (6, 4)) * sc
+    h = pgw.Handle(seg, abi.PREC_FP64)
+    x, out, rounds = followup.mads(followup.FStatNet(h), starts, 2.0 * sc, nod)
+    S = orc.Search(seg, nthreads=8)
+
+    def ofs(pts, grad=False):
+        return S.fstatnet(np.asarray(pts).reshape(-1, 4), grad=False)[0], None
+
+    xo, outo, ro = followup.mads(ofs, starts, 2.0 * sc, nod)
+    print("rounds device", rounds, "oracle", ro, flush=True)
+    assert np.all(np.abs(out[:, 5] - outo[:, 5]) <= 1e-8 * np.abs(outo[:, 5]))
+    assert np.sum(rounds == ro) >= len(starts) // 2
+    F0 = -h.fstatnet(starts, grad=False)[0][:, 5]
+    assert np.all(-out[:, 5] > F0)

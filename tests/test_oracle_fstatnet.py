@@ -147,3 +147,48 @@ def test_batched_amoeba_equals_reference_steps(orc):
     # an evaluation budget ends a simplex unconverged
     _, _, nev2, conv2 = followup.amoeba(_OracleFS(S), starts[:2], 0.5 * sc, tol=1e-14, maxeval=12)
     assert not conv2.any() and np.all(nev2 >= 12)
+
+
+def _mads_reference_steps(S, start, p2, nod, limit=700):
+    """invertedMADS (followup.c:537-754) one candidate and one mesh point at a time, with the
+    oracle's Fstatnet: the sequential form the batched driver must reproduce."""
+    from polgraw_allsky_amd import followup
+    step = (0.01 * 6.0 / nod) * np.asarray(p2, dtype=np.float64)
+    param = step.copy()
+    x = np.asarray(start, dtype=np.float64).copy()
+    best = S.fstatnet(x[None, :], grad=False)[0][0]
+    count = 0
+    while np.any(param <= p2) and count < limit:
+        count += 1
+        maxres = best.copy()
+        arg = None
+        for d in followup.MADS_STENCIL:
+            p = x + d * param
+            res = S.fstatnet(p[None, :], grad=False)[0][0]
+            if res[5] < maxres[5]:
+                maxres, arg = res, p
+        if arg is None:                  # the centre is the best: grow the mesh
+            param = param + step
+        else:                            # move to the better point, restart the mesh
+            x, best, param = arg, maxres, step.copy()
+    return x, best, count
+
+
+def test_batched_mads_equals_reference_steps(orc):
+    """followup.mads (batched over candidates and mesh points) against the mesh search one point
+    at a time: identical points, outputs and rounds; the end point improves on the start."""
+    from polgraw_allsky_amd import followup
+    seg, truth, amp, rho2 = noise_free_signal(N=8192, seed=8, amp=[2.8, 0.0, -1.6, 0.0])
+    S = orc.Search(seg, nthreads=2)
+    sc = phase_scales(seg, truth)
+    nod = seg.sett["N"] * seg.sett["dt"] / 86400.0
+    starts = truth + np.random.default_rng(6).uniform(-0.6, 0.6, (2, 4)) * sc
+    p2 = 0.5 * sc  # a coarse mesh limit relative to the short segment keeps the search short
+    x, out, rounds = followup.mads(_OracleFS(S), starts, p2, nod)
+    assert np.all(rounds < 700)
+    for k in range(len(starts)):
+        xr, outr, cr = _mads_reference_steps(S, starts[k], p2, nod)
+        assert rounds[k] == cr
+        assert np.array_equal(x[k], xr) and np.array_equal(out[k], outr, equal_nan=True)
+        F0 = -S.fstatnet(starts[k][None, :], grad=False)[0][0, 5]
+        assert -out[k, 5] > F0
