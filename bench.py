@@ -147,8 +147,10 @@ def measured_flops(kernel):
 
 
 def alg_bytes(seg, p):
-    """Algorithmic bytes per template of the column and row passes (both stated in DESIGN.md);
-    p = bytes per real of the series and spectra."""
+    """Algorithmic bytes per template of the column and row passes: the column pass reads the
+    nifo resampled series a, b (p-byte reals) and the FP64 shifts and writes two L-point spectra;
+    the row pass reads the two spectra and writes F once.  p = bytes per real of the series and
+    spectra."""
     s = seg.sett
     nifo, N, L = len(seg.dets), s["N"], s["nfftf"]
     M = s["nmax"] - s["nmin"]
@@ -478,7 +480,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(sky points shard across GPUs with no collectives; see DESIGN.md)", file=sys.stderr)
+              "(sky points would shard across GPUs, one process and handle each, with no collectives)",
+              file=sys.stderr)
         sys.exit(2)
     if args.copy_calib:
         ms = pgw.copy_bench(256 << 20, reps=3)

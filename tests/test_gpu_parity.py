@@ -1,6 +1,6 @@
 """HIP search core vs the CPU oracle on the same seeded inputs (FP64 path).
 
-Tolerances (north star): candidate identities (ss, bin, sky) identical, so the records'
+Tolerances: candidate identities (ss, bin, sky) identical, so the records'
 frequency/spindown/sky fields are bit-equal; F within 1e-9 relative, measured as
 |dF| / max(|F|, 1) so that the handful of bins with F near 0 (absolute FFT round-off) do
 not dominate.
