@@ -243,10 +243,12 @@ def _settings_dict(s) -> dict:
 
 
 def load_frame(dtaprefix, ident: int, band: int, dt: float = 2.0, nod: int = 2, label: str = "",
-               fpo: float | None = None, dets=None, range_file=None) -> Frame:
+               fpo: float | None = None, dets=None, range_file=None, addsig=None) -> Frame:
     """Reads a frame the way main() does (read_grid, search_settings, detectors_settings,
-    init_arrays, rogcvir and set_search_range; search/network/src-cpu/main.c:72-120).
-    dets: detector names in order (default: the subdirectories holding this band's data, sorted)."""
+    init_arrays, rogcvir, then add_signal or set_search_range; search/network/src-cpu/main.c:72-121).
+    dets: detector names in order (default: the subdirectories holding this band's data, sorted).
+    addsig: a signal file (the -addsig option): the signal is added to the data after the noise
+    variances are taken and the range is the box around it; range_file is then not read."""
     from . import pgw
     base = Path(dtaprefix) / f"{ident:03d}"
     xname = f"xdatc_{ident:03d}_{band:04d}{label}.bin"
@@ -273,6 +275,10 @@ def load_frame(dtaprefix, ident: int, band: int, dt: float = 2.0, nod: int = 2, 
         phir.append(ph)
         amod.append(pgw.rogcvir(DETECTORS[d][0], DETECTORS[d][2]))
         sig2.append(pgw.sig2(x))
-    rng = read_range(range_file) if range_file else {}
-    return Frame(sett=_settings_dict(s), M=M, dets=list(dets), xDat=xs, DetSSB=ssb, phir=phir, epsm=epsm,
-                 amod=amod, sig2=sig2, range=rng)
+    rng = read_range(range_file) if range_file and not addsig else {}
+    fr = Frame(sett=_settings_dict(s), M=M, dets=list(dets), xDat=xs, DetSSB=ssb, phir=phir, epsm=epsm,
+               amod=amod, sig2=sig2, range=rng)
+    if addsig:
+        from . import inject
+        inject.add_signal(fr, inject.read_signal(addsig), ident)
+    return fr
