@@ -156,10 +156,11 @@ template <int P1> struct RowShape {
 // twr[m] = W_P1^m (forward sign).  Barriers inside; ends after a barrier-free read (caller must
 // barrier before reusing lds).
 // pc (optional): the stage-C twiddle powers of this thread, computed on the first call (pc_have =
-// false) and reused on later calls of the same thread (the second spectrum of a row).
+// false) and reused on later calls of the same thread (the second spectrum of a row); pb likewise
+// the stage-B powers.
 template <typename T2, int P1, int NA, int DIR>
 __device__ __forceinline__ void row_fft(T2 (&x)[NA][16], T2* lds, const T2* __restrict__ twr, T2* pc = nullptr,
-                                        bool pc_have = false) {
+                                        bool pc_have = false, T2* pb = nullptr) {
   using S = RowShape<P1>;
   constexpr int NT = S::NT, R3 = S::R3, PADN = S::PADN;
   const int t = threadIdx.x;
@@ -183,7 +184,12 @@ __device__ __forceinline__ void row_fft(T2 (&x)[NA][16], T2* lds, const T2* __re
     for (int a = 0; a < NA; a++) {
 #pragma unroll
       for (int r = 0; r < 16; r++) x[a][r] = lds[a * PADN + rpad(t + NT * r)];
-      twiddle_run<16, DIR>(x[a], wB);  // W_256^{k r}
+      if (pb) {  // W_256^{k r} from the cached powers
+        if (!pc_have && a == 0) tw_powers<16>(tw_dir<DIR>(wB), pb);
+        apply_tw_pow<16, DIR>(x[a], pb);
+      } else {
+        twiddle_run<16, DIR>(x[a], wB);  // W_256^{k r}
+      }
       dftR<16, DIR>(x[a]);
     }
     __syncthreads();

@@ -180,13 +180,21 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
     // registers at the peak: they live while the prefetched row no longer does)
     constexpr bool PC = sizeof(T) == 4;
     T2 pc[PC ? 16 : 1];
-    row_fft<T2, P1, 1, -1>(x, lds, twr, PC ? pc : nullptr, false);
+    // and the stage-B powers too (-3 % VALU instructions), except for rows of 2048, whose kernel
+    // fits 128 VGPRs (four waves per SIMD) only without them
+#ifndef PGW_ROW_PB
+#define PGW_ROW_PB 1
+#endif
+    constexpr bool PB = PC && P1 != 2048 && PGW_ROW_PB;
+    T2 pb[PB ? 16 : 1];
+    T2* pbp = PB ? pb : nullptr;
+    row_fft<T2, P1, 1, -1>(x, lds, twr, PC ? pc : nullptr, false, pbp);
 #pragma unroll
     for (int m = 0; m < MO; m++) F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * raa;
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 16; m++) x[0][m] = PREF ? y[0][PREF ? m : 0] : rowb[tid + NT * m];
-    row_fft<T2, P1, 1, -1>(x, lds, twr, PC ? pc : nullptr, true);
+    row_fft<T2, P1, 1, -1>(x, lds, twr, PC ? pc : nullptr, true, pbp);
 #pragma unroll
     for (int m = 0; m < MO; m++) F[m] = F[m] + (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * rbb;
   }
