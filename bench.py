@@ -182,6 +182,8 @@ def roofline(kt, segw, prec_bytes, workload, tuned=None):
     return dict(bound="hbm", kernel=kname, achieved=round(achieved, 1),
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
                 traffic=(int(tr * ntl) if tr else None), traffic_unit="bytes per launch",
+                # counter-measured HBM bytes over this run's kernel duration
+                traffic_GBps=(round(tr * n / (kt[k + "_ms"] * 1e-3) / 1e9, 1) if tr else None),
                 valu_busy=measured_valu(kname, prec),
                 timing=f"HIP events around every {KTIME_EVERY}th template launch of the timed steps",
                 launches=nl, templates_per_launch=round(ntl, 2),
@@ -271,6 +273,7 @@ def bench_templates(args, name):
         print(f"fp64: {n64} templates in {t64:.3f} s -> {v64:.0f} templates/s (gpu {g64:.1f} ms)",
               file=sys.stderr, flush=True)
         out_fp64 = dict(value=round(v64, 1), ms_per_step=round(1e3 * t64 / args.steps, 3),
+                        ms_per_sky_point=round(1e3 * t64 / (args.steps * per_step), 4),
                         gpu_ms_per_step=round(g64 / args.steps, 3), templates=n64,
                         roofline=roofline(kt64, [(seg, 1)], 8, name, tune.get("fp64")))
         acc = cand_compare(r32, r64)
@@ -285,7 +288,8 @@ def bench_templates(args, name):
            "L": s["nfftf"], "detectors": len(seg.dets), "sky_points": len(sky), "spindowns": nspin,
            "sky_per_step": per_step, "tuning": tune}
     line = base_line(args, v32, t32, cfg)
-    line.update({"templates": n32, "gpu_ms_per_step": round(g32 / args.steps, 3), "fp64": out_fp64,
+    line.update({"templates": n32, "gpu_ms_per_step": round(g32 / args.steps, 3),
+                 "ms_per_sky_point": round(1e3 * t32 / (args.steps * per_step), 4), "fp64": out_fp64,
                  "speedup_fp32_vs_fp64": (round(v32 / out_fp64["value"], 3) if out_fp64 else None),
                  "accuracy_fp32_vs_fp64": acc, "roofline": rl, "cpu_baseline": cpu})
     return line
