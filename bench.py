@@ -185,7 +185,8 @@ def roofline(kt, segw, prec_bytes, workload, tuned=None):
                 # counter-measured HBM bytes over this run's kernel duration
                 traffic_GBps=(round(tr * n / (kt[k + "_ms"] * 1e-3) / 1e9, 1) if tr else None),
                 valu_busy=measured_valu(kname, prec),
-                timing=f"HIP events around every {KTIME_EVERY}th template launch of the timed steps",
+                timing=(f"HIP events around every {KTIME_EVERY}th template launch of the timed steps; "
+                        "each launch's threshold runs inside the next launch's column pass"),
                 launches=nl, templates_per_launch=round(ntl, 2),
                 ms_per_launch={x: round(kt[x + "_ms"] / nl, 4) for x in ("col", "row", "thr")},
                 bytes_per_template=dict(col=round(b_col), row=round(b_row)),

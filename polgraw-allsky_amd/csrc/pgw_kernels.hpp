@@ -11,7 +11,9 @@
 //   k_spin_col2       phase modulation fused into the column pass of the F-statistic FFT
 //                     (jobcore.c:456-627)
 //   k_spin_row2/8     row pass + F = |xa|^2/aa + |xb|^2/bb + FStat block partials (jobcore.c:629-651)
-//   k_threshold       FStat means, normalisation, run-max threshold, candidates (jobcore.c:659-695)
+//   k_threshold       FStat means, normalisation, run-max threshold, candidates (jobcore.c:659-695);
+//                     in a search its waves run in the first grid rows of the next launch's
+//                     column pass (thr_wave), only a group's last launch has a launch of its own
 #pragma once
 #include "pgw_common.hpp"
 
@@ -457,13 +459,22 @@ __device__ __forceinline__ double part_mean(const double sum, int part, const Th
   return (part < p.nblk && !p.white) ? sum / (2. * PGW_NAV_DEV) : 1.0;
 }
 
-template <typename T>
-__global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restrict__ Ft, const T* __restrict__ psum,
-                                                  const T* __restrict__ pmax, double* __restrict__ mu_out,
-                                                  Cand* __restrict__ cand, int* __restrict__ ncand) {
-  constexpr int MAXP = 256;
-  __shared__ double mu[MAXP];
-  const int part = blockIdx.x, t = blockIdx.y, lane = threadIdx.x;
+constexpr int THR_MAXP = 256;  // FStat blocks per template at most
+
+// LDS written by some lanes of a wave made visible to the wave's other lanes
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// One wave (lane = 0..63) thresholds FStat block `part` of template t; mu: THR_MAXP doubles of LDS
+// owned by this wave.  TJ, QB: loads in flight per batch in the flagged path (bins, block means).
+template <typename T, int TJ = (sizeof(T) == 4 ? 32 : 16), int QB = 16>
+__device__ __forceinline__ void thr_wave(const ThrParams& p, int part, int t, int lane, double* mu,
+                                         const T* __restrict__ Ft, const T* __restrict__ psum,
+                                         const T* __restrict__ pmax, double* __restrict__ mu_out,
+                                         Cand* __restrict__ cand, int* __restrict__ ncand) {
   const size_t L = (size_t)p.L1 * p.L2;
   const T* F = Ft + (size_t)t * L;
   const T* ps0 = psum + (size_t)t * p.nparts * p.L2;
@@ -488,7 +499,7 @@ __global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restri
   auto at = [&](int k) { return (double)F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)]; };
   // (TJ loads per batch keep the kernel's register count, and so the occupancy of the far more
   // common unflagged waves, moderate)
-  constexpr int TJ = sizeof(T) == 4 ? 32 : 16;
+  static_assert(NJ % TJ == 0, "whole batches");
   const double pre = p.trl * m * (1. - 1e-14);
   uint64_t hits = 0;
 #pragma unroll 1
@@ -504,7 +515,6 @@ __global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restri
       hits |= (uint64_t)(lok + lane + 64 * (j0 + j) < hik && (double)f[j] > pre) << (j0 + j);
   }
   // means of all parts with the unflagged waves' arithmetic, QB parts' loads in flight at a time
-  constexpr int QB = 16;
 #pragma unroll 1
   for (int q0 = 0; q0 < p.nparts; q0 += QB) {
     double v[QB];
@@ -523,7 +533,7 @@ __global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restri
       if (lane == 0 && q0 + j < p.nparts) mu[q0 + j] = part_mean(sq, q0 + j, p);
     }
   }
-  __syncthreads();
+  wave_lds_sync();
   // neighbours' pre-selection bits: bin k - 1 and k + 1 of this part are lane - 1 / lane + 1 (same
   // j), or lane 63 (j - 1) / lane 0 (j + 1) across the wave's edge.  A neighbour without its bit is
   // certainly not above trl, so its exact test (a dependent load) is skipped.
@@ -565,6 +575,28 @@ __global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restri
     }
   }
 }
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restrict__ Ft, const T* __restrict__ psum,
+                                                  const T* __restrict__ pmax, double* __restrict__ mu_out,
+                                                  Cand* __restrict__ cand, int* __restrict__ ncand) {
+  __shared__ double mu[THR_MAXP];
+  thr_wave<T>(p, blockIdx.x, blockIdx.y, threadIdx.x, mu, Ft, psum, pmax, mu_out, cand, ncand);
+}
+
+// A threshold launch folded into the next column-pass launch: the first `rows` grid rows of that
+// launch run it, one (part, template) per wave, beside the column pass's own workgroups.
+template <typename T> struct ThrFuse {
+  ThrParams p;
+  const T* Ft;
+  const T* psum;
+  const T* pmax;
+  double* mu;
+  Cand* cand;
+  int* ncand;
+  int nt;    // templates of the folded threshold (0: none)
+  int rows;  // grid rows appended for it
+};
 
 // zero n ints (used instead of a memset so the launch sequence can be captured in a graph)
 __global__ void k_zero_int(int* __restrict__ p, int n) {

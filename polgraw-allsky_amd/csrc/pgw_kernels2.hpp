@@ -28,16 +28,29 @@ template <> struct ColE<double, 256> { static constexpr int E = 16; static const
 // zero for n >= N; the column FFT and the W_L^{n1 k2} twiddle follow; Z[t][{a,b}][k2][n1].
 // NZ: register rows r >= NZ hold n >= L1 P2 NZ / E >= N for every template of the launch
 // (zero-padded input), so they are neither demodulated nor transformed.
+// tf: the previous launch's threshold, run by the first tf.rows grid rows (none when tf.rows = 0):
+// dispatched first, their short dependent loads overlap the column pass's workgroups.
 template <typename T, int P2, bool SEQ, bool UNI, int NZ = ColE<T, P2>::E>
 __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, const SpinTab tab,
                                                    const TmplDesc* __restrict__ desc,
                                                    cplx<T>* __restrict__ Z, const cplx<T>* __restrict__ twc,
-                                                   const cplx<T>* __restrict__ hi, const cplx<T>* __restrict__ lo) {
+                                                   const cplx<T>* __restrict__ hi, const cplx<T>* __restrict__ lo,
+                                                   const ThrFuse<T> tf) {
   using T2 = cplx<T>;
   constexpr int E = ColE<T, P2>::E;
   using S = ColShape<P2, E>;
   __shared__ T2 lds[(SEQ ? 1 : 2) * P2 * S::C];
-  const int t = blockIdx.y;
+  static_assert(sizeof(lds) >= 4 * THR_MAXP * sizeof(double), "threshold waves' mean tables fit the LDS");
+  if ((int)blockIdx.y < tf.rows) {  // whole workgroups: no barrier is split
+    const int w = threadIdx.x >> 6;
+    const int item = ((int)blockIdx.y * (int)gridDim.x + (int)blockIdx.x) * 4 + w;
+    // few loads in flight in the (rare) flagged path, so the column pass keeps its registers
+    if (item < tf.p.nparts * tf.nt)
+      thr_wave<T, 4, 2>(tf.p, item % tf.p.nparts, item / tf.p.nparts, threadIdx.x & 63,
+                  reinterpret_cast<double*>(lds) + w * THR_MAXP, tf.Ft, tf.psum, tf.pmax, tf.mu, tf.cand, tf.ncand);
+    return;
+  }
+  const int t = blockIdx.y - tf.rows;
   int slot, N;
   double het1, sgnlt1;
   if constexpr (UNI) {

@@ -162,13 +162,20 @@ void launch_row(const Plan<T>& pl, cplx<T>* Z, int nbatch, hipStream_t st) {
 
 // Nl: the largest N of the launch's templates; register rows of the column pass that only see
 // zero padding for that N are pruned (three variants: all E rows, E/2, 3E/8).
+// tf: a threshold to fold into the launch (tf.nt = 0: none); its rows are set here.
 template <typename T>
 void launch_spin_col(const Plan<T>& pl, const SlotBases& sb, const SpinTab& tab, const TmplDesc* desc, cplx<T>* Z,
-                     int nt, int Nl, const cplx<T>* hi, const cplx<T>* lo, bool seq, hipStream_t st) {
+                     int nt, int Nl, const cplx<T>* hi, const cplx<T>* lo, bool seq, const ThrFuse<T>& tf,
+                     hipStream_t st) {
   const SpinParams sp{pl.L1, pl.twshift};
 #define PGW_SCOL_NZ(P2, SQ, U, NZ)                                                                        \
-  hipLaunchKernelGGL((k_spin_col2<T, P2, SQ, U, NZ>), dim3(pl.L1 / col_tile<T, P2>(), nt), dim3(256), 0, st, sp, \
-                     sb, tab, desc, Z, pl.twc.p, hi, lo)
+  do {                                                                                                    \
+    const long gx_ = pl.L1 / col_tile<T, P2>();                                                           \
+    ThrFuse<T> tf_ = tf;                                                                                  \
+    tf_.rows = tf.nt > 0 ? (int)(((long)tf.p.nparts * tf.nt + 4 * gx_ - 1) / (4 * gx_)) : 0;             \
+    hipLaunchKernelGGL((k_spin_col2<T, P2, SQ, U, NZ>), dim3(gx_, nt + tf_.rows), dim3(256), 0, st, sp, sb, tab, \
+                       desc, Z, pl.twc.p, hi, lo, tf_);                                                   \
+  } while (0)
 #define PGW_SCOL_GO(P2, SQ, U)                                                                           \
   do {                                                                                                   \
     constexpr int E_ = ColE<T, P2>::E;                                                                   \
@@ -344,6 +351,10 @@ template <typename T> struct Engine final : pgw_handle {
   // row pass with 8 elements per thread (1 with the second row prefetched, 2 without): the FP64
   // default (two workgroups per CU instead of one); FP32 keeps 16 elements per thread
   int row8 = std::is_same<T, double>::value ? 2 : 0;
+  // each launch's threshold folded into the next launch's column pass (the group's last one runs
+  // on its own); thr_prev: the threshold still to run (nt = 0: none)
+  bool thr_fuse = true;
+  ThrFuse<T> thr_prev{};
 
   // kernel timing over searches: 4 events per template launch (column, row, threshold)
   int ktime = 0;         // record every ktime-th template launch (0: off)
@@ -506,6 +517,7 @@ template <typename T> struct Engine final : pgw_handle {
       if (const char* q = strstr(tune, "spl_k=")) if (sscanf(q, "spl_k=%d", &v) == 1 && (v == 4 || v == 8 || v == 16)) spl_k = v;
       if (const char* q = strstr(tune, "half=")) if (sscanf(q, "half=%d", &v) == 1) row_half = v != 0;
       if (const char* q = strstr(tune, "row8=")) if (sscanf(q, "row8=%d", &v) == 1 && v >= 0 && v <= 2) row8 = v;
+      if (const char* q = strstr(tune, "thr_fuse=")) if (sscanf(q, "thr_fuse=%d", &v) == 1) thr_fuse = v != 0;
     }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
@@ -695,10 +707,22 @@ template <typename T> struct Engine final : pgw_handle {
     }
   }
 
+  // runs a threshold left pending by a folded launch
+  int flush_thr() {
+    if (thr_prev.nt > 0) {
+      const ThrFuse<T>& f = thr_prev;
+      hipLaunchKernelGGL((k_threshold<T>), dim3(f.p.nparts, f.nt), dim3(64), 0, stream, f.p, f.Ft, f.psum, f.pmax,
+                         f.mu, f.cand, f.ncand);
+      thr_prev.nt = 0;
+    }
+    return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
+  }
+
   // template stage for descriptors [t0, t0 + nt) of buffer buf; ev (optional) = 4 events around the kernels
   // uni: all nt templates belong to one item (its templates j0, j0 + 1, ...), else nullptr
+  // fold: leave this launch's threshold pending, to run inside the next column launch (or flush_thr)
   int launch_templates(const Seg& g, int buf, int t0, int nt, const Item* uni, int j0, int Nl, const pgw_opts* o,
-                       hipEvent_t* ev = nullptr) {
+                       hipEvent_t* ev = nullptr, bool fold = false) {
     const TmplDesc* d = desc.p + (size_t)buf * kDescCap + t0;
     if (ev) HIPCHK(hipEventRecord(ev[0], stream));
     SlotBases sb{xm_slots.p, shft_slots.p, aabb_slots.p, slot_xm, slot_shft, nifo, -1, 0, 0};
@@ -712,14 +736,27 @@ template <typename T> struct Engine final : pgw_handle {
         for (int i = 0; i < nt; i++) spin_coeffs(q, uni->ss_at(j0 + i), tab.het1[i], tab.sgnlt1[i]);
       }
     }
-    launch_spin_col<T>(*g.ps, sb, tab, d, Z.p, nt, uni ? segs[uni->seg]->sett.N : Nl, hi.p, lo.p, col_seq, stream);
+    // the pending threshold reads F and the FStat partials of the previous launch, which the row
+    // pass below overwrites only after this column launch has finished
+    ThrFuse<T> tf{};
+    if (thr_prev.nt > 0) {
+      if (fold) tf = thr_prev;
+      else if (int s = flush_thr()) return s;
+      thr_prev.nt = 0;
+    }
+    launch_spin_col<T>(*g.ps, sb, tab, d, Z.p, nt, uni ? segs[uni->seg]->sett.N : Nl, hi.p, lo.p, col_seq, tf,
+                       stream);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));
     RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2)};
     launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, row_half, row8, stream);
     if (ev) HIPCHK(hipEventRecord(ev[2], stream));
     ThrParams tp{g.ps->L1, g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, o->white_flag, t0, kCandCap, ilog2(g.ps->L2), o->trl};
-    hipLaunchKernelGGL((k_threshold<T>), dim3(g.nparts, nt), dim3(64), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p,
-                       cand.p + (size_t)buf * kCandCap, ncand.p + buf);
+    if (fold) {
+      thr_prev = ThrFuse<T>{tp, Ft.p, psum.p, pmax.p, mu.p, cand.p + (size_t)buf * kCandCap, ncand.p + buf, nt, 0};
+    } else {
+      hipLaunchKernelGGL((k_threshold<T>), dim3(g.nparts, nt), dim3(64), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p,
+                         cand.p + (size_t)buf * kCandCap, ncand.p + buf);
+    }
     if (ev) HIPCHK(hipEventRecord(ev[3], stream));
     return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
   }
@@ -752,9 +789,12 @@ template <typename T> struct Engine final : pgw_handle {
       const bool one = t0 + nt <= it.t0 + it.ns();
       const bool timed = ktime && kseq++ % ktime == 0;
       if ((st = launch_templates(g0, gr.buf, t0, nt, one ? &it : nullptr, t0 - it.t0, Ng, o,
-                                 timed ? kquad(nt) : nullptr)))
+                                 timed ? kquad(nt) : nullptr, thr_fuse))) {
+        thr_prev.nt = 0;
         return st;
+      }
     }
+    if ((st = flush_thr())) return st;
     HIPCHK(hipMemcpyAsync(hn_pin.p + gr.buf, ncand.p + gr.buf, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(hc_pin.p + (size_t)gr.buf * kCandChunk, cand.p + (size_t)gr.buf * kCandCap,
                           sizeof(Cand) * kCandChunk, hipMemcpyDeviceToHost, stream));
