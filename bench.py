@@ -255,7 +255,10 @@ def bench_templates(args, name):
     nspin = rng.spndr[1] - rng.spndr[0] + 1
     per_step = args.sky_per_step or min(len(sky), {"network": 16, "plain": 4, "precision": 64}[name])
     if args.steps is None:
-        args.steps = max(1, len(sky) // per_step) if name != "plain" else 16
+        # plain: 4 sky points are one step of 0.3 ms; 256 repetitions time ~70 ms, so one-time
+        # host-side costs of the runtime (a stall of a few ms after ~1000 queued operations was
+        # seen) do not dominate the timed region
+        args.steps = max(1, len(sky) // per_step) if name != "plain" else 256
     s = seg.sett
     print(f"workload {name}: N={s['N']} nfft={s['nfft']} fftpad={s['fftpad']} L={s['nfftf']} "
           f"sky={len(sky)} spindowns={nspin}", file=sys.stderr, flush=True)
