@@ -241,7 +241,9 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
     T s0 = 0, s1 = 0, m0 = 0, m1 = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-      const int j = j0 + ((i + tid) & 15);  // rotated start: lanes spread over the LDS banks
+      // rotated start (by tid and tid / 16, so lanes 16 apart differ too): lanes spread over the
+      // LDS banks
+      const int j = j0 + ((i + tid + (tid >> 4)) & 15);
       const T f = Fs[j];
       if (j < jsplit) { s0 += f; m0 = f > m0 ? f : m0; }
       else { s1 += f; m1 = f > m1 ? f : m1; }
