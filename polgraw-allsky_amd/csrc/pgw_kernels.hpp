@@ -2,7 +2,7 @@
 //
 // Sky-point stage (once per sky point, amortised over its spindowns; each launch covers all the
 // sky points of a group):
-//   k_sky_prologue    modvir + shft + first demodulation    (jobcore.c:270-313, settings.c:281-325)
+//   k_sky_prologue    modvir + shft, _tmp1 + first demodulation (jobcore.c:270-313, settings.c:281-325)
 //   k_fft_col2/k_fft_row2 + loaders  resampling FFTs         (jobcore.c:319-362)
 //   k_untranspose     T-layout -> natural order, times interpftpad/Ninterp
 //   k_spline          natural cubic spline (auxi.c:55-81) and the noise-weighted sums aa, bb
@@ -38,7 +38,7 @@ struct SkyParams {
 };
 
 // One sky point of a batched sky stage: its coefficients, its segment's inputs and the sky slot
-// that receives its resampled series xm, shifts shft and aa/bb.  Sky points of one batch share
+// that receives its resampled series xm, _tmp1 and aa/bb.  Sky points of one batch share
 // nfft, Ninterp and the detector count; N may differ.
 struct SkyItem {
   SkyParams p;
@@ -47,7 +47,7 @@ struct SkyItem {
   const double* cosmodf;  // [N]
   const double* sinmodf;  // [N]
   void* xm;               // slot series cplx<T>[nifo][{a,b}][N]
-  double* shft;           // slot shifts [nifo][N]
+  double* tmp1;           // slot _tmp1 = i^2 + 2 i shft [nifo][N]
   double* aabb;           // slot aa, bb
   int nspl, pad;          // spline knots interpftpad N
 };
@@ -66,7 +66,7 @@ __device__ __forceinline__ double lin2_nc(double a0, double b0, double a1, doubl
 // ------------------------------------------------------------- sky stage --
 
 // grid (ceil(nfft/NT), nifo, sky points).  For sky point j: R[j][2d][i] = xDat aa e^{-i phase},
-// R[j][2d+1][i] = xDat bb e^{-i phase} for i < N, zero for N <= i < nfft; shft (slot), shftf
+// R[j][2d+1][i] = xDat bb e^{-i phase} for i < N, zero for N <= i < nfft; _tmp1 (slot), shftf
 // [j][d][Nstride]; block partial sums of aa^2, bb^2 at paa/pbb [j][d][block].
 template <typename T>
 __global__ __launch_bounds__(NT) void k_sky_prologue(const SkyItem* __restrict__ items, cplx<T>* __restrict__ R,
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(NT) void k_sky_prologue(const SkyItem* __restrict__
   const double* __restrict__ DetSSB = it.DetSSB;
   const double* __restrict__ cosmodf = it.cosmodf;
   const double* __restrict__ sinmodf = it.sinmodf;
-  double* __restrict__ shft = it.shft;
+  double* __restrict__ tmp1 = it.tmp1;
   shftf += (size_t)j * p.nifo * Nstride;
   R += (size_t)j * 2 * p.nifo * p.nfft;
   paa += (size_t)j * p.nifo * gridDim.x;
@@ -103,7 +103,8 @@ __global__ __launch_bounds__(NT) void k_sky_prologue(const SkyItem* __restrict__
     // shft = n.DetSSB and the phase in the reference's operation order, no contraction:
     // shftf[0] must come out exactly 0 (the spline returns 0 for x < 0)
     const double sh = dot3_nc(q.nS0, D[0], q.nS1, D[1], q.nS2, D[2]);
-    shft[(size_t)d * p.N + i] = sh;
+    // _tmp1 = i^2 + 2 i shft, once per sky point as the reference does (jobcore.c:293), uncontracted
+    tmp1[(size_t)d * p.N + i] = lin2_nc((double)i, (double)i, (double)(2 * i), sh);
     shftf[(size_t)d * Nstride + i] = sh - q.shft1;
     double sp, cp;
     sincos(lin2_nc(p.het0, (double)i, p.oms, sh), &sp, &cp);
@@ -370,7 +371,7 @@ __global__ __launch_bounds__(NT) void k_spline_eval(const cplx<T>* __restrict__ 
 // noise-weighted aa/bb, and its frequency offset het1 and spindown sgnlt1 computed on the host
 // with the reference's expressions (jobcore.c:456,466-467).  Templates of one launch may come
 // from different sky points and segments that share the F-statistic FFT length.
-// One template: its sky slot (whose resampled data xm, barycentric shifts shft and aa/bb sums the
+// One template: its sky slot (whose resampled data xm, _tmp1 and aa/bb sums the
 // kernels find at slot * stride from the slot bases), spindown phase coefficients and data length.
 struct TmplDesc {
   double het1, sgnlt1;
@@ -378,7 +379,7 @@ struct TmplDesc {
 };
 static_assert(sizeof(TmplDesc) == 32, "descriptor layout");
 
-// Sky-slot bases: xm cplx<T>[slot][nifo][{a,b}][N], shft double[slot][nifo][N], aabb double[slot][2].
+// Sky-slot bases: xm cplx<T>[slot][nifo][{a,b}][N], tmp1 double[slot][nifo][N], aabb double[slot][2].
 // het1 and sgnlt[1] of spindown ss (jobcore.c:456,466-467), uncontracted.
 struct SpinGrid {
   double M0, M4, M5, M9, M13;
@@ -397,9 +398,9 @@ inline void spin_coeffs(const SpinGrid& q, int ss, double& het1, double& sgnlt1)
 // templates), else in the descriptors (a load that no data load waits for).
 struct SlotBases {
   const void* xm;
-  const double* shft;
+  const double* tmp1;
   const double* aabb;
-  unsigned long long xm_stride, shft_stride;  // elements per slot
+  unsigned long long xm_stride, tmp1_stride;  // elements per slot
   int nifo, uslot, uN, tab;
 };
 

@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
   }
   const int nifo = sb.nifo;
   const T2* __restrict__ xm = static_cast<const T2*>(sb.xm) + (size_t)slot * sb.xm_stride;
-  const double* __restrict__ shft = sb.shft + (size_t)slot * sb.shft_stride;
+  const double* __restrict__ tmp1s = sb.tmp1 + (size_t)slot * sb.tmp1_stride;
   const int c = threadIdx.x % S::C, g = threadIdx.x / S::C;
   const int n1 = blockIdx.x * S::C + c;
   const int L1 = p.L1;
@@ -86,9 +86,8 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
       const double dn = (double)n;
 #pragma unroll 2
       for (int d = 0; d < nifo; d++) {  // two detectors' loads in flight together
-        const double sh = ld_global(shft, (size_t)d * N + n);
-        // _tmp1 = i^2 + 2 i shft, phase = het1 i + sgnlt1 _tmp1 (jobcore.c:294,530), uncontracted
-        const double tmp1 = lin2_nc(dn, dn, (double)(2 * n), sh);
+        // phase = het1 i + sgnlt1 _tmp1 (jobcore.c:530), uncontracted; _tmp1 from the sky slot
+        const double tmp1 = ld_global(tmp1s, (size_t)d * N + n);
         const T2 ex = PhaseRot<T>::eval(lin2_nc(het1, dn, sgnlt1, tmp1));
         xa = cadd(xa, cmul(ld_global(xm, (size_t)(2 * d) * N + n), ex));
         xb = cadd(xb, cmul(ld_global(xm, (size_t)(2 * d + 1) * N + n), ex));

@@ -327,8 +327,8 @@ template <typename T> struct Engine final : pgw_handle {
   PinBuf<SkyItem> sky_pin;  // 2 x kSlots
   // sky slots
   DevBuf<T2> xm_slots;
-  DevBuf<double> shft_slots, aabb_slots;
-  size_t slot_xm = 0, slot_shft = 0;
+  DevBuf<double> tmp1_slots, aabb_slots;
+  size_t slot_xm = 0, slot_tmp1 = 0;
   // template stage
   DevBuf<T2> Z;
   DevBuf<T> Ft, psum, pmax;
@@ -527,7 +527,7 @@ template <typename T> struct Engine final : pgw_handle {
     // W_{2^20}^m two-level tables: hi[h] = W^{1024 h}, lo[l] = W^l
     if ((st = upload_tw<T>(hi, 1 << 20, 10, 1024)) || (st = upload_tw<T>(lo, 1 << 20, 0, 1024))) return st;
     slot_xm = (size_t)2 * nifo * Nmax;
-    slot_shft = (size_t)nifo * Nmax;
+    slot_tmp1 = (size_t)nifo * Nmax;
     // sky-stage scratch for a group of up to kSlots sky points
     const size_t K = kSlots;
     if ((st = R.alloc(K * 2 * nifo * nfftmax)) || (st = Zf.alloc(K * 2 * nifo * nfftmax)) ||
@@ -536,7 +536,7 @@ template <typename T> struct Engine final : pgw_handle {
         (st = shftf.alloc(K * nifo * Nmax)) || (st = paa.alloc(K * nifo * npromax)) ||
         (st = pbb.alloc(K * nifo * npromax)) || (st = spl.alloc(1)) || (st = sky_dev.alloc(2 * K)) ||
         (st = sky_pin.alloc(2 * K)) ||
-        (st = xm_slots.alloc((size_t)kSlots * slot_xm)) || (st = shft_slots.alloc((size_t)kSlots * slot_shft)) ||
+        (st = xm_slots.alloc((size_t)kSlots * slot_xm)) || (st = tmp1_slots.alloc((size_t)kSlots * slot_tmp1)) ||
         (st = aabb_slots.alloc((size_t)kSlots * 2)) || (st = desc.alloc((size_t)2 * kDescCap)) ||
         (st = desc_pin.alloc((size_t)2 * kDescCap)) || (st = cand.alloc((size_t)2 * kCandCap)) ||
         (st = ncand.alloc(2)) || (st = hc_pin.alloc((size_t)2 * kCandChunk)) || (st = hn_pin.alloc(2)) ||
@@ -562,7 +562,7 @@ template <typename T> struct Engine final : pgw_handle {
   }
 
   T2* slot_xm_ptr(int slot) { return xm_slots.p + (size_t)slot * slot_xm; }
-  double* slot_shft_ptr(int slot) { return shft_slots.p + (size_t)slot * slot_shft; }
+  double* slot_tmp1_ptr(int slot) { return tmp1_slots.p + (size_t)slot * slot_tmp1; }
   double* slot_aabb_ptr(int slot) { return aabb_slots.p + (size_t)slot * 2; }
 
   // Sky coordinates of a grid point (jobcore.c:243-264); 0 when outside the disk.
@@ -613,7 +613,7 @@ template <typename T> struct Engine final : pgw_handle {
     it.cosmodf = g.cosmodf.p;
     it.sinmodf = g.sinmodf.p;
     it.xm = slot_xm_ptr(slot);
-    it.shft = slot_shft_ptr(slot);
+    it.tmp1 = slot_tmp1_ptr(slot);
     it.aabb = slot_aabb_ptr(slot);
     it.nspl = g.nspl;
     return 1;
@@ -725,7 +725,7 @@ template <typename T> struct Engine final : pgw_handle {
                        hipEvent_t* ev = nullptr, bool fold = false) {
     const TmplDesc* d = desc.p + (size_t)buf * kDescCap + t0;
     if (ev) HIPCHK(hipEventRecord(ev[0], stream));
-    SlotBases sb{xm_slots.p, shft_slots.p, aabb_slots.p, slot_xm, slot_shft, nifo, -1, 0, 0};
+    SlotBases sb{xm_slots.p, tmp1_slots.p, aabb_slots.p, slot_xm, slot_tmp1, nifo, -1, 0, 0};
     SpinTab tab;
     if (uni) {
       sb.uslot = uni->slot;
