@@ -193,6 +193,12 @@ def roofline(kt, segw, prec_bytes, workload, tuned=None):
                 template_stage_GBps=round((b_col + b_row) * n / (stage_ms * 1e-3) / 1e9, 1))
 
 
+def threads_env():
+    """The OpenMP settings of the CPU baseline's run (thread count and pinning; unset = the
+    runtime's defaults, threads unpinned)."""
+    return {k: os.environ.get(k, "unset") for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -226,7 +232,8 @@ def cpu_baseline(seg, rng, opts, sky=None, budget_s=10.0):
         nsky += 1
     return dict(value=round(fn / dt, 2), unit="templates/s", cores=threads, kind="port",
                 sample=f"{nsky} sky-point searches x all spindowns ({fn} templates) of the same workload (FP64 oracle, "
-                       f"own Stockham FFT, OpenMP over spindowns; {dt:.1f} s wall)", cpu_model=cpu_model())
+                       f"own Stockham FFT, OpenMP over spindowns; {dt:.1f} s wall)", cpu_model=cpu_model(),
+                threads_env=threads_env(), host_cpus=len(os.sched_getaffinity(0)))
 
 
 def tune_handle(args, h, segi, seg, opts):
@@ -439,7 +446,8 @@ def bench_differentiation(args, npts=1024):
             reps += 1
         cpu = dict(value=round(ns / tcd, 2), unit="points/s", cores=threads, kind="port",
                    sample=f"{reps} x the {len(pts)} points with gradient (FP64 oracle Fstatnet, OpenMP over points; "
-                          f"{tcd:.1f} s wall)", cpu_model=cpu_model(),
+                          f"{tcd:.1f} s wall)", cpu_model=cpu_model(), threads_env=threads_env(),
+                   host_cpus=len(os.sched_getaffinity(0)),
                    max_rel_dF_vs_device=float(np.max(np.abs(out_o[:, 5] - out[:, 5]) / np.abs(out_o[:, 5]))))
     nifo, N = len(seg.dets), seg.sett["N"]
     fps = measured_flops("k_fstatnet_part")
