@@ -221,9 +221,8 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
       const bool ok = (k >= p.nmin && k < p.nmax);
       if (ok) Frow[k1] = F[m < MO ? m : 0];
       Fs[k1] = ok ? F[m < MO ? m : 0] : (T)0;
-    } else {
-      Fs[k1] = (T)0;
     }
+    // (HALF, m >= MO: the upper half of Fs, k1 >= P1/2, holds no bin of [nmin, nmax) and is not read)
   }
   __syncthreads();
   // FStat block partials of this row: block b holds bins k in [nmin + 4096 b, nmin + 4096 (b+1))
@@ -231,7 +230,9 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
   // consecutive bins j = 16 tid + i, which fall into at most two blocks b0 <= b1 (bins outside
   // [nmin, nmax) are 0 and clamped into the first or last block), ...
   T* rs = Fs + P1;   // [2][NT] sums, then [2][NT] maxima
-  {
+  // HALF: threads whose 16 bins lie in the upper half (tid >= NT/2) have nothing to sum, and no
+  // block's bin range reaches their entries of rs (its bins end below nmax <= L/2)
+  if (!HALF || tid < NT / 2) {
     const int j0 = 16 * tid;
     const int nb = p.nparts - 1;
     const int b0 = min(max((p.L2 * j0 + k2 - p.nmin) >> 12, 0), nb);
@@ -325,15 +326,16 @@ __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const doub
       const bool ok = (k >= p.nmin && k < p.nmax);
       if (ok) Frow[k1] = F[m < MO ? m : 0];
       Fs[k1] = ok ? F[m < MO ? m : 0] : (T)0;
-    } else {
-      Fs[k1] = (T)0;
     }
+    // (HALF, m >= MO: the upper half of Fs, k1 >= P1/2, holds no bin of [nmin, nmax) and is not read)
   }
   __syncthreads();
   // FStat block partials of this row as in k_spin_row2, with runs of 8 consecutive bins per thread
   // (a block spans 4096 / L2 >= 16 consecutive k1, so a run falls into at most two blocks).
   T* rs = Fs + P1;   // [2][NT] sums, then [2][NT] maxima
-  {
+  // HALF: threads whose 8 bins lie in the upper half (tid >= NT/2) have nothing to sum, and no
+  // block's bin range reaches their entries of rs (its bins end below nmax <= L/2)
+  if (!HALF || tid < NT / 2) {
     const int j0 = 8 * tid;
     const int nb = p.nparts - 1;
     const int b0 = min(max((p.L2 * j0 + k2 - p.nmin) >> 12, 0), nb);
