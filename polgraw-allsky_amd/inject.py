@@ -19,7 +19,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .synth import C_EPSMA, ast2lin, modvir
+from .synth import C_EPSMA, amplitudes, ast2lin, lin2ast, modvir
 
 
 @dataclass
@@ -47,6 +47,30 @@ def write_signal(path, sig: Signal) -> None:
     with open(path, "w") as f:
         f.write("%s %e\n%d\n%d\n" % (sig.kind, sig.value, sig.gsize, sig.reffr))
         f.write("".join("%.16e\n" % v for v in sig.sgnlo))
+
+
+def random_signal(rng, sett: dict, kind: str = "snr", value: float = 10.0, gsize: int = 2,
+                  reffr: int = 0) -> Signal:
+    """A random signal as sigen draws it (sigen.c:270-367), without its known-line and near-pole
+    options: spindown uniform in [-Smax, 0]; hemisphere 1 or 2; sky position uniform over the
+    disk of linear sky coordinates (be1^2 + be2^2 <= 1) mapped by lin2ast; initial phase and
+    polarisation angle uniform in [0, 2 pi), cos(inclination) uniform in [-1, 1].  The frequency
+    is uniform over the searched band [2 pi nmin, 2 pi nmax) / (fftpad nfft) at frame reffr (the
+    reference's draw, pi U (nmax - nmin) / nfft + nmin / nfft, is near that band for fftpad 2).
+    rng: a numpy Generator."""
+    L = sett["fftpad"] * sett["nfft"]
+    fdot = -sett["Smax"] * rng.uniform()
+    f = 2.0 * math.pi * (sett["nmin"] + rng.uniform() * (sett["nmax"] - sett["nmin"])) / L
+    pm = 1 + int(rng.integers(0, 2))
+    while True:
+        be1, be2 = 2.0 * rng.uniform() - 1.0, 2.0 * rng.uniform() - 1.0
+        if be1 * be1 + be2 * be2 <= 1.0:
+            break
+    delta, alpha = lin2ast(be1, be2, pm)
+    ph_o, psik = 2.0 * math.pi * rng.uniform(), 2.0 * math.pi * rng.uniform()
+    amp = amplitudes(ph_o, psik, 2.0 * rng.uniform() - 1.0)
+    return Signal(kind=kind, value=value, gsize=gsize, reffr=reffr,
+                  sgnlo=np.array([f, fdot, delta, alpha, *amp]))
 
 
 def _cround(x: float) -> int:

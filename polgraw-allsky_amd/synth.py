@@ -127,6 +127,26 @@ def ast2lin(alfa: float, delta: float, epsm: float):
     return pm, (be0, be1)
 
 
+def amplitudes(ph_o: float, psik: float, hoc: float) -> np.ndarray:
+    """The four amplitudes of a signal of initial phase ph_o, polarisation angle psik and
+    cos(inclination) hoc (sigen.c:348-357; Phys. Rev. D 82, 022005, Eqs. 2.13a-d)."""
+    hop = (1.0 + hoc * hoc) / 2.0
+    c2p, s2p, cph, sph = math.cos(2 * psik), math.sin(2 * psik), math.cos(ph_o), math.sin(ph_o)
+    return np.array([c2p * hop * cph - s2p * hoc * sph, s2p * hop * cph + c2p * hoc * sph,
+                     -c2p * hop * sph - s2p * hoc * cph, -s2p * hop * sph + c2p * hoc * cph])
+
+
+def lin2ast(be1: float, be2: float, pm: int, epsm: float = C_EPSMA):
+    """(delta, alpha) of linear sky coordinates in hemisphere pm (auxi.c:15-24, with the
+    declination and right ascension of jobcore.c:261-262)."""
+    sepsm, cepsm = math.sin(epsm), math.cos(epsm)
+    sindel = be1 * sepsm - (2 * pm - 3) * math.sqrt(1.0 - be1 * be1 - be2 * be2) * cepsm
+    cosdel = math.sqrt(1.0 - sindel * sindel)
+    sinal = (be1 - sepsm * sindel) / (cepsm * cosdel)
+    cosal = be2 / cosdel
+    return math.asin(sindel), math.fmod(math.atan2(sinal, cosal) + 2 * math.pi, 2 * math.pi)
+
+
 @dataclass
 class Segment:
     sett: dict
@@ -208,13 +228,7 @@ def make_segment(N: int = 86400, dets=("H1",), sigmas=(1.0,), dt: float = 2.0, f
             if ok:
                 break
         ph_o, psik = rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi)
-        hoc = rng.uniform(-1.0, 1.0)
-        hop = (1.0 + hoc * hoc) / 2.0
-        amp = np.array([
-            math.cos(2 * psik) * hop * math.cos(ph_o) - math.sin(2 * psik) * hoc * math.sin(ph_o),
-            math.sin(2 * psik) * hop * math.cos(ph_o) + math.cos(2 * psik) * hoc * math.sin(ph_o),
-            -math.cos(2 * psik) * hop * math.sin(ph_o) - math.sin(2 * psik) * hoc * math.cos(ph_o),
-            -math.sin(2 * psik) * hop * math.sin(ph_o) + math.cos(2 * psik) * hoc * math.cos(ph_o)])
+        amp = amplitudes(ph_o, psik, rng.uniform(-1.0, 1.0))
         cof = sett["oms"] + f
         sa, ca, sd, cd = math.sin(alpha), math.cos(alpha), math.sin(delta), math.cos(delta)
         nS = np.array([ca * cd, sa * cd, sd])

@@ -206,3 +206,88 @@ def test_device_search_recovers_injection(injected, orc, abi, tmp_path):
     assert np.array_equal(got[:, :4], ref[:, :4])
     Fg, Fr = got[:, 4] ** 2 / 2 + 2, ref[:, 4] ** 2 / 2 + 2
     assert np.max(np.abs(Fg - Fr) / Fr) <= 1e-9
+
+
+def test_random_signal_draws(frame):
+    """sigen's draws (sigen.c:270-357): spindown in [-Smax, 0], frequency in the searched band, sky
+    position on the grid disk in the drawn hemisphere, and amplitudes with the invariants of their
+    parametrisation: a1^2 + a2^2 + a3^2 + a4^2 = hop^2 + hoc^2 and a1 a4 - a2 a3 = hop hoc
+    (hop = (1 + hoc^2) / 2) for one cos(inclination) hoc in [-1, 1]."""
+    s = frame.sett
+    L = s["fftpad"] * s["nfft"]
+    rng = np.random.default_rng(99)
+    pms = set()
+    for _ in range(200):
+        sig = inject.random_signal(rng, s, kind="amp", value=0.5, gsize=2, reffr=4)
+        f, fdot, delta, alpha = sig.sgnlo[:4]
+        assert (sig.kind, sig.value, sig.gsize, sig.reffr) == ("amp", 0.5, 2, 4)
+        assert -s["Smax"] <= fdot <= 0
+        assert 2 * math.pi * s["nmin"] / L <= f < 2 * math.pi * s["nmax"] / L
+        pm, be = synth.ast2lin(alpha, delta, synth.C_EPSMA)
+        assert be[0] ** 2 + be[1] ** 2 <= 1 + 1e-12
+        pms.add(pm)
+        a = sig.sgnlo[4:]
+        q = float(np.sum(a * a))
+        d = a[0] * a[3] - a[1] * a[2]
+        # d = hop hoc fixes hoc (h^3 + h = 2 d has one real root); q must then be hop^2 + hoc^2
+        r = np.roots([1.0, 0.0, 1.0, -2.0 * d])
+        h = float(r[np.argmin(np.abs(r.imag))].real)
+        assert abs(h) <= 1 + 1e-9
+        assert q == pytest.approx(((1 + h * h) / 2) ** 2 + h * h, abs=1e-9)
+    assert pms == {1, 2}
+
+
+def test_amplitude_invariants():
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        ph_o, psik, hoc = rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi), rng.uniform(-1, 1)
+        a = synth.amplitudes(ph_o, psik, hoc)
+        hop = (1 + hoc * hoc) / 2
+        assert float(np.sum(a * a)) == pytest.approx(hop * hop + hoc * hoc, abs=1e-14)
+        assert a[0] * a[3] - a[1] * a[2] == pytest.approx(hop * hoc, abs=1e-14)
+
+
+@pytest.mark.gpu
+def test_injection_recovery_rate(abi):
+    """The reference's sensitivity pipeline in miniature (sigen's random signals, the -addsig search
+    of the +-gsize box, detection): 12 random signals of snr 60 (about 20 in this frame's data,
+    whose noise variances are 3.4 and 7.4, with gaps) added one at a time to the reference's test
+    frame and searched on the device in the FP32 form.  The templates of the box lie within
+    gsize + 1/2 grid steps of the signal in spindown and sky, which moves the frequency of the
+    best template by at most (|M4| + |M8| + |M12|) (gsize + 1/2) (the grid's frequency row).
+    Detection: a record in that window with at least 0.7 of the signal-to-noise ratio in the data,
+    where the same box of the frame without the signal has no record (this frame has strong
+    non-Gaussian features elsewhere in the band).  All 12 are detected."""
+    import copy
+    from polgraw_allsky_amd import pgw
+    base = io.load_frame(FIX, 1, 666)
+    M = base.M
+    gsize = 1
+    win = (abs(M[4]) + abs(M[8]) + abs(M[12])) * (gsize + 0.5)
+    opts = abi.make_opts(trl=20.0)
+    h_noise = pgw.Handle(base, abi.PREC_FP32)
+    rng = np.random.default_rng(2024)
+    found = 0
+    for k in range(12):
+        sig = inject.random_signal(rng, base.sett, kind="snr", value=60.0, gsize=gsize, reffr=1)
+        fr = copy.deepcopy(base)
+        box = inject.add_signal(fr, sig, ident=1)
+        rho = math.sqrt(sum(float(np.sum((x - y) ** 2)) / s for x, y, s in zip(fr.xDat, base.xDat, base.sig2)))
+        sky = [(box["pmr"][0], mm, nn) for mm in range(box["mr"][0], box["mr"][1] + 1)
+               for nn in range(box["nr"][0], box["nr"][1] + 1)]
+        rng_box = abi.make_range(fr)
+        h = pgw.Handle(fr, abi.PREC_FP32)
+        try:
+            rec, fn = h.job_batch(sky, rng_box, opts)
+        finally:
+            h.close()
+        ref, _ = h_noise.job_batch(sky, rng_box, opts)
+        near = lambda r: r[np.abs(r[:, 0] - sig.sgnlo[0]) <= win] if len(r) else r  # noqa: E731
+        w, wn = near(rec), near(ref)
+        best = float(w[:, 4].max()) if len(w) else 0.0
+        print(k, "f %.3f rho %.1f templates %d records %d in window %d best %.1f noise in window %d"
+              % (sig.sgnlo[0], rho, fn, len(rec), len(w), best, len(wn)), flush=True)
+        assert len(wn) == 0
+        found += best >= 0.7 * rho
+    h_noise.close()
+    assert found == 12
