@@ -3,9 +3,10 @@
 //
 // A handle holds one or more segments (detector data resident in HBM) and one HIP stream.  Work
 // is a list of (segment, pm, mm, nn) sky points processed in groups: the sky-point stage of each
-// point of a group fills one of kSlots slots (resampled series, shifts, aa/bb), then template
+// point of a group fills one of kSlots slots (resampled series, _tmp1, aa/bb), then template
 // launches of up to TB templates run over all spindowns of all points of the group, described by
-// per-template descriptors in device memory.  Points of one group share the F-statistic FFT
+// per-template descriptors in device memory; each launch's threshold runs in the first
+// workgroups of the next launch's column pass.  Points of one group share the F-statistic FFT
 // length and bin range, so segments of uneven N with the same nfft batch together.  The host does
 // only the scalar bookkeeping of job_core (search/network/src-cpu/jobcore.c:189-785): sky
 // coordinates, spindown range, per-template het1/sgnlt1 and the candidate records; it turns the
@@ -356,7 +357,8 @@ template <typename T> struct Engine final : pgw_handle {
   bool thr_fuse = true;
   ThrFuse<T> thr_prev{};
 
-  // kernel timing over searches: 4 events per template launch (column, row, threshold)
+  // kernel timing over searches: 4 events per template launch (column, row, threshold; with the
+  // threshold folded into the next column pass the last interval is empty)
   int ktime = 0;         // record every ktime-th template launch (0: off)
   long kseq = 0;         // template launches since recording was switched on
   std::vector<hipEvent_t> kpool;
