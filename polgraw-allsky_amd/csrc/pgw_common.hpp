@@ -91,17 +91,35 @@ template <int DIR, typename T2> __device__ __forceinline__ void dft8(T2* v) {
   v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
 }
 
+// multiply by exp(DIR 2 pi i K / 16), K = 0..15 (K >= 8: the negated rotation by K - 8; the
+// negation folds into the following additions as a source modifier)
+template <int DIR, int K, typename T2> __device__ __forceinline__ T2 rot16n(T2 a) {
+  if constexpr (K == 0) return a;
+  else if constexpr (K == 4) return mul_w4<DIR>(a);
+  else if constexpr (K < 8) return rot16<DIR, K>(a);
+  else {
+    const T2 r = rot16n<DIR, K - 8>(a);
+    return mk<T2>(-r.x, -r.y);
+  }
+}
+
+// 16 = 4 x 4: r = q + 4 p, k = k1 + 4 k2; four 4-point DFTs over p, twiddles W16^(q k1), four
+// 4-point DFTs over q (four general and four pi/4 rotations: 8 instructions fewer than the
+// radix-2 split into two 8-point DFTs)
 template <int DIR, typename T2> __device__ __forceinline__ void dft16(T2* v) {
-  T2 e[8], o[8];
+  T2 a0[4], a1[4], a2[4], a3[4];
 #pragma unroll
-  for (int i = 0; i < 8; i++) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
-  dft8<DIR>(e);
-  dft8<DIR>(o);
-  o[1] = rot16<DIR, 1>(o[1]); o[2] = rot16<DIR, 2>(o[2]); o[3] = rot16<DIR, 3>(o[3]);
-  o[4] = mul_w4<DIR>(o[4]);   o[5] = rot16<DIR, 5>(o[5]); o[6] = rot16<DIR, 6>(o[6]);
-  o[7] = rot16<DIR, 7>(o[7]);
+  for (int p = 0; p < 4; p++) { a0[p] = v[4 * p]; a1[p] = v[1 + 4 * p]; a2[p] = v[2 + 4 * p]; a3[p] = v[3 + 4 * p]; }
+  dft4<DIR>(a0); dft4<DIR>(a1); dft4<DIR>(a2); dft4<DIR>(a3);
+  a1[1] = rot16n<DIR, 1>(a1[1]); a1[2] = rot16n<DIR, 2>(a1[2]); a1[3] = rot16n<DIR, 3>(a1[3]);
+  a2[1] = rot16n<DIR, 2>(a2[1]); a2[2] = rot16n<DIR, 4>(a2[2]); a2[3] = rot16n<DIR, 6>(a2[3]);
+  a3[1] = rot16n<DIR, 3>(a3[1]); a3[2] = rot16n<DIR, 6>(a3[2]); a3[3] = rot16n<DIR, 9>(a3[3]);
 #pragma unroll
-  for (int k = 0; k < 8; k++) { v[k] = cadd(e[k], o[k]); v[k + 8] = csub(e[k], o[k]); }
+  for (int k1 = 0; k1 < 4; k1++) {
+    T2 w0 = a0[k1], w1 = a1[k1], w2 = a2[k1], w3 = a3[k1];
+    dft4<DIR>(w0, w1, w2, w3);
+    v[k1] = w0; v[k1 + 4] = w1; v[k1 + 8] = w2; v[k1 + 12] = w3;
+  }
 }
 
 // a * exp(DIR 2 pi i k / R), R in {4, 8, 16}, 0 < k < R/2 (k a compile-time constant after
@@ -128,9 +146,9 @@ template <int R, int DIR, typename T2> __device__ __forceinline__ void dftR(T2* 
   else static_assert(R == 2, "unsupported radix");
 }
 
-// R-point DFT of v[0..R-1] whose inputs v[r], r >= NZ, are zero (input pruning): the same
-// even/odd split as dft4/dft8/dft16, with sub-transforms of one nonzero input reduced to copies
-// and of none to zeros, so the result equals dftR's on such inputs.
+// R-point DFT of v[0..R-1] whose inputs v[r], r >= NZ, are zero (input pruning): an even/odd
+// split (the one of dft4 and dft8), with sub-transforms of one nonzero input reduced to copies and
+// of none to zeros; the result equals dftR's on such inputs to rounding (bit for bit for R <= 8).
 template <int R, int NZ, int DIR, typename T2> __device__ __forceinline__ void dftP(T2* v) {
   if constexpr (NZ >= R) {
     dftR<R, DIR>(v);
