@@ -21,7 +21,6 @@ namespace pgw {
 
 constexpr int NT = 256;          // threads per block everywhere
 constexpr int PGW_NAV_DEV = 4096; // FStat block length (NAVFSTAT, auxi.h:13)
-constexpr int SPL_K = 16;        // spline chunk per thread
 constexpr int SPL_W = 32;        // spline warm-up (|coefficient| ~ 0.268: 0.268^32 ~ 5e-19)
 constexpr int SPL_NTAB = 64;     // spline recursion coefficients tabulated below this index
 
@@ -210,8 +209,8 @@ __global__ __launch_bounds__(NT) void k_untranspose(const cplx<T>* __restrict__ 
 }
 
 // ----------------------------------------------------------------- spline --
-// Natural cubic spline on n unit-spaced knots (auxi.c:55-81), chunked: each thread owns SPL_K
-// outputs and starts its recursion SPL_W samples early from a zero state (exact when the
+// Natural cubic spline on n unit-spaced knots (auxi.c:55-81), chunked: each thread owns K
+// outputs (4, 8 or 16) and starts its recursion SPL_W samples early from a zero state (exact when the
 // chunk is within SPL_W of the true start).  invp_i, c_i = -0.5 invp_i tabulated for i < SPL_NTAB.
 struct SplineTab {
   double invp[SPL_NTAB];

@@ -347,7 +347,7 @@ template <typename T> struct Engine final : pgw_handle {
   int group = kSlots;  // sky points per group
   int row_na = 1;
   bool col_seq = true;  // column pass: the two spectra share one LDS buffer in turn
-  int spl_k = 8;  // spline outputs per thread (4, 8 or 16)
+  int spl_k = 4;  // spline outputs per thread (4, 8 or 16)
   bool row_half = true;  // row pass forms only the outputs that can fall in [nmin, nmax)
   // row pass with 8 elements per thread (1 with the second row prefetched, 2 without): the FP64
   // default (two workgroups per CU instead of one); FP32 keeps 16 elements per thread
@@ -508,7 +508,7 @@ template <typename T> struct Engine final : pgw_handle {
     int st;
     for (int k = 0; k < nseg; k++)
       if ((st = add_segment(&s[k], ifo + (size_t)k * ndet))) return st;
-    // kernel variants; PGW_TUNE="row_na=2,col_seq=1,tb=64,group=4,spl_k=8,half=1" overrides (used when
+    // kernel variants; PGW_TUNE="row_na=2,col_seq=1,tb=64,group=4,spl_k=4,half=1" overrides (used when
     // measuring variants)
     if (const char* tune = getenv("PGW_TUNE")) {
       int v;

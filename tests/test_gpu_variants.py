@@ -13,7 +13,7 @@ from cases import injected_sky, segment
 pytestmark = pytest.mark.gpu
 
 VARIANTS = ["", "row8=0", "row8=1", "row8=2", "half=0", "row8=0,half=0", "row8=2,half=0", "row_na=2,row8=0",
-            "col_seq=1", "spl_k=4", "spl_k=16", "tb=32,group=3", "thr_fuse=0", "tb=8"]
+            "col_seq=1", "spl_k=8", "spl_k=16", "tb=32,group=3", "thr_fuse=0", "tb=8"]
 CASES = ["plain-1det-32k", "net-2det-32k", "net-2det-86k"]
 
 
@@ -67,7 +67,7 @@ def test_variant_fp64(pgw, abi, oracle_runs, name, tune):
     assert np.max(np.abs(Fg - Fr) / Fr) <= 1e-9
 
 
-@pytest.mark.parametrize("tune", ["", "row8=1", "row8=2", "half=0", "row_na=2", "spl_k=4", "thr_fuse=0"])
+@pytest.mark.parametrize("tune", ["", "row8=1", "row8=2", "half=0", "row_na=2", "spl_k=8", "thr_fuse=0"])
 @pytest.mark.parametrize("name", CASES)
 def test_variant_fp32(pgw, abi, oracle_runs, name, tune):
     sky, ref, fr = oracle_runs[name]
