@@ -32,6 +32,39 @@ template <typename T2> __device__ __forceinline__ T2 ld_global(const T2* p, size
   const V v = ((gptr<V>)p)[i];
   return mk<T2>(v.x, v.y);
 }
+// Raw buffer over [base, base + bytes) with a workgroup-uniform base: an access's address is the
+// base + a 32-bit per-lane byte offset + a uniform byte offset held in a scalar register, so no
+// 64-bit vector address arithmetic is spent per access (a plain pointer access costs one 64-bit
+// vector add each).  Resource word 3 = 0x00020000: the gfx9-family raw buffer (32-bit data
+// format, no swizzle, no stride); accesses at or beyond `bytes` are range-checked (loads read 0,
+// stores are dropped).  Stores hold at most 64 bits of data: a buffer store of more data with a
+// scalar offset meets the store-data hazard (a VALU write to its data registers right after it,
+// for which the compiler places no wait state), which corrupted FP64 spectra.
+struct BufRes {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ BufRes(const void* base, uint32_t bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000)) {}
+  // V = double, float2 or double2 (8 or 16 bytes)
+  template <typename V> __device__ __forceinline__ V load(uint32_t voff, uint32_t soff) const {
+    if constexpr (sizeof(V) == 8) {
+      return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+    } else {
+      static_assert(sizeof(V) == 16, "8- or 16-byte loads");
+      return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+    }
+  }
+  __device__ __forceinline__ void store(float v, uint32_t voff, uint32_t soff) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+  }
+  __device__ __forceinline__ void store(double v, uint32_t voff, uint32_t soff) const {
+    using U = uint32_t __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U, v), r, (int)voff, (int)soff, 0);
+  }
+  __device__ __forceinline__ void store(float2 v, uint32_t voff, uint32_t soff) const {
+    using U = uint32_t __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U, v), r, (int)voff, (int)soff, 0);
+  }
+};
 
 template <typename T2> __device__ __forceinline__ T2 cadd(T2 a, T2 b) { return mk<T2>(a.x + b.x, a.y + b.y); }
 template <typename T2> __device__ __forceinline__ T2 csub(T2 a, T2 b) { return mk<T2>(a.x - b.x, a.y - b.y); }

@@ -135,11 +135,16 @@ __device__ __forceinline__ void col_out_twiddles(T2 u, T2* w) {
 }
 
 // output index k2 of register slot s (= q*G + kg) for the thread
+// output slot s of a thread holds row k2 = g Q + col_out_k2c(s): the thread's group g gives the
+// first term, the slot alone the second
+template <int P2, int E> __device__ __forceinline__ constexpr int col_out_k2c(int s) {
+  using S = ColShape<P2, E>;
+  return s / S::G + E * (s % S::G);
+}
 template <int P2, int E> __device__ __forceinline__ int col_out_k2(int s) {
   using S = ColShape<P2, E>;
   const int g = threadIdx.x / S::C;
-  const int q = s / S::G, kg = s % S::G;
-  return (g * S::Q + q) + E * kg;
+  return g * S::Q + col_out_k2c<P2, E>(s);
 }
 
 // -------------------------------------------------------------- row pass --

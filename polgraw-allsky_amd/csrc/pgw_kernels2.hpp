@@ -71,8 +71,10 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
     sgnlt1 = dsc.sgnlt1;
   }
   const int nifo = sb.nifo;
-  const T2* __restrict__ xm = static_cast<const T2*>(sb.xm) + (size_t)slot * sb.xm_stride;
-  const double* __restrict__ tmp1s = sb.tmp1 + (size_t)slot * sb.tmp1_stride;
+  // the slot's series xm[d][{a,b}][N] and _tmp1[d][N] as raw buffers: lane offset n, uniform
+  // offset d N (at most 2^28 bytes: nifo <= 8, N <= 2^20)
+  const BufRes xmb(static_cast<const T2*>(sb.xm) + (size_t)slot * sb.xm_stride, (uint32_t)(2 * nifo * N * sizeof(T2)));
+  const BufRes t1b(sb.tmp1 + (size_t)slot * sb.tmp1_stride, (uint32_t)(nifo * N * sizeof(double)));
   const int c = threadIdx.x % S::C, g = threadIdx.x / S::C;
   const int n1 = blockIdx.x * S::C + c;
   const int L1 = p.L1;
@@ -84,28 +86,40 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
     T2 xa = mk<T2>(0, 0), xb = mk<T2>(0, 0);
     if (r < NZ && n < N) {
       const double dn = (double)n;
+      const uint32_t vx = (uint32_t)n * (uint32_t)sizeof(T2), vt = (uint32_t)n * (uint32_t)sizeof(double);
 #pragma unroll 2
       for (int d = 0; d < nifo; d++) {  // two detectors' loads in flight together
+        const uint32_t dN = (uint32_t)d * (uint32_t)N;
         // phase = het1 i + sgnlt1 _tmp1 (jobcore.c:530), uncontracted; _tmp1 from the sky slot
-        const double tmp1 = ld_global(tmp1s, (size_t)d * N + n);
+        const double tmp1 = t1b.load<double>(vt, dN * (uint32_t)sizeof(double));
         const T2 ex = PhaseRot<T>::eval(lin2_nc(het1, dn, sgnlt1, tmp1));
-        xa = cadd(xa, cmul(ld_global(xm, (size_t)(2 * d) * N + n), ex));
-        xb = cadd(xb, cmul(ld_global(xm, (size_t)(2 * d + 1) * N + n), ex));
+        xa = cadd(xa, cmul(xmb.load<T2>(vx, 2 * dN * (uint32_t)sizeof(T2)), ex));
+        xb = cadd(xb, cmul(xmb.load<T2>(vx, (2 * dN + (uint32_t)N) * (uint32_t)sizeof(T2)), ex));
       }
     }
     x[0][r] = xa;
     x[1][r] = xb;
   }
-  T2* Za = Z + (size_t)t * 2 * L;
+  // the template's two spectra, 2 L elements (byte offsets fit 32 bits: L <= 2^20).  Output slot s
+  // is row k2 = g Q + col_out_k2c(s): lane offset (n1 + g Q L1), uniform offset col_out_k2c(s) L1
+  // (FP64 elements, 16 bytes, are stored through a plain pointer at the same offsets)
+  T2* const Zt = Z + (size_t)t * 2 * L;
+  const BufRes Zb(Zt, (uint32_t)(2 * L * sizeof(T2)));
+  const uint32_t voff = (uint32_t)(n1 + g * S::Q * L1) * (uint32_t)sizeof(T2);
+  const uint32_t rowb = (uint32_t)L1 * (uint32_t)sizeof(T2), specb = (uint32_t)(L * sizeof(T2));
+  auto put = [&](T2 v, uint32_t soff) {
+    if constexpr (sizeof(T2) == 8) Zb.store(v, voff, soff);
+    else *reinterpret_cast<T2*>(reinterpret_cast<char*>(Zt) + voff + soff) = v;
+  };
   T2 w[E];
   col_out_twiddles<P2, E>(tw_big(hi, lo, (uint32_t)n1 << p.twshift), w);
   if constexpr (!SEQ) {
     col_fft<T2, P2, E, 2, -1, NZ>(x, lds, twc);
 #pragma unroll
     for (int s = 0; s < E; s++) {
-      const int k2 = col_out_k2<P2, E>(s);
-      Za[(size_t)k2 * L1 + n1] = cmul(x[0][s], w[s]);
-      Za[L + (size_t)k2 * L1 + n1] = cmul(x[1][s], w[s]);
+      const uint32_t k2c = col_out_k2c<P2, E>(s);
+      put(cmul(x[0][s], w[s]), k2c * rowb);
+      put(cmul(x[1][s], w[s]), specb + k2c * rowb);
     }
   } else {
     // FP32 with 16 rows per thread: the step-1 twiddle powers of the first spectrum serve the
@@ -120,8 +134,7 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
       col_fft<T2, P2, E, 1, -1, NZ>(xa, lds, twc, PW ? pw : nullptr, a > 0);
 #pragma unroll
       for (int s = 0; s < E; s++) {
-        const int k2 = col_out_k2<P2, E>(s);
-        Za[a * L + (size_t)k2 * L1 + n1] = cmul(xa[0][s], w[s]);
+        put(cmul(xa[0][s], w[s]), (uint32_t)a * specb + (uint32_t)col_out_k2c<P2, E>(s) * rowb);
       }
     }
   }
@@ -163,8 +176,11 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
   // the first row workgroups find them still in the last-level cache (MALL)
   const int k2 = blockIdx.x, t = gridDim.y - 1 - blockIdx.y, tid = threadIdx.x;
   const size_t L = (size_t)P1 * p.L2;
-  const T2* rowa = Z + (size_t)t * 2 * L + (size_t)k2 * P1;
-  const T2* rowb = rowa + L;
+  // rows k2 of the template's two spectra and of its F, as raw buffers: lane offset tid, uniform
+  // offset k2 P1 + NT m (at most 2^25 bytes: L <= 2^20)
+  const BufRes Zr(Z + (size_t)t * 2 * L, (uint32_t)(2 * L * sizeof(T2)));
+  const uint32_t vr = (uint32_t)tid * (uint32_t)sizeof(T2), mb = NT * (uint32_t)sizeof(T2);
+  const uint32_t ra = (uint32_t)(k2 * P1) * (uint32_t)sizeof(T2), rb = ra + (uint32_t)(L * sizeof(T2));
   const double* aabb = aabb_slots + 2 * desc[t].slot;
   // F = |xa|^2/aa + |xb|^2/bb (jobcore.c:635-638) with the two reciprocals taken once: each
   // product differs from the quotient by at most one rounding
@@ -174,7 +190,7 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
   if constexpr (NA == 2) {
     T2 x[2][16];
 #pragma unroll
-    for (int m = 0; m < 16; m++) { x[0][m] = rowa[tid + NT * m]; x[1][m] = rowb[tid + NT * m]; }
+    for (int m = 0; m < 16; m++) { x[0][m] = Zr.load<T2>(vr, ra + mb * m); x[1][m] = Zr.load<T2>(vr, rb + mb * m); }
     row_fft<T2, P1, 2, -1>(x, lds, twr);
 #pragma unroll
     for (int m = 0; m < MO; m++)
@@ -183,10 +199,10 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
     constexpr bool PREF = RowOcc<T, P1, NA>::PREF;
     T2 x[1][16], y[1][PREF ? 16 : 1];
 #pragma unroll
-    for (int m = 0; m < 16; m++) x[0][m] = rowa[tid + NT * m];
+    for (int m = 0; m < 16; m++) x[0][m] = Zr.load<T2>(vr, ra + mb * m);
     if constexpr (PREF) {
 #pragma unroll
-      for (int m = 0; m < 16; m++) y[0][m] = rowb[tid + NT * m];  // in flight during the first FFT
+      for (int m = 0; m < 16; m++) y[0][m] = Zr.load<T2>(vr, rb + mb * m);  // in flight during the first FFT
     }
     // FP32: the stage-C twiddle powers of the first transform serve the second (no more
     // registers at the peak: they live while the prefetched row no longer does)
@@ -205,21 +221,22 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
     for (int m = 0; m < MO; m++) F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * raa;
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < 16; m++) x[0][m] = PREF ? y[0][PREF ? m : 0] : rowb[tid + NT * m];
+    for (int m = 0; m < 16; m++) x[0][m] = PREF ? y[0][PREF ? m : 0] : Zr.load<T2>(vr, rb + mb * m);
     row_fft<T2, P1, 1, -1>(x, lds, twr, PC ? pc : nullptr, true, pbp);
 #pragma unroll
     for (int m = 0; m < MO; m++) F[m] = F[m] + (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * rbb;
   }
   __syncthreads();
   T* Fs = reinterpret_cast<T*>(lds);
-  T* Frow = Ft + (size_t)t * L + (size_t)k2 * P1;
+  const BufRes Fb(Ft + (size_t)t * L, (uint32_t)(L * sizeof(T)));
+  const uint32_t vf = (uint32_t)tid * (uint32_t)sizeof(T), fo = (uint32_t)(k2 * P1) * (uint32_t)sizeof(T);
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const int k1 = tid + NT * m;
     if (m < MO) {
       const int k = p.L2 * k1 + k2;
       const bool ok = (k >= p.nmin && k < p.nmax);
-      if (ok) Frow[k1] = F[m < MO ? m : 0];
+      if (ok) Fb.store(F[m < MO ? m : 0], vf, fo + (uint32_t)(NT * m * sizeof(T)));
       Fs[k1] = ok ? F[m < MO ? m : 0] : (T)0;
     }
     // (HALF, m >= MO: the upper half of Fs, k1 >= P1/2, holds no bin of [nmin, nmax) and is not read)
@@ -293,38 +310,42 @@ __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const doub
   // the first row workgroups find them still in the last-level cache (MALL)
   const int k2 = blockIdx.x, t = gridDim.y - 1 - blockIdx.y, tid = threadIdx.x;
   const size_t L = (size_t)P1 * p.L2;
-  const T2* rowa = Z + (size_t)t * 2 * L + (size_t)k2 * P1;
-  const T2* rowb = rowa + L;
+  // rows k2 of the template's two spectra and of its F, as raw buffers: lane offset tid, uniform
+  // offset k2 P1 + NT m (at most 2^25 bytes: L <= 2^20)
+  const BufRes Zr(Z + (size_t)t * 2 * L, (uint32_t)(2 * L * sizeof(T2)));
+  const uint32_t vr = (uint32_t)tid * (uint32_t)sizeof(T2), mb = NT * (uint32_t)sizeof(T2);
+  const uint32_t ra = (uint32_t)(k2 * P1) * (uint32_t)sizeof(T2), rb = ra + (uint32_t)(L * sizeof(T2));
   const double* aabb = aabb_slots + 2 * desc[t].slot;
   const T raa = (T)(1.0 / ld_global(aabb, 0)), rbb = (T)(1.0 / ld_global(aabb, 1));
   constexpr int MO = HALF ? 4 : 8;  // output slots that can hold a bin of [nmin, nmax)
   T F[MO];
   T2 x[8], y[PREF ? 8 : 1];
 #pragma unroll
-  for (int m = 0; m < 8; m++) x[m] = rowa[tid + NT * m];
+  for (int m = 0; m < 8; m++) x[m] = Zr.load<T2>(vr, ra + mb * m);
   if constexpr (PREF) {
 #pragma unroll
-    for (int m = 0; m < 8; m++) y[m] = rowb[tid + NT * m];
+    for (int m = 0; m < 8; m++) y[m] = Zr.load<T2>(vr, rb + mb * m);
   }
   row_fft8<T2, P1, -1>(x, lds, twr);
 #pragma unroll
   for (int m = 0; m < MO; m++) F[m] = (x[m].x * x[m].x + x[m].y * x[m].y) * raa;
   __syncthreads();
 #pragma unroll
-  for (int m = 0; m < 8; m++) x[m] = PREF ? y[PREF ? m : 0] : rowb[tid + NT * m];
+  for (int m = 0; m < 8; m++) x[m] = PREF ? y[PREF ? m : 0] : Zr.load<T2>(vr, rb + mb * m);
   row_fft8<T2, P1, -1>(x, lds, twr);
 #pragma unroll
   for (int m = 0; m < MO; m++) F[m] = F[m] + (x[m].x * x[m].x + x[m].y * x[m].y) * rbb;
   __syncthreads();
   T* Fs = reinterpret_cast<T*>(lds);
-  T* Frow = Ft + (size_t)t * L + (size_t)k2 * P1;
+  const BufRes Fb(Ft + (size_t)t * L, (uint32_t)(L * sizeof(T)));
+  const uint32_t vf = (uint32_t)tid * (uint32_t)sizeof(T), fo = (uint32_t)(k2 * P1) * (uint32_t)sizeof(T);
 #pragma unroll
   for (int m = 0; m < 8; m++) {
     const int k1 = tid + NT * m;
     if (m < MO) {
       const int k = p.L2 * k1 + k2;
       const bool ok = (k >= p.nmin && k < p.nmax);
-      if (ok) Frow[k1] = F[m < MO ? m : 0];
+      if (ok) Fb.store(F[m < MO ? m : 0], vf, fo + (uint32_t)(NT * m * sizeof(T)));
       Fs[k1] = ok ? F[m < MO ? m : 0] : (T)0;
     }
     // (HALF, m >= MO: the upper half of Fs, k1 >= P1/2, holds no bin of [nmin, nmax) and is not read)
