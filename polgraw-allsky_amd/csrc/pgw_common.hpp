@@ -36,10 +36,13 @@ template <typename T2> __device__ __forceinline__ T2 ld_global(const T2* p, size
 // base + a 32-bit per-lane byte offset + a uniform byte offset held in a scalar register, so no
 // 64-bit vector address arithmetic is spent per access (a plain pointer access costs one 64-bit
 // vector add each).  Resource word 3 = 0x00020000: the gfx9-family raw buffer (32-bit data
-// format, no swizzle, no stride); accesses at or beyond `bytes` are range-checked (loads read 0,
-// stores are dropped).  Stores hold at most 64 bits of data: a buffer store of more data with a
-// scalar offset meets the store-data hazard (a VALU write to its data registers right after it,
-// for which the compiler places no wait state), which corrupted FP64 spectra.
+// format, no swizzle, no stride).  The hardware range check compares only the lane offset (voff
+// plus the instruction offset) with `bytes`, not the scalar offset, so callers keep
+// voff + soff + access size <= bytes by their own index bounds (every use below does) and never
+// rely on the check.  Stores hold at most 64 bits of data: with 128-bit buffer stores and a
+// scalar offset, FP64 spectra came out wrong at the tail of some dispatches (most likely a
+// store-data hazard the compiler left uncovered; plain 16-byte pointer stores are exact, and
+// the FP64 variant test with 8 templates per launch guards this).
 struct BufRes {
   __amdgpu_buffer_rsrc_t r;
   __device__ __forceinline__ BufRes(const void* base, uint32_t bytes)
