@@ -116,8 +116,10 @@ int pgw_search(pgw_handle* h, pgw_range* r, const pgw_opts* opts, const pgw_outp
       const int FN0 = *FNum;
       int cnt = 0;
       for (;;) {
-        st = pgw_job_batch(h, &sky[k].pm, (int)n, r, opts, &cnt, part.data(), (int)(part.size() / PGW_NPAR), FNum);
+        const int room = (int)(part.size() / PGW_NPAR);
+        st = pgw_job_batch(h, &sky[k].pm, (int)n, r, opts, &cnt, part.data(), room, FNum);
         if (st != PGW_ERR_CAPACITY) break;
+        if (cnt <= room) return PGW_ERR_CAPACITY;  // no larger count to make room for: do not spin
         part.resize((size_t)PGW_NPAR * cnt);       // room for every record of the batch, then again
         cnt = 0;
         *FNum = FN0;

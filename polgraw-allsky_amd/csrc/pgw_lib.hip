@@ -312,6 +312,8 @@ template <typename T> struct Engine final : pgw_handle {
   struct Group {
     std::vector<Item> items;
     int ntempl = 0, buf = 0;
+    Cand* cbuf = nullptr;  // device candidate buffer (default: cand + buf * kCandCap)
+    int ccap = 0;          // its capacity (default: cand_cap)
   };
 
   std::vector<std::unique_ptr<Seg>> segs;
@@ -344,6 +346,7 @@ template <typename T> struct Engine final : pgw_handle {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evd[2] = {nullptr, nullptr};
   int TB = 128, TBalloc = 0;
+  int cand_cap = kCandCap;  // candidates per group in the device buffer (PGW_TUNE candcap= lowers it)
   int group = kSlots;  // sky points per group
   int row_na = 1;
   bool col_seq = true;  // column pass: the two spectra share one LDS buffer in turn
@@ -520,6 +523,7 @@ template <typename T> struct Engine final : pgw_handle {
       if (const char* q = strstr(tune, "half=")) if (sscanf(q, "half=%d", &v) == 1) row_half = v != 0;
       if (const char* q = strstr(tune, "row8=")) if (sscanf(q, "row8=%d", &v) == 1 && v >= 0 && v <= 2) row8 = v;
       if (const char* q = strstr(tune, "thr_fuse=")) if (sscanf(q, "thr_fuse=%d", &v) == 1) thr_fuse = v != 0;
+      if (const char* q = strstr(tune, "candcap=")) if (sscanf(q, "candcap=%d", &v) == 1 && v >= 1 && v <= kCandCap) cand_cap = v;
     }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
@@ -723,8 +727,8 @@ template <typename T> struct Engine final : pgw_handle {
   // template stage for descriptors [t0, t0 + nt) of buffer buf; ev (optional) = 4 events around the kernels
   // uni: all nt templates belong to one item (its templates j0, j0 + 1, ...), else nullptr
   // fold: leave this launch's threshold pending, to run inside the next column launch (or flush_thr)
-  int launch_templates(const Seg& g, int buf, int t0, int nt, const Item* uni, int j0, int Nl, const pgw_opts* o,
-                       hipEvent_t* ev = nullptr, bool fold = false) {
+  int launch_templates(const Seg& g, int buf, Cand* cb, int ccap, int t0, int nt, const Item* uni, int j0, int Nl,
+                       const pgw_opts* o, hipEvent_t* ev = nullptr, bool fold = false) {
     const TmplDesc* d = desc.p + (size_t)buf * kDescCap + t0;
     if (ev) HIPCHK(hipEventRecord(ev[0], stream));
     SlotBases sb{xm_slots.p, tmp1_slots.p, aabb_slots.p, slot_xm, slot_tmp1, nifo, -1, 0, 0};
@@ -752,12 +756,12 @@ template <typename T> struct Engine final : pgw_handle {
     RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2)};
     launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, row_half, row8, stream);
     if (ev) HIPCHK(hipEventRecord(ev[2], stream));
-    ThrParams tp{g.ps->L1, g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, o->white_flag, t0, kCandCap, ilog2(g.ps->L2), o->trl};
+    ThrParams tp{g.ps->L1, g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, o->white_flag, t0, ccap, ilog2(g.ps->L2), o->trl};
     if (fold) {
-      thr_prev = ThrFuse<T>{tp, Ft.p, psum.p, pmax.p, mu.p, cand.p + (size_t)buf * kCandCap, ncand.p + buf, nt, 0};
+      thr_prev = ThrFuse<T>{tp, Ft.p, psum.p, pmax.p, mu.p, cb, ncand.p + buf, nt, 0};
     } else {
       hipLaunchKernelGGL((k_threshold<T>), dim3(g.nparts, nt), dim3(64), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p,
-                         cand.p + (size_t)buf * kCandCap, ncand.p + buf);
+                         cb, ncand.p + buf);
     }
     if (ev) HIPCHK(hipEventRecord(ev[3], stream));
     return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
@@ -771,6 +775,10 @@ template <typename T> struct Engine final : pgw_handle {
   // Queues a group: sky stages into slots, descriptors, template launches, async candidate copy.
   int launch_group(Group& gr, const pgw_opts* o) {
     int st = PGW_OK;
+    if (!gr.cbuf) {
+      gr.cbuf = cand.p + (size_t)gr.buf * kCandCap;
+      gr.ccap = cand_cap;
+    }
     SkyItem si[kSlots];
     for (Item& it : gr.items)
       if (!sky_item(*segs[it.seg], it.pm, it.mm, it.nn, it.slot, it.sk, si[it.slot])) return PGW_ERR_DEVICE;
@@ -790,7 +798,7 @@ template <typename T> struct Engine final : pgw_handle {
       const Item& it = gr.items[k];
       const bool one = t0 + nt <= it.t0 + it.ns();
       const bool timed = ktime && kseq++ % ktime == 0;
-      if ((st = launch_templates(g0, gr.buf, t0, nt, one ? &it : nullptr, t0 - it.t0, Ng, o,
+      if ((st = launch_templates(g0, gr.buf, gr.cbuf, gr.ccap, t0, nt, one ? &it : nullptr, t0 - it.t0, Ng, o,
                                  timed ? kquad(nt) : nullptr, thr_fuse))) {
         thr_prev.nt = 0;
         return st;
@@ -798,22 +806,41 @@ template <typename T> struct Engine final : pgw_handle {
     }
     if ((st = flush_thr())) return st;
     HIPCHK(hipMemcpyAsync(hn_pin.p + gr.buf, ncand.p + gr.buf, sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(hc_pin.p + (size_t)gr.buf * kCandChunk, cand.p + (size_t)gr.buf * kCandCap,
-                          sizeof(Cand) * kCandChunk, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(hc_pin.p + (size_t)gr.buf * kCandChunk, gr.cbuf, sizeof(Cand) * kCandChunk,
+                          hipMemcpyDeviceToHost, stream));
     HIPCHK(hipEventRecord(evd[gr.buf], stream));
     return PGW_OK;
   }
 
   // Waits for a group's candidates and appends its records (jobcore.c:659-695) in (sky point, ss, bin) order.
-  int finish_group(const Group& gr, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap, int* rec_work) {
+  int finish_group(const Group& gr0, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap, int* rec_work) {
 #pragma clang fp contract(off)
-    HIPCHK(hipEventSynchronize(evd[gr.buf]));
-    const int nc = hn_pin.p[gr.buf];
-    if (nc > kCandCap) return PGW_ERR_CAPACITY;
+    HIPCHK(hipEventSynchronize(evd[gr0.buf]));
+    int nc = hn_pin.p[gr0.buf];  // the device count keeps counting past the buffer's capacity
+    const Group* pg = &gr0;
+    Group big;
+    DevBuf<Cand> bigbuf;
+    if (nc > gr0.ccap) {
+      // More candidates than the group's device buffer holds: run the group again on its own into
+      // a buffer of exactly that many.  The next group (other buffer set) is drained first; the
+      // rerun rebuilds this group's sky slots, which that group no longer needs.
+      HIPCHK(hipStreamSynchronize(stream));
+      int st;
+      if ((st = bigbuf.alloc((size_t)std::max(nc, kCandChunk)))) return st;
+      big = gr0;
+      big.cbuf = bigbuf.p;
+      big.ccap = nc;
+      if ((st = launch_group(big, o))) return st;
+      HIPCHK(hipEventSynchronize(evd[big.buf]));
+      if (hn_pin.p[big.buf] != nc) return PGW_ERR_DEVICE;  // the same work found a different count
+      if (hcand.size() < (size_t)nc) hcand.resize(nc);
+      pg = &big;
+    }
+    const Group& gr = *pg;
     memcpy(hcand.data(), hc_pin.p + (size_t)gr.buf * kCandChunk, sizeof(Cand) * std::min(nc, kCandChunk));
     if (nc > kCandChunk) {  // rare: the rest synchronously (the buffer is not reused before this returns)
-      HIPCHK(hipMemcpyAsync(hcand.data() + kCandChunk, cand.p + (size_t)gr.buf * kCandCap + kCandChunk,
-                            sizeof(Cand) * (nc - kCandChunk), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipMemcpyAsync(hcand.data() + kCandChunk, gr.cbuf + kCandChunk, sizeof(Cand) * (nc - kCandChunk),
+                            hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
     }
     std::sort(hcand.begin(), hcand.begin() + nc,
@@ -1039,7 +1066,7 @@ template <typename T> struct Engine final : pgw_handle {
       acc[0] = t;
       for (int rr = 0; rr < reps && !st; rr++) {
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p, 1);
-        st = launch_templates(g, 0, 0, nt, &it, 0, g.sett.N, o, ev);
+        st = launch_templates(g, 0, cand.p, cand_cap, 0, nt, &it, 0, g.sett.N, o, ev);
         HIPCHK(hipStreamSynchronize(stream));
         for (int k = 0; k < 3; k++) {
           HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
