@@ -58,10 +58,68 @@ class FStatNet:
         self.h = handle
         self.seg = seg
 
-    def __call__(self, pts, grad: bool = True):
+    def __call__(self, pts, grad: bool = True, owner=None):
         """pts (n, 4) = (omega, omegadot, delta, alpha) -> (out (n, 11), dF/dpts (n, 4) or None).
-        out[:, 5] = -F as the reference; the gradient is of F."""
+        out[:, 5] = -F as the reference; the gradient is of F.  owner (the candidate each point
+        belongs to) is not needed with one segment."""
         return self.h.fstatnet(pts, seg=self.seg, grad=grad)
+
+
+class MultiFStatNet:
+    """Fstatnet for candidates spread over the segments of one pgw.Handle: candidate k lives in
+    segment seg_of[k].  The searches below pass each point's candidate (`owner`); the points of
+    each segment go to the device in one call."""
+
+    def __init__(self, handle: pgw.Handle, seg_of):
+        self.h = handle
+        self.seg_of = np.asarray(seg_of, dtype=np.int64)
+
+    def __call__(self, pts, grad: bool = True, owner=None):
+        pts = np.asarray(pts, dtype=np.float64).reshape(-1, 4)
+        if owner is None:
+            if len(pts) != len(self.seg_of):
+                raise ValueError("points of several segments need their owners")
+            owner = np.arange(len(pts))
+        seg = self.seg_of[np.asarray(owner, dtype=np.int64)]
+        out = np.zeros((len(pts), 11))
+        g = np.zeros((len(pts), 4)) if grad else None
+        for sgi in np.unique(seg):
+            sel = np.nonzero(seg == sgi)[0]
+            o, gg = self.h.fstatnet(pts[sel], seg=int(sgi), grad=grad)
+            out[sel] = o
+            if grad:
+                g[sel] = gg
+        return out, g
+
+
+def grid_point(M, oms: float, point, epsm: float):
+    """Hemisphere and grid coordinates g (M g = (omega, omegadot, al1, al2)) of a point
+    (omega, omegadot, delta, alpha), with al = (oms + omega) x the linear sky coordinates (the
+    followup's mapping, followup/src/followup.c:1482-1490)."""
+    from .synth import ast2lin
+    om, omd, de, al = (float(v) for v in point)
+    pm, be = ast2lin(al, de, epsm)
+    Mm = np.asarray(M, dtype=np.float64).reshape(4, 4).T
+    return pm, np.linalg.solve(Mm, np.array([om, omd, be[0] * (oms + om), be[1] * (oms + om)]))
+
+
+def point_of_grid(M, oms: float, g, pm: int, epsm: float):
+    """The inverse of grid_point: (omega, omegadot, delta, alpha) at grid coordinates g."""
+    from .synth import lin2ast
+    Mm = np.asarray(M, dtype=np.float64).reshape(4, 4).T
+    om, omd, a1, a2 = Mm @ np.asarray(g, dtype=np.float64)
+    de, al = lin2ast(a1 / (oms + om), a2 / (oms + om), pm, epsm)
+    return np.array([om, omd, de, al])
+
+
+def cell_extent(M, oms: float, point, epsm: float) -> np.ndarray:
+    """Extent of one grid cell along each parameter at `point`: |theta(g + 1) - theta(g)|
+    between two opposite corners of the cell (all four grid indices + 1), the span the followup
+    doubles into its first simplex steps (followup/src/followup.c:1555-1567)."""
+    pm, g = grid_point(M, oms, point, epsm)
+    a = point_of_grid(M, oms, g, pm, epsm)
+    b = point_of_grid(M, oms, g + 1.0, pm, epsm)
+    return np.abs(b - a)
 
 
 def _torch_fn():
@@ -104,7 +162,8 @@ def ascend(fs: FStatNet, start, metric, steps: int = 20, trust0: float = 1.0, tr
     G = np.broadcast_to(np.asarray(metric, dtype=np.float64), (n, 4, 4)).copy()
     # regularise the near-degenerate directions of short segments
     G += 1e-9 * np.trace(G, axis1=1, axis2=2)[:, None, None] * np.eye(4)
-    out, g = fs(x)
+    own = np.arange(n)
+    out, g = fs(x, owner=own)
     F = -out[:, 5]
     H = np.linalg.inv(2 * F[:, None, None] * G)
     trust = np.full(n, float(trust0))
@@ -114,7 +173,7 @@ def ascend(fs: FStatNet, start, metric, steps: int = 20, trust0: float = 1.0, tr
         d = np.einsum("nij,nj->ni", H, g)
         rms = np.sqrt(np.maximum(np.einsum("ni,nij,nj->n", d, G, d), 1e-300))
         d *= np.minimum(1.0, trust / rms)[:, None]
-        out_t, g_t = fs(x + d)
+        out_t, g_t = fs(x + d, owner=own)
         nev += 1
         Ft = -out_t[:, 5]
         ok = Ft > F
@@ -147,14 +206,14 @@ def amoeba(fs: FStatNet, starts, steps, tol: float = 1e-7, maxeval: int = 20000)
     sx = np.repeat(x0[:, None, :], D + 1, axis=1)          # (n, 5, 4)
     for i in range(D):
         sx[:, i, i] += st[:, i]
-    out = fs(sx.reshape(-1, D), grad=False)[0].reshape(n, D + 1, 11)
+    out = fs(sx.reshape(-1, D), grad=False, owner=np.repeat(np.arange(n), D + 1))[0].reshape(n, D + 1, 11)
     nev = np.full(n, D + 1)
     done = np.zeros(n, bool)
     conv = np.zeros(n, bool)
     ar = np.arange(n)
 
-    def evaluate(idx, pts):
-        return fs(pts, grad=False)[0] if len(idx) else np.zeros((0, 11))
+    def evaluate(owner, pts):
+        return fs(pts, grad=False, owner=owner)[0] if len(owner) else np.zeros((0, 11))
 
     while True:
         act = np.nonzero(~done)[0]
@@ -222,7 +281,7 @@ def amoeba(fs: FStatNet, starts, steps, tol: float = 1e-7, maxeval: int = 20000)
                         sx[k, i] = (sx[k, ilo[s]] + sx[k, i]) * 0.5
                         pts.append(sx[k, i])
                         where.append((k, i))
-                o = evaluate(idx, np.array(pts))
+                o = evaluate(np.array([k for k, _ in where]), np.array(pts))
                 for (k, i), oo in zip(where, o):
                     out[k, i] = oo
                 nev[idx] += D
@@ -269,14 +328,14 @@ def mads(fs: FStatNet, starts, p2, nod: float, limit: int = 700):
     p2 = np.broadcast_to(np.asarray(p2, dtype=np.float64), x.shape)
     step = (1.0 - (1.0 - 0.01 * 6.0 / nod)) * p2
     param = step.copy()
-    best = fs(x, grad=False)[0]
+    best = fs(x, grad=False, owner=np.arange(n))[0]
     rounds = np.zeros(n, int)
     act = np.ones(n, bool)
     while act.any():
         idx = np.nonzero(act)[0]
         rounds[idx] += 1
         pts = x[idx, None, :] + MADS_STENCIL[None, :, :] * param[idx, None, :]
-        out = fs(pts.reshape(-1, 4), grad=False)[0].reshape(len(idx), 80, 11)
+        out = fs(pts.reshape(-1, 4), grad=False, owner=np.repeat(idx, 80))[0].reshape(len(idx), 80, 11)
         j = np.argmin(out[:, :, 5], axis=1)              # first minimum in stencil order
         cand = out[np.arange(len(idx)), j]
         better = cand[:, 5] < best[idx, 5]

@@ -160,6 +160,7 @@ class Segment:
     sig2: list
     injection: dict = field(default_factory=dict)
     range: dict = field(default_factory=dict)
+    injections: list = field(default_factory=list)
 
 
 def _sig2(x: np.ndarray) -> float:
@@ -257,6 +258,54 @@ def make_segment(N: int = 86400, dets=("H1",), sigmas=(1.0,), dt: float = 2.0, f
                    epsm=C_EPSMA, amod=amods, sig2=sig2, injection=inj, range=rng_s)
 
 
+def make_injections(N: int = 86400, dets=("H1", "L1"), sigmas=(1.0, 1.3), dt: float = 2.0, fftpad: int = 2,
+                    seed: int = 1, nsig: int = 64, h0_range=(0.05, 0.3), band: int = 666) -> Segment:
+    """One seeded noise segment carrying nsig injected signals (add_signal(),
+    search/network/src-cpu/init.c:562-642, summed).  Signal k's frequency is drawn from the middle
+    half of the k-th of nsig equal strata of [0.1 pi, 0.9 pi], so neighbours are at least half a
+    stratum apart (~270 Fourier bins for 64 signals at N=86400), far beyond a signal's Doppler and
+    spindown spread (< 20 bins): each candidate's F sees its own signal.  Spindown
+    U[-1e-8, 0] rad/sample^2, uniform sky, psi and phi0 U[0, 2 pi), cos(iota) U[-1, 1],
+    h0 U[h0_range] (units of the first detector's sigma).  seg.injections lists the signals
+    (f, fdot, delta, alpha, amp, h0, pm) in frequency order."""
+    rng = np.random.default_rng(seed)
+    base = make_segment(N=N, dets=dets, sigmas=sigmas, dt=dt, fftpad=fftpad, seed=seed, inject=False,
+                        nsky=(1, 1), nspin=1, band=band)
+    sett = base.sett
+    t = np.arange(N, dtype=np.float64)
+    cosmodf, sinmodf = np.cos(sett["omr"] * t), np.sin(sett["omr"] * t)
+    width = 0.8 * math.pi / nsig
+    cof0 = sett["oms"]
+    injs = []
+    for k in range(nsig):
+        f = 0.1 * math.pi + width * (k + rng.uniform(0.25, 0.75))
+        fdot = -1e-8 * rng.uniform(0.0, 1.0)
+        delta = math.asin(rng.uniform(-1.0, 1.0))
+        alpha = rng.uniform(0.0, 2 * math.pi)
+        ph_o, psik = rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi)
+        amp = amplitudes(ph_o, psik, rng.uniform(-1.0, 1.0))
+        h0 = rng.uniform(*h0_range) * sigmas[0]
+        pm, _ = ast2lin(alpha, delta, C_EPSMA)
+        injs.append(dict(f=f, fdot=fdot, delta=delta, alpha=alpha, amp=amp, h0=h0, pm=pm))
+    for d in range(len(dets)):
+        sig = np.zeros(N)
+        for j in injs:
+            sa, ca, sd, cd = math.sin(j["alpha"]), math.cos(j["alpha"]), math.sin(j["delta"]), math.cos(j["delta"])
+            aa, bb = modvir(sa, ca, sd, cd, math.sin(base.phir[d]), math.cos(base.phir[d]), base.amod[d],
+                            cosmodf, sinmodf)
+            shift = base.DetSSB[d].reshape(-1, 3) @ np.array([ca * cd, sa * cd, sd])
+            phase = j["f"] * t + j["fdot"] * t * t + (cof0 + j["f"] + 2.0 * j["fdot"] * t) * shift
+            cph, sph = np.cos(phase), np.sin(phase)
+            a = j["amp"]
+            sig += j["h0"] * (a[0] * aa * cph + a[2] * aa * sph + a[1] * bb * cph + a[3] * bb * sph)
+        nzm = base.xDat[d] != 0.0
+        base.xDat[d][nzm] += sig[nzm]
+    base.sig2 = [_sig2(x) for x in base.xDat]
+    base.injection = {}
+    base.injections = injs
+    return base
+
+
 def sky_list(seg: Segment, max_points: int | None = None):
     """(pm, mm, nn) in the reference's loop order (jobcore.c:90-98) over seg.range."""
     out = []
@@ -266,3 +315,37 @@ def sky_list(seg: Segment, max_points: int | None = None):
             for nn in range(r["nr"][0], r["nr"][1] + 1):
                 out.append((pm, mm, nn))
     return out if max_points is None else out[:max_points]
+
+
+# The measured configurations: make_segment arguments (seed added by workload_segments) and text.
+WORKLOADS = {
+    "network": (dict(N=86400, dets=("H1", "L1"), sigmas=(1.0, 1.3), fftpad=2, h0=0.1, nsky=(16, 16), nspin=512),
+                "network: 2 detectors (H1, L1), N=86400, nfft=2^17, fftpad=2, L=2^18, 256 sky x 512 spindowns, trl=20"),
+    "plain": (dict(N=86400, dets=("H1",), sigmas=(1.0,), fftpad=1, h0=0.3, nsky=(2, 2), nspin=32),
+              "plain: 1 detector (H1), N=86400, nfft=L=2^17, fftpad=1, 4 sky x 32 spindowns, trl=20"),
+    "precision": (dict(N=259200, dets=("H1", "L1"), sigmas=(1.0, 1.3), fftpad=2, h0=0.06, nsky=(32, 32), nspin=256),
+                  "precision: 2 detectors (H1, L1), N=259200, nfft=2^18, fftpad=2, L=2^19, 1024 sky x 256 spindowns, "
+                  "trl=20"),
+}
+BATCHING_TEXT = ("batching: 3 detectors (H1, L1, V1), 64 segments of N in [60000, 86400] with 5-30 % gaps, "
+                 "nfft 2^16 or 2^17, fftpad=1, 32 sky x 512 spindowns per segment, trl=20")
+
+
+def workload_segment(name: str, seed: int = 2024) -> Segment:
+    """The segment of a single-segment configuration (plain, network, precision)."""
+    return make_segment(seed=seed, **WORKLOADS[name][0])
+
+
+def batching_segments(seed: int = 2024, nseg: int = 64) -> list:
+    """The batching configuration: nseg segments of 3 detectors, N uniform in [60000, 86400]
+    (nfft 2^16 or 2^17), zeroed gaps over a fraction uniform in [0.05, 0.3], one injection
+    (h0 = 0.12) each, 4 x 8 sky points x 512 spindowns around it."""
+    rs = np.random.default_rng(seed)
+    segs = []
+    for k in range(nseg):
+        N = int(rs.integers(60000, 86401))
+        segs.append(make_segment(N=N, dets=("H1", "L1", "V1"), sigmas=(1.0, 1.3, 1.1), fftpad=1,
+                                 seed=seed + 1 + k, h0=0.12, gap_frac=float(rs.uniform(0.05, 0.3)),
+                                 nsky=(4, 8), nspin=512))
+    return segs
+

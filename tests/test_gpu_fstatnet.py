@@ -162,7 +162,7 @@ def test_batched_mads_on_device(pgw, abi, orc):
     x, out, rounds = followup.mads(followup.FStatNet(h), starts, 2.0 * sc, nod)
     S = orc.Search(seg, nthreads=8)
 
-    def ofs(pts, grad=False):
+    def ofs(pts, grad=False, owner=None):
         return S.fstatnet(np.asarray(pts).reshape(-1, 4), grad=False)[0], None
 
     xo, outo, ro = followup.mads(ofs, starts, 2.0 * sc, nod)

@@ -124,7 +124,7 @@ class _OracleFS:
     def __init__(self, S):
         self.S = S
 
-    def __call__(self, pts, grad=False):
+    def __call__(self, pts, grad=False, owner=None):
         out, _ = self.S.fstatnet(np.asarray(pts, dtype=np.float64).reshape(-1, 4), grad=False)
         return out, None
 
