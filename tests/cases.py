@@ -87,3 +87,38 @@ def noise_free_signal(N=32768, dets=("H1", "L1"), seed=5, h0=0.05, fftpad=2, amp
         seg.sig2[k] = 1.0
         rho2 += float(np.sum(sig ** 2))
     return seg, np.array([f, fdot, delta, alpha]), amp * h0, rho2
+
+
+FP32_MARGIN = 1e-4  # the FP32 form's bound on relative F error against the FP64 path
+
+
+def fp32_record_diff(got, ref, trl, margin=FP32_MARGIN):
+    """FP32-form records `got` against FP64 records `ref` (rows of 5 doubles; F = snr^2/2 + 2).
+
+    The rule: records are keyed by their template (fields 1-3) and frequency (field 0), which the
+    host computes identically in both forms.  A record in both sets must agree in F within
+    `margin` relative.  A record in one set only is allowed only where FP32 rounding of F (at most
+    `margin` relative) can change the outcome of the run-max threshold:
+      (a) threshold: its F lies within margin * trl of trl, so its run can appear or vanish;
+      (b) run maximum: the other set has an unmatched record of the same template whose F agrees
+          within `margin`, i.e. the maximum of a run moved between two bins of nearly equal F.
+    Returns (max relative dF over matched records, matched count, [(record, reason)] of allowed
+    exceptions, [record] violations)."""
+    F = lambda snr: snr * snr / 2 + 2  # noqa: E731
+    kg = {tuple(r[:4]): F(r[4]) for r in got}
+    kr = {tuple(r[:4]): F(r[4]) for r in ref}
+    common = set(kg) & set(kr)
+    rel = max((abs(kg[k] - kr[k]) / kr[k] for k in common), default=0.0)
+    viol = [k for k in common if abs(kg[k] - kr[k]) > margin * kr[k]]
+    only_g = {k: kg[k] for k in set(kg) - common}
+    only_r = {k: kr[k] for k in set(kr) - common}
+    allowed = []
+    for mine, other in ((only_g, only_r), (only_r, only_g)):
+        for k, f in mine.items():
+            if abs(f - trl) <= margin * trl:
+                allowed.append((k, "threshold"))
+            elif any(o[1:4] == k[1:4] and abs(fo - f) <= margin * max(f, fo) for o, fo in other.items()):
+                allowed.append((k, "run maximum"))
+            else:
+                viol.append(k)
+    return rel, len(common), allowed, viol

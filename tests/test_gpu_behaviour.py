@@ -8,7 +8,7 @@ and the FP32 form against the FP64 oracle.
 import numpy as np
 import pytest
 
-from cases import injected_sky, segment
+from cases import fp32_record_diff, injected_sky, segment
 
 pytestmark = pytest.mark.gpu
 
@@ -104,7 +104,7 @@ def test_capacity_error(pgw, abi):
 @pytest.mark.parametrize("name", ["net-2det-86k", "plain-1det-86k", "batch-3det-60k", "prec-2det-259k"])
 def test_fp32_form_vs_fp64_oracle(name, pgw, abi, orc):
     """FP32 series/FFTs (FP64 phases and spline coefficients) vs the FP64 oracle: max relative dF
-    <= 1e-4 on bins with F >= 1 and the same records except where F is within 1e-4 of the threshold."""
+    <= 1e-4 on bins with F >= 1, and the records by the rule of cases.fp32_record_diff."""
     seg = segment(name)
     pm, mm, nn = injected_sky(seg)
     ss = int(seg.injection["grid"][1])
@@ -121,11 +121,7 @@ def test_fp32_form_vs_fp64_oracle(name, pgw, abi, orc):
     got, fg = h.job_core(pm, mm, nn, rng, opts)
     ref, fr = o.job_core(pm, mm, nn, rng, opts)
     assert fg == fr
-    kg = {tuple(r[:4]): r[4] ** 2 / 2 + 2 for r in got}
-    kr = {tuple(r[:4]): r[4] ** 2 / 2 + 2 for r in ref}
-    for k in set(kg) ^ set(kr):
-        F = kg.get(k, kr.get(k))
-        assert abs(F - 20.0) <= 1e-4 * 20.0, (k, F)
-    common = set(kg) & set(kr)
-    assert len(common) > 0
-    assert max(abs(kg[k] - kr[k]) / kr[k] for k in common) <= 1e-4
+    rel, nmatch, allowed, viol = fp32_record_diff(got, ref, 20.0)
+    print(name, "records", len(got), len(ref), "max rel dF", rel, "allowed", allowed, flush=True)
+    assert not viol, viol[:5]
+    assert nmatch > 0
