@@ -403,7 +403,7 @@ def differentiation_setup(seed, nseg=16, per_seg=64):
     segs = [synth.make_injections(N=86400, dets=("H1", "L1"), sigmas=(1.0, 1.3), fftpad=2, seed=seed + 101 + k,
                                   nsig=per_seg) for k in range(nseg)]
     rs = np.random.default_rng(seed)
-    truth, starts, cells, seg_of, h0 = [], [], [], [], []
+    truth, starts, cells, diags, seg_of, h0 = [], [], [], [], [], []
     for k, sg in enumerate(segs):
         oms = sg.sett["oms"]
         for j in sg.injections:
@@ -417,14 +417,16 @@ def differentiation_setup(seed, nseg=16, per_seg=64):
                     continue
             truth.append(t)
             starts.append(x)
-            cells.append(followup.cell_extent(sg.M, oms, x, sg.epsm))
+            cells.append(followup.cell_width(sg.M, oms, x, sg.epsm))
+            diags.append(followup.cell_diagonal(sg.M, oms, x, sg.epsm))
             seg_of.append(k)
             h0.append(j["h0"])
-    return segs, np.array(truth), np.array(starts), np.array(cells), np.array(seg_of), np.array(h0)
+    return (segs, np.array(truth), np.array(starts), np.array(cells), np.array(diags), np.array(seg_of),
+            np.array(h0))
 
 
 def param_errors(x, truth, cells):
-    """|x - truth| per parameter in grid cells (right ascension wrapped to (-pi, pi])."""
+    """|x - truth| per parameter in grid-cell widths (right ascension wrapped to (-pi, pi])."""
     d = x - truth
     d[:, 3] = (d[:, 3] + np.pi) % (2 * np.pi) - np.pi
     return np.abs(d) / cells
@@ -445,7 +447,7 @@ def bench_differentiation(args, steps=None, warmup=None, cpu=True, mads_points=6
     steps = (args.steps or 20) if steps is None else steps
     warmup = args.warmup if warmup is None else warmup
     t_setup = time.perf_counter()
-    segs, truth, pts, cells, seg_of, h0 = differentiation_setup(args.seed)
+    segs, truth, pts, cells, diags, seg_of, h0 = differentiation_setup(args.seed)
     npts = len(pts)
     print(f"differentiation: {len(segs)} segments x {npts // len(segs)} injections ({time.perf_counter() - t_setup:.1f} s "
           f"setup)", file=sys.stderr, flush=True)
@@ -484,7 +486,7 @@ def bench_differentiation(args, steps=None, warmup=None, cpu=True, mads_points=6
     t_asc = time.perf_counter() - t1
     # the followup's simplex from the same points
     t3 = time.perf_counter()
-    xs, outs, nev_s, conv_s = followup.amoeba(fs, pts, 2.0 * cells, tol=1e-7)
+    xs, outs, nev_s, conv_s = followup.amoeba(fs, pts, 2.0 * diags, tol=1e-7)
     t_nm = time.perf_counter() - t3
     Fs = -outs[:, 5]
     print(f"ascent {t_asc:.2f} s, simplex {t_nm:.2f} s ({int(nev_s.sum())} evaluations)", file=sys.stderr, flush=True)
@@ -492,7 +494,7 @@ def bench_differentiation(args, steps=None, warmup=None, cpu=True, mads_points=6
     nm = min(mads_points, npts)
     nod = 86400 * 2.0 / 86400.0
     t4 = time.perf_counter()
-    xm, outm, rounds_m = followup.mads(followup.MultiFStatNet(h, seg_of[:nm]), pts[:nm], 2.0 * cells[:nm], nod)
+    xm, outm, rounds_m = followup.mads(followup.MultiFStatNet(h, seg_of[:nm]), pts[:nm], 2.0 * diags[:nm], nod)
     t_mads = time.perf_counter() - t4
     e0, ea, es, em = (param_errors(x, t, c) for x, t, c in ((pts, truth, cells), (xa, truth, cells),
                                                              (xs, truth, cells), (xm, truth[:nm], cells[:nm])))
@@ -544,7 +546,7 @@ def bench_differentiation(args, steps=None, warmup=None, cpu=True, mads_points=6
             "vs_fp64_oracle": orc_grad},
         "ascent": {"steps": 20, "evaluations_per_point": nev, "seconds": round(t_asc, 3),
                    "F_median": float(np.median(Fa)), "param_error": err_summary(ea)},
-        "simplex": {"tol": 1e-7, "first_steps": "2 grid cells", "evaluations_per_point_mean":
+        "simplex": {"tol": 1e-7, "first_steps": "twice the grid cell's diagonal", "evaluations_per_point_mean":
                     round(float(nev_s.mean()), 1), "converged": int(conv_s.sum()), "seconds": round(t_nm, 3),
                     "F_median": float(np.median(Fs)), "param_error": err_summary(es)},
         "ascent_vs_simplex": {"F_rel_diff_median": float(np.median(rel)), "F_rel_diff_min": float(rel.min()),

@@ -112,14 +112,30 @@ def point_of_grid(M, oms: float, g, pm: int, epsm: float):
     return np.array([om, omd, de, al])
 
 
-def cell_extent(M, oms: float, point, epsm: float) -> np.ndarray:
-    """Extent of one grid cell along each parameter at `point`: |theta(g + 1) - theta(g)|
-    between two opposite corners of the cell (all four grid indices + 1), the span the followup
-    doubles into its first simplex steps (followup/src/followup.c:1555-1567)."""
+def _corner(M, oms, g, step, pm, epsm):
+    """The point at grid coordinates g + step, or at g - step where that corner lies beyond the
+    sky disk's edge."""
+    try:
+        return point_of_grid(M, oms, g + step, pm, epsm)
+    except ValueError:
+        return point_of_grid(M, oms, g - step, pm, epsm)
+
+
+def cell_diagonal(M, oms: float, point, epsm: float) -> np.ndarray:
+    """|theta(g + 1) - theta(g)| per parameter between two opposite corners of the grid cell at
+    `point` (all four grid indices + 1): twice this is the followup's first simplex step
+    (followup/src/followup.c:1555-1567)."""
     pm, g = grid_point(M, oms, point, epsm)
     a = point_of_grid(M, oms, g, pm, epsm)
-    b = point_of_grid(M, oms, g + 1.0, pm, epsm)
-    return np.abs(b - a)
+    return np.abs(_corner(M, oms, g, np.ones(4), pm, epsm) - a)
+
+
+def cell_width(M, oms: float, point, epsm: float) -> np.ndarray:
+    """Width of the grid cell at `point` along each parameter: sum over the four grid directions
+    of |theta(g + e_j) - theta(g)| (the cell's bounding box, to first order)."""
+    pm, g = grid_point(M, oms, point, epsm)
+    a = point_of_grid(M, oms, g, pm, epsm)
+    return sum(np.abs(_corner(M, oms, g, e, pm, epsm) - a) for e in np.eye(4))
 
 
 def _torch_fn():
