@@ -61,6 +61,8 @@ def lib():
         L.orc_fstat_norm.argtypes = [_dp, C.c_int, C.c_int]
         L.orc_fstatnet_batch.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp]
         L.orc_amoeba.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, _dp]
+        L.orc_mads.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, _dp]
+        L.orc_mads_stencil.argtypes = [C.POINTER(C.c_byte)]
         _lib = L
     return _lib
 
@@ -174,11 +176,27 @@ class Search:
                                out.ctypes.data_as(_dp))
         return x, out, nev
 
+    def mads(self, point, p2, nod, limit=700):
+        """followup's invertedMADS from point; returns (final centre, its 11 values, rounds)."""
+        x = np.ascontiguousarray(np.asarray(point, dtype=np.float64).copy())
+        p = np.ascontiguousarray(np.asarray(p2, dtype=np.float64))
+        out = np.zeros(11)
+        r = lib().orc_mads(self.h, x.ctypes.data_as(_dp), p.ctypes.data_as(_dp), nod, limit,
+                           out.ctypes.data_as(_dp))
+        return x, out, r
+
     def spindown_range(self, mm, nn, rng, opts):
         lo, hi = C.c_int(), C.c_int()
         lib().orc_spindown_range(C.byref(self.sett), mm, nn, C.byref(rng), C.byref(opts),
                                  C.byref(lo), C.byref(hi))
         return lo.value, hi.value
+
+
+def mads_stencil() -> np.ndarray:
+    """The oracle's 80 mesh offsets (signs) in the reference's order."""
+    st = np.zeros((80, 4), np.int8)
+    lib().orc_mads_stencil(st.ctypes.data_as(C.POINTER(C.c_byte)))
+    return st
 
 
 def default_threads() -> int:

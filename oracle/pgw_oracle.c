@@ -758,3 +758,71 @@ int orc_amoeba(const orc_search *s, double point[4], const double pc2[4], double
   memcpy(out, fx[ilo], sizeof(double) * 11);
   return nev;
 }
+
+/* The 80 points of the 3^4 mesh around the centre as sign vectors (+1, 0, -1) x the mesh sizes of
+ * (omega, omegadot, delta, alpha), in the order invertedMADS fills array[1..80]
+ * (followup.c:573-672): the 16 vertices (every parameter moved; omega slowest, + before -), the
+ * 32 edges (one parameter kept, in the order omega, omegadot, delta, alpha; the other three in the
+ * vertices' order), the 24 faces (two kept) and the 8 cells (one moved: omegadot, alpha, delta,
+ * omega).  The order decides only exact ties (the first strict minimum wins here; the reference's
+ * OpenMP loop leaves them to its schedule). */
+static const signed char mads_faces[24][4] = {
+  {0, 0, 1, 1}, {0, 0, 1, -1}, {0, 0, -1, 1}, {0, 0, -1, -1},
+  {0, 1, 0, 1}, {0, 1, 0, -1}, {0, 1, 1, 0}, {0, 1, -1, 0},
+  {0, -1, 0, 1}, {0, -1, 0, -1}, {0, -1, 1, 0}, {0, -1, -1, 0},
+  {1, 0, 0, 1}, {1, 0, 0, -1}, {1, 0, 1, 0}, {1, 0, -1, 0}, {1, 1, 0, 0}, {1, -1, 0, 0},
+  {-1, 0, 0, 1}, {-1, 0, 0, -1}, {-1, 0, 1, 0}, {-1, 0, -1, 0}, {-1, 1, 0, 0}, {-1, -1, 0, 0}};
+static const signed char mads_cells[8][4] = {
+  {0, 1, 0, 0}, {0, -1, 0, 0}, {0, 0, 0, 1}, {0, 0, 0, -1}, {0, 0, 1, 0}, {0, 0, -1, 0}, {1, 0, 0, 0}, {-1, 0, 0, 0}};
+
+void orc_mads_stencil(signed char st[80][4]) {
+  int j = 0;
+  for (int v = 0; v < 16; v++, j++)          /* bit 3 omega ... bit 0 alpha; 0 = +, 1 = - */
+    for (int k = 0; k < 4; k++) st[j][k] = (v >> (3 - k)) & 1 ? -1 : 1;
+  for (int z = 0; z < 4; z++)                /* edges: parameter z kept */
+    for (int v = 0; v < 8; v++, j++) {
+      int b = 2;
+      for (int k = 0; k < 4; k++) st[j][k] = k == z ? 0 : ((v >> b--) & 1 ? -1 : 1);
+    }
+  for (int f = 0; f < 24; f++, j++) memcpy(st[j], mads_faces[f], 4);
+  for (int c = 0; c < 8; c++, j++) memcpy(st[j], mads_cells[c], 4);
+}
+
+/* invertedMADS, followup.c:537-754, from point with final mesh sizes p2 over a segment of nod
+ * days: mesh sizes start at (1 - scale) p2 with scale = 1 - 0.06 / nod; each round evaluates the
+ * centre and its 80 mesh points; a point with strictly lower -F than the centre (the first such
+ * minimum in mesh order) becomes the centre and the mesh restarts, otherwise every mesh size grows
+ * by (1 - scale) p2; it ends when every size exceeds its p2, or after `limit` rounds.  point and
+ * out[11] receive the final centre and its Fstatnet values; returns the rounds. */
+int orc_mads(const orc_search *s, double point[4], const double p2[4], double nod, int limit, double out[11]) {
+  signed char st[80][4];
+  orc_mads_stencil(st);
+  const double scale = 1.0 - (0.01 * 6.0 / nod);
+  double param[4], start[4], centre[4], mesh[80][4], res[80][11], cur[11];
+  for (int r = 0; r < 4; r++) start[r] = param[r] = (1.0 - scale) * p2[r];
+  memcpy(centre, point, sizeof centre);
+  int rounds = 0;
+  while (param[0] <= p2[0] || param[1] <= p2[1] || param[2] <= p2[2] || param[3] <= p2[3]) {
+    rounds++;
+    orc_fstatnet(s, centre, cur, NULL);
+    for (int j = 0; j < 80; j++)
+      for (int k = 0; k < 4; k++) mesh[j][k] = centre[k] + st[j][k] * param[k];
+#pragma omp parallel for schedule(static) num_threads(s->nthreads)
+    for (int j = 0; j < 80; j++) orc_fstatnet(s, mesh[j], res[j], NULL);
+    int best = -1;
+    double fbest = cur[5];
+    for (int j = 0; j < 80; j++)
+      if (res[j][5] < fbest) { fbest = res[j][5]; best = j; }
+    if (best < 0) {
+      for (int r = 0; r < 4; r++) param[r] += (1.0 - scale) * p2[r];
+    } else {
+      memcpy(centre, mesh[best], sizeof centre);
+      for (int r = 0; r < 4; r++) param[r] = start[r];
+    }
+    if (rounds >= limit) break;
+  }
+  orc_fstatnet(s, centre, out, NULL);
+  memcpy(point, centre, sizeof centre);
+  return rounds;
+}
+

@@ -17,7 +17,9 @@
  *   fstatnet - noise-free injection (F = rho^2/2 + 2 at the true point, amplitude
  *     estimates equal the injected amplitudes), agreement with the grid search's F
  *     at the grid candidate, gradient against central differences, phase against
- *     an mpmath evaluation; amoeba - converges to the injected point.
+ *     an mpmath evaluation; amoeba - converges to the injected point; mads - its
+ *     mesh order against the reference's array (hand-derived), its trajectory against the
+ *     product's batched mesh search and an improvement from every start.
  */
 #ifndef PGW_ORACLE_H
 #define PGW_ORACLE_H
@@ -91,6 +93,10 @@ void orc_spindown_range(const pgw_settings *sett, int mm, int nn, const pgw_rang
 int orc_fstatnet(const orc_search *s, const double pt[4], double out[11], double grad[4]);
 /* n points pts[4n] -> out[11n], grad[4n] (optional); OpenMP over points */
 int orc_fstatnet_batch(const orc_search *s, const double *pts, int n, double *out, double *grad);
+/* invertedMADS (followup.c:537-754): mesh search from point with final mesh sizes p2 */
+int orc_mads(const orc_search *s, double point[4], const double p2[4], double nod, int limit, double out[11]);
+/* its 80 mesh offsets (signs) in the reference's array order */
+void orc_mads_stencil(signed char st[80][4]);
 /* followup's Nelder-Mead (amoeba, followup.c:971-1173) on -F from point with initial steps pc2 */
 int orc_amoeba(const orc_search *s, double point[4], const double pc2[4], double tol, int maxeval,
                double out[11]);

@@ -319,13 +319,29 @@ def _last_ilo(f):
     return ilo
 
 
-# The 80 points of the 3^4 mesh around the centre as multiples (-1, 0, +1) of the mesh sizes,
-# grouped as in the reference (followup/src/followup.c:576-683): vertices (all four parameters
-# moved), edges (three), faces (two), cells (one).  The order inside the list only decides exact
-# ties, which the reference leaves to its OpenMP schedule.
+# The 80 points of the 3^4 mesh around the centre as multiples (+1, 0, -1) of the mesh sizes of
+# (omega, omegadot, delta, alpha), in the order the reference fills its mesh array
+# (followup/src/followup.c:573-672): 16 vertices (all four moved; omega slowest, + before -), 32
+# edges (one kept, in parameter order; the rest in the vertices' order), 24 faces (two kept) and 8
+# cells (one moved: omegadot, alpha, delta, omega).  The order decides only exact ties: the first
+# strict minimum wins (the reference's OpenMP loop leaves ties to its schedule).
 def _mads_stencil() -> np.ndarray:
-    pts = [p for p in itertools.product((1.0, 0.0, -1.0), repeat=4) if any(p)]
-    return np.array(sorted(pts, key=lambda p: -sum(v != 0.0 for v in p)))
+    vert = list(itertools.product((1.0, -1.0), repeat=4))
+    edges = []
+    for z in range(4):
+        for v in itertools.product((1.0, -1.0), repeat=3):
+            p = list(v)
+            p.insert(z, 0.0)
+            edges.append(tuple(p))
+    faces = [(0, 0, d, a) for d in (1, -1) for a in (1, -1)]
+    for f in (0, 1, -1):
+        for s_ in ((1, -1) if f == 0 else (0,)):
+            faces += [(f, s_, 0, 1), (f, s_, 0, -1), (f, s_, 1, 0), (f, s_, -1, 0)]
+        if f != 0:
+            faces += [(f, 1, 0, 0), (f, -1, 0, 0)]
+    cells = [(0, 1, 0, 0), (0, -1, 0, 0), (0, 0, 0, 1), (0, 0, 0, -1), (0, 0, 1, 0), (0, 0, -1, 0),
+             (1, 0, 0, 0), (-1, 0, 0, 0)]
+    return np.array(vert + edges + faces + cells, dtype=np.float64)
 
 
 MADS_STENCIL = _mads_stencil()

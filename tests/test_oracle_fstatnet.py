@@ -149,46 +149,54 @@ def test_batched_amoeba_equals_reference_steps(orc):
     assert not conv2.any() and np.all(nev2 >= 12)
 
 
-def _mads_reference_steps(S, start, p2, nod, limit=700):
-    """invertedMADS (followup.c:537-754) one candidate and one mesh point at a time, with the
-    oracle's Fstatnet: the sequential form the batched driver must reproduce."""
+def test_mads_mesh_order_is_the_references(orc):
+    """The oracle's 80 mesh offsets against entries read off the reference's array assignments
+    (followup.c:573-672; index j of array[j], 1-based), each of the 80 nonzero sign vectors once,
+    and the product's stencil in the same order."""
     from polgraw_allsky_amd import followup
-    step = (0.01 * 6.0 / nod) * np.asarray(p2, dtype=np.float64)
-    param = step.copy()
-    x = np.asarray(start, dtype=np.float64).copy()
-    best = S.fstatnet(x[None, :], grad=False)[0][0]
-    count = 0
-    while np.any(param <= p2) and count < limit:
-        count += 1
-        maxres = best.copy()
-        arg = None
-        for d in followup.MADS_STENCIL:
-            p = x + d * param
-            res = S.fstatnet(p[None, :], grad=False)[0][0]
-            if res[5] < maxres[5]:
-                maxres, arg = res, p
-        if arg is None:                  # the centre is the best: grow the mesh
-            param = param + step
-        else:                            # move to the better point, restart the mesh
-            x, best, param = arg, maxres, step.copy()
-    return x, best, count
+    st = orc.mads_stencil()
+    hand = {1: (1, 1, 1, 1), 2: (1, 1, 1, -1), 9: (-1, 1, 1, 1), 16: (-1, -1, -1, -1),
+            17: (0, 1, 1, 1), 24: (0, -1, -1, -1), 25: (1, 0, 1, 1), 33: (1, 1, 0, 1), 41: (1, 1, 1, 0),
+            48: (-1, -1, -1, 0), 49: (0, 0, 1, 1), 53: (0, 1, 0, 1), 55: (0, 1, 1, 0), 60: (0, -1, -1, 0),
+            61: (1, 0, 0, 1), 63: (1, 0, 1, 0), 65: (1, 1, 0, 0), 66: (1, -1, 0, 0), 70: (-1, 0, -1, 0),
+            72: (-1, -1, 0, 0), 73: (0, 1, 0, 0), 75: (0, 0, 0, 1), 77: (0, 0, 1, 0), 79: (1, 0, 0, 0),
+            80: (-1, 0, 0, 0)}
+    for j, v in hand.items():
+        assert tuple(st[j - 1]) == v, (j, st[j - 1], v)
+    assert len({tuple(r) for r in st}) == 80 and not np.any(np.all(st == 0, axis=1))
+    assert np.array_equal(st.astype(np.float64), followup.MADS_STENCIL)
 
 
-def test_batched_mads_equals_reference_steps(orc):
-    """followup.mads (batched over candidates and mesh points) against the mesh search one point
-    at a time: identical points, outputs and rounds; the end point improves on the start."""
+def test_batched_mads_equals_oracle_mads(orc):
+    """followup.mads (batched over candidates and mesh points, fed by the oracle's Fstatnet) against
+    the oracle's invertedMADS (orc_mads): identical final points, outputs and rounds; the end point
+    improves on the start."""
     from polgraw_allsky_amd import followup
     seg, truth, amp, rho2 = noise_free_signal(N=8192, seed=8, amp=[2.8, 0.0, -1.6, 0.0])
     S = orc.Search(seg, nthreads=2)
     sc = phase_scales(seg, truth)
     nod = seg.sett["N"] * seg.sett["dt"] / 86400.0
-    starts = truth + np.random.default_rng(6).uniform(-0.6, 0.6, (2, 4)) * sc
+    starts = truth + np.random.default_rng(6).uniform(-0.6, 0.6, (3, 4)) * sc
     p2 = 0.5 * sc  # a coarse mesh limit relative to the short segment keeps the search short
     x, out, rounds = followup.mads(_OracleFS(S), starts, p2, nod)
     assert np.all(rounds < 700)
     for k in range(len(starts)):
-        xr, outr, cr = _mads_reference_steps(S, starts[k], p2, nod)
+        xr, outr, cr = S.mads(starts[k], p2, nod)
+        print("start", k, "rounds", rounds[k], cr, "F", -out[k, 5], -outr[5], flush=True)
         assert rounds[k] == cr
         assert np.array_equal(x[k], xr) and np.array_equal(out[k], outr, equal_nan=True)
         F0 = -S.fstatnet(starts[k][None, :], grad=False)[0][0, 5]
         assert -out[k, 5] > F0
+
+
+def test_oracle_mads_round_limit(orc):
+    """The round limit ends the search (followup.c:740-742): with mesh steps of 6e-7 of the mesh
+    limit (nod = 1e5 days) the oracle stops after exactly `limit` rounds, as the batched driver does."""
+    from polgraw_allsky_amd import followup
+    seg, truth, amp, rho2 = noise_free_signal(N=8192, seed=8, amp=[2.8, 0.0, -1.6, 0.0])
+    S = orc.Search(seg, nthreads=2)
+    sc = phase_scales(seg, truth)
+    x, out, r = S.mads(truth + 0.1 * sc, 0.5 * sc, 1e5, limit=5)
+    xb, outb, rb = followup.mads(_OracleFS(S), (truth + 0.1 * sc)[None, :], 0.5 * sc, 1e5, limit=5)
+    assert r == 5 and rb[0] == 5
+    assert np.array_equal(x, xb[0]) and np.array_equal(out, outb[0])
