@@ -394,39 +394,12 @@ def bench_batching(args):
     return line
 
 
-def differentiation_setup(seed, nseg=16, per_seg=64):
-    """1024 injections (h0 in [0.05, 0.3]) in the 2-detector N=86400 setup: nseg noise realisations
-    of H1 and L1 (sigma 1 and 1.3, fftpad 2) with per_seg signals each at separated frequencies.
-    The candidate point of each injection is drawn uniformly in the grid cell around the truth
-    (grid coordinates + U(-0.5, 0.5)^4), where a search's grid would leave it."""
-    from polgraw_allsky_amd import followup
-    segs = [synth.make_injections(N=86400, dets=("H1", "L1"), sigmas=(1.0, 1.3), fftpad=2, seed=seed + 101 + k,
-                                  nsig=per_seg) for k in range(nseg)]
-    rs = np.random.default_rng(seed)
-    truth, starts, cells, diags, seg_of, h0 = [], [], [], [], [], []
-    for k, sg in enumerate(segs):
-        oms = sg.sett["oms"]
-        for j in sg.injections:
-            t = np.array([j["f"], j["fdot"], j["delta"], j["alpha"]])
-            pm, g = followup.grid_point(sg.M, oms, t, sg.epsm)
-            for _ in range(100):
-                try:
-                    x = followup.point_of_grid(sg.M, oms, g + rs.uniform(-0.5, 0.5, 4), pm, sg.epsm)
-                    break
-                except ValueError:  # the offset left the sky disk: draw again
-                    continue
-            truth.append(t)
-            starts.append(x)
-            cells.append(followup.cell_width(sg.M, oms, x, sg.epsm))
-            diags.append(followup.cell_diagonal(sg.M, oms, x, sg.epsm))
-            seg_of.append(k)
-            h0.append(j["h0"])
-    return (segs, np.array(truth), np.array(starts), np.array(cells), np.array(diags), np.array(seg_of),
-            np.array(h0))
+def differentiation_setup(seed, nseg=16, per_seg=64, mismatch=0.25):
+    return synth.differentiation_workload(seed=seed, nseg=nseg, per_seg=per_seg, mismatch=mismatch)
 
 
 def param_errors(x, truth, cells):
-    """|x - truth| per parameter in grid-cell widths (right ascension wrapped to (-pi, pi])."""
+    """|x - truth| per parameter in cells (right ascension wrapped to (-pi, pi])."""
     d = x - truth
     d[:, 3] = (d[:, 3] + np.pi) % (2 * np.pi) - np.pi
     return np.abs(d) / cells
@@ -447,7 +420,7 @@ def bench_differentiation(args, steps=None, warmup=None, cpu=True, mads_points=6
     steps = (args.steps or 20) if steps is None else steps
     warmup = args.warmup if warmup is None else warmup
     t_setup = time.perf_counter()
-    segs, truth, pts, cells, diags, seg_of, h0 = differentiation_setup(args.seed)
+    segs, truth, pts, cells, seg_of, h0 = differentiation_setup(args.seed)
     npts = len(pts)
     print(f"differentiation: {len(segs)} segments x {npts // len(segs)} injections ({time.perf_counter() - t_setup:.1f} s "
           f"setup)", file=sys.stderr, flush=True)
@@ -486,7 +459,7 @@ def bench_differentiation(args, steps=None, warmup=None, cpu=True, mads_points=6
     t_asc = time.perf_counter() - t1
     # the followup's simplex from the same points
     t3 = time.perf_counter()
-    xs, outs, nev_s, conv_s = followup.amoeba(fs, pts, 2.0 * diags, tol=1e-7)
+    xs, outs, nev_s, conv_s = followup.amoeba(fs, pts, 2.0 * cells, tol=1e-7)
     t_nm = time.perf_counter() - t3
     Fs = -outs[:, 5]
     print(f"ascent {t_asc:.2f} s, simplex {t_nm:.2f} s ({int(nev_s.sum())} evaluations)", file=sys.stderr, flush=True)
@@ -494,7 +467,7 @@ def bench_differentiation(args, steps=None, warmup=None, cpu=True, mads_points=6
     nm = min(mads_points, npts)
     nod = 86400 * 2.0 / 86400.0
     t4 = time.perf_counter()
-    xm, outm, rounds_m = followup.mads(followup.MultiFStatNet(h, seg_of[:nm]), pts[:nm], 2.0 * diags[:nm], nod)
+    xm, outm, rounds_m = followup.mads(followup.MultiFStatNet(h, seg_of[:nm]), pts[:nm], 2.0 * cells[:nm], nod)
     t_mads = time.perf_counter() - t4
     e0, ea, es, em = (param_errors(x, t, c) for x, t, c in ((pts, truth, cells), (xa, truth, cells),
                                                              (xs, truth, cells), (xm, truth[:nm], cells[:nm])))
@@ -534,19 +507,21 @@ def bench_differentiation(args, steps=None, warmup=None, cpu=True, mads_points=6
                   frac=round(ach / 78.6, 4), traffic=None, flops_per_sample=fps)
     cfg = {"workload": "differentiation: 2 detectors (H1, L1), N=86400, fftpad 2, 1024 injections (16 noise "
                        "realisations x 64, h0 in [0.05, 0.3] sigma), Fstatnet + dF/d(omega, omegadot, delta, alpha) "
-                       "at one candidate point in the grid cell of each", "N": N, "detectors": nifo, "points": npts}
+                       "at one candidate point within phase-metric mismatch 0.25 of each",
+           "cell": "1 / sqrt(G_kk) of the phase metric G at the injection: one radian rms of phase per parameter",
+           "N": N, "detectors": nifo, "points": npts}
     line = base_line(args, v, dt, cfg, dtype="fp64", unit="points/s", steps=steps, warmup=warmup,
                      metric="Fstatnet evaluations with gradient per second on 1 MI355X")
     line.update({
         "values_only_points_per_s": round(v_nograd, 1), "samples_per_s": round(v * nifo * N, 1),
         "gradient_check": {
-            "fd_step": "1e-3 grid cell per parameter, central differences",
+            "fd_step": "1e-3 cell per parameter, central differences",
             "max_abs_grad_minus_fd_per_cell_over_F": float(fd_err.max()),
             "median_abs_grad_minus_fd_per_cell_over_F": float(np.median(fd_err)),
             "vs_fp64_oracle": orc_grad},
         "ascent": {"steps": 20, "evaluations_per_point": nev, "seconds": round(t_asc, 3),
                    "F_median": float(np.median(Fa)), "param_error": err_summary(ea)},
-        "simplex": {"tol": 1e-7, "first_steps": "twice the grid cell's diagonal", "evaluations_per_point_mean":
+        "simplex": {"tol": 1e-7, "first_steps": "two cells", "evaluations_per_point_mean":
                     round(float(nev_s.mean()), 1), "converged": int(conv_s.sum()), "seconds": round(t_nm, 3),
                     "F_median": float(np.median(Fs)), "param_error": err_summary(es)},
         "ascent_vs_simplex": {"F_rel_diff_median": float(np.median(rel)), "F_rel_diff_min": float(rel.min()),

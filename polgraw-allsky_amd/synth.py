@@ -349,3 +349,38 @@ def batching_segments(seed: int = 2024, nseg: int = 64) -> list:
                                  nsky=(4, 8), nspin=512))
     return segs
 
+
+def differentiation_workload(seed: int = 2024, nseg: int = 16, per_seg: int = 64, mismatch: float = 0.25):
+    """The differentiation configuration: nseg * per_seg injections (h0 in [0.05, 0.3]) in the
+    2-detector N=86400 setup, as nseg noise realisations of H1 and L1 (sigma 1 and 1.3, fftpad 2)
+    with per_seg signals each at separated frequencies (make_injections).
+
+    Cells come from the phase metric G at the injection (followup.phase_metric): the cell along
+    parameter k is 1 / sqrt(G_kk), the step that moves the phase by one radian rms, the spacing of
+    a covering grid.  (The grid matrix of make_segment is scaled from the reference's test grid;
+    its sky vectors do not follow this ephemeris's Doppler direction, so its cells are far wider
+    than an F peak.)  The candidate point of each injection is drawn uniformly inside the ellipsoid
+    d^T G d <= mismatch around the truth, where a grid search of minimal match 1 - mismatch leaves
+    its best template.  Returns (segments, truth (n, 4), candidates (n, 4), cells (n, 4), segment
+    of each (n,), h0 (n,))."""
+    from .followup import phase_metric
+    segs = [make_injections(N=86400, dets=("H1", "L1"), sigmas=(1.0, 1.3), fftpad=2, seed=seed + 101 + k,
+                            nsig=per_seg) for k in range(nseg)]
+    rs = np.random.default_rng(seed)
+    truth, starts, cells, seg_of, h0 = [], [], [], [], []
+    for k, sg in enumerate(segs):
+        oms, N = sg.sett["oms"], sg.sett["N"]
+        for j in sg.injections:
+            t = np.array([j["f"], j["fdot"], j["delta"], j["alpha"]])
+            G = phase_metric(sg.DetSSB, N, oms, t)
+            G = G + 1e-9 * np.trace(G) * np.eye(4)
+            z = rs.standard_normal(4)
+            z *= math.sqrt(mismatch) * rs.uniform() ** 0.25 / np.linalg.norm(z)
+            d = np.linalg.solve(np.linalg.cholesky(G).T, z)          # d^T G d = |z|^2
+            truth.append(t)
+            starts.append(t + d)
+            cells.append(1.0 / np.sqrt(np.diag(G)))
+            seg_of.append(k)
+            h0.append(j["h0"])
+    return segs, np.array(truth), np.array(starts), np.array(cells), np.array(seg_of), np.array(h0)
+

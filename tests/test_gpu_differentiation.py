@@ -1,0 +1,80 @@
+"""The differentiation configuration on 64 of its 1024 injections (one noise realisation of H1 and
+L1, N=86400, fftpad 2, h0 in [0.05, 0.3]): the device's analytic dF/d(omega, omegadot, delta,
+alpha) at the candidate points against central differences of the device's F (step 1e-3 cell per
+parameter; a cell is the one-radian step of the phase metric) and against the FP64 oracle's
+gradient; then the 20-step gradient ascent against the followup's simplex (its tolerance 1e-7,
+first steps of two cells; followup/src/followup.c:1126-1173,1192) on recovered F and on the
+parameter error against the injected truth."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def work():
+    from polgraw_allsky_amd import synth
+    return synth.differentiation_workload(seed=2024, nseg=1, per_seg=64)
+
+
+@pytest.fixture(scope="module")
+def fs(work, abi):
+    from polgraw_allsky_amd import followup, pgw
+    segs, truth, pts, cells, seg_of, h0 = work
+    h = pgw.Handle(segs, abi.PREC_FP64)
+    yield followup.MultiFStatNet(h, seg_of)
+    h.close()
+
+
+def _errors(x, truth, cells):
+    d = x - truth
+    d[:, 3] = (d[:, 3] + np.pi) % (2 * np.pi) - np.pi
+    return np.abs(d) / cells
+
+
+def test_gradient_vs_differences_and_oracle(work, fs, orc):
+    segs, truth, pts, cells, seg_of, h0 = work
+    out, g = fs(pts)
+    F0 = -out[:, 5]
+    fd = np.zeros_like(g)
+    for k in range(4):
+        e = np.zeros(4)
+        e[k] = 1.0
+        hk = 1e-3 * cells[:, k]
+        fd[:, k] = (-fs(pts + hk[:, None] * e, grad=False)[0][:, 5] + fs(pts - hk[:, None] * e, grad=False)[0][:, 5]) \
+            / (2 * hk)
+    err_fd = np.abs(g - fd) * cells / F0[:, None]
+    out_o, g_o = orc.Search(segs[0], nthreads=8).fstatnet(pts)
+    rel_F = np.abs(out_o[:, 5] - out[:, 5]) / np.abs(out_o[:, 5])
+    err_o = np.abs(g - g_o) * cells / F0[:, None]
+    print("F start median", np.median(F0), "max |g - fd| cell / F", err_fd.max(), "vs oracle: rel dF", rel_F.max(),
+          "|g - g_oracle| cell / F", err_o.max(), flush=True)
+    # per cell and relative to F: differences of a step of 1e-3 cell are good to ~1e-7
+    assert err_fd.max() <= 1e-5
+    assert rel_F.max() <= 1e-10
+    assert err_o.max() <= 1e-8
+
+
+def test_ascent_vs_simplex(work, fs):
+    from polgraw_allsky_amd import followup
+    segs, truth, pts, cells, seg_of, h0 = work
+    F0 = -fs(pts, grad=False)[0][:, 5]
+    Ft = -fs(truth, grad=False)[0][:, 5]
+    G = np.array([followup.phase_metric(segs[0].DetSSB, 86400, segs[0].sett["oms"], p) for p in pts])
+    xa, Fa, nev = followup.ascend(fs, pts, G, steps=20)
+    xs, outs, nev_s, conv = followup.amoeba(fs, pts, 2.0 * cells, tol=1e-7)
+    Fs = -outs[:, 5]
+    e0, ea, es = (_errors(x, truth, cells) for x in (pts, xa, xs))
+    rel = (Fa - Fs) / Fs
+    print("F median: truth", np.median(Ft), "start", np.median(F0), "ascent", np.median(Fa), "simplex",
+          np.median(Fs), "| ascent evaluations", nev, "simplex mean", nev_s.mean(), flush=True)
+    print("ascent - simplex relative F: median", np.median(rel), "min", rel.min(),
+          "ascent within 1e-3 of simplex:", int(np.sum(rel >= -1e-3)), "of", len(rel), flush=True)
+    print("median error (cells) start", np.median(e0, axis=0), "ascent", np.median(ea, axis=0), "simplex",
+          np.median(es, axis=0), flush=True)
+    assert conv.all()
+    # both searches end at least at their starting F and near the maximum of F, which noise moves
+    # away from the injected truth by a fraction of a cell
+    assert np.all(Fa >= F0) and np.all(Fs >= F0)
+    assert np.sum(rel >= -1e-3) >= 0.9 * len(rel)
+    assert np.all(np.median(ea, axis=0) <= 0.5) and np.all(np.median(es, axis=0) <= 0.5)
