@@ -41,6 +41,11 @@ extern "C" {
 #define PGW_ERR_UNSUPPORTED (-5)/* size outside what the kernels are built for */
 #define PGW_ERR_NODEVICE (-6)   /* no HIP device visible */
 #define PGW_ERR_IO (-7)         /* a trigger or state file could not be written or read */
+#define PGW_ERR_SCRATCH (-8)    /* input larger than the allocation-free entry point's scratch */
+
+/* Device scratch of pgw_fstatnet_noalloc, per GPU: 8 MiB, i.e. npts * ceil(N / 4096) <= 40329
+ * (1833 points at N = 86400, 630 at N = 259200). */
+#define PGW_FSN_SCRATCH_BYTES (8u << 20)
 
 /* arithmetic of the modulated series and the FFTs; phases are FP64 in both */
 #define PGW_PREC_FP64 0
@@ -174,6 +179,17 @@ int pgw_job_batch_multi(pgw_handle *h, const int *work, int nwork, pgw_range *s_
  * F = -out[5] (the analytic gradient; no counterpart in the reference, whose followup searches with
  * Nelder-Mead and MADS, followup.c:537-1173).  FP64 on the device. */
 int pgw_fstatnet(pgw_handle *h, int seg, const double *pts, int npts, double *out, double *grad);
+
+/* Allocation-free pgw_fstatnet.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  pts (4 * npts), out (11 * npts) and grad (4 * npts, optional) are DEVICE addresses
+ * owned by the caller, with pgw_fstatnet's layouts and values.  The partial sums live in a fixed
+ * device array of PGW_FSN_SCRATCH_BYTES per GPU, set up when the handle is created: no hipMalloc,
+ * hipFree or allocating library call happens here.  npts * ceil(N / 4096) * 26 doubles must fit
+ * (else PGW_ERR_SCRATCH, before any device work).  The kernels run on the handle's stream and the
+ * call returns when they are done.  All handles and calls on one GPU share the scratch: calls
+ * that overlap in time on one GPU (other host threads) race and must be serialised by the caller. */
+int pgw_fstatnet_noalloc(pgw_handle *h, int seg, const double *pts, int npts, double *out, double *grad);
 
 /* Autotuner (no counterpart in the reference): times a small search around sky point
  * (pm, mm, nn) of segment `seg` (which must lie inside the sky disk; opts->trl is not used) for

@@ -17,6 +17,7 @@ PGW_ERR_NOMEM = -4
 PGW_ERR_UNSUPPORTED = -5
 PGW_ERR_NODEVICE = -6
 PGW_ERR_IO = -7
+PGW_ERR_SCRATCH = -8
 
 PREC_FP64 = 0
 PREC_FP32 = 1

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -46,6 +47,7 @@ constexpr int kCandChunk = 4096;    // candidates copied back with the count, wi
 constexpr int kSlots = 16;          // sky points per group
 constexpr int kDescCap = 1 << 16;   // templates per group
 constexpr int kTBMax = 256;         // templates per launch, upper bound
+constexpr size_t kScratchDoubles = PGW_FSN_SCRATCH_BYTES / sizeof(double);
 
 int ilog2(int v) {
   int l = 0;
@@ -265,6 +267,34 @@ void lin2ast(double be1, double be2, int pm, double sepsm, double cepsm, double*
 
 }  // namespace
 
+// Scratch of the allocation-free Fstatnet (pgw_fstatnet_noalloc): the (point, chunk) partial
+// sums.  Every GPU the library runs on holds one copy until the process ends; all handles and
+// calls on a GPU share it.  Host code reaches it only through the device address that
+// scratch_address() obtains per device, never through the array's name.
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr;  // device -> address of that device's copy of `scratch`
+
+// The address of the current device's `scratch`, looked up once per device.
+int scratch_address(double** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return PGW_ERR_DEVICE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return PGW_ERR_DEVICE;
+    it = g_scratch_ptr.emplace(dev, static_cast<double*>(p)).first;
+  }
+  *out = it->second;
+  return PGW_OK;
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------- handle --
 struct pgw_handle {
   virtual ~pgw_handle() {}
@@ -279,6 +309,7 @@ struct pgw_handle {
   virtual int autotune(int seg, int pm, int mm, int nn, const pgw_opts* o, int* tb, int* row_na, int* col_seq) = 0;
   virtual int kernel_times(int enable, double* out) = 0;
   virtual int fstatnet(int seg, const double* pts, int n, double* out, double* grad) = 0;
+  virtual int fstatnet_noalloc(int seg, const double* d_pts, int n, double* d_out, double* d_grad) = 0;
   virtual int settings(int seg, pgw_settings* out) const = 0;
   double last_ms = 0.0;
 };
@@ -526,6 +557,8 @@ template <typename T> struct Engine final : pgw_handle {
       if (const char* q = strstr(tune, "candcap=")) if (sscanf(q, "candcap=%d", &v) == 1 && v >= 1 && v <= kCandCap) cand_cap = v;
     }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    // this device's copy of the allocation-free Fstatnet's scratch (the device is current here)
+    if ((st = scratch_address(&fsn_scratch))) return st;
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
     HIPCHK(hipEventCreateWithFlags(&evd[0], hipEventDisableTiming));
@@ -975,10 +1008,9 @@ template <typename T> struct Engine final : pgw_handle {
   }
 
   DevBuf<double> fsn_pts, fsn_part, fsn_out, fsn_grad;
-  int fstatnet(int sgi, const double* pts, int n, double* out, double* grad) override {
-    constexpr int kFsnBatch = 4096;
-    if (sgi < 0 || sgi >= (int)segs.size() || n < 0 || (n && (!pts || !out))) return PGW_ERR_ARG;
-    const Seg& g = *segs[sgi];
+  double* fsn_scratch = nullptr;  // this device's `scratch` (pgw_fstatnet_noalloc)
+
+  FsnParams fsn_params(const Seg& g) const {
     FsnParams q{};
     q.N = g.sett.N;
     q.nifo = nifo;
@@ -990,6 +1022,29 @@ template <typename T> struct Engine final : pgw_handle {
       q.det[d].cphir = g.cphir[d];
       q.det[d].w = 1.0 / g.sig2[d];
     }
+    return q;
+  }
+
+  // the two Fstatnet kernels for m points at device address pts, partial sums in part
+  // (m * nchunk * FSN_NSUM doubles), results to device addresses out, grad (optional)
+  int fsn_enqueue(const Seg& g, const FsnParams& q, const double* pts, int m, double* part, double* out,
+                  double* grad) {
+    if (grad)
+      hipLaunchKernelGGL(k_fstatnet_part<true>, dim3(m, q.nchunk), dim3(FSN_NT), 0, stream, q, pts, g.xDat.p,
+                         g.DetSSB.p, g.cosmodf.p, g.sinmodf.p, part);
+    else
+      hipLaunchKernelGGL(k_fstatnet_part<false>, dim3(m, q.nchunk), dim3(FSN_NT), 0, stream, q, pts, g.xDat.p,
+                         g.DetSSB.p, g.cosmodf.p, g.sinmodf.p, part);
+    hipLaunchKernelGGL(k_fstatnet_final, dim3((m + 63) / 64), dim3(64), 0, stream, m, q.nchunk, part, pts, out,
+                       grad);
+    return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
+  }
+
+  int fstatnet(int sgi, const double* pts, int n, double* out, double* grad) override {
+    constexpr int kFsnBatch = 4096;
+    if (sgi < 0 || sgi >= (int)segs.size() || n < 0 || (n && (!pts || !out))) return PGW_ERR_ARG;
+    const Seg& g = *segs[sgi];
+    const FsnParams q = fsn_params(g);
     const int nb = std::min(n, kFsnBatch);
     int st;
     if (nb > 0 && ((st = fsn_pts.alloc_min((size_t)4 * nb)) || (st = fsn_part.alloc_min((size_t)nb * q.nchunk * FSN_NSUM)) ||
@@ -999,21 +1054,34 @@ template <typename T> struct Engine final : pgw_handle {
     for (int p0 = 0; p0 < n; p0 += kFsnBatch) {
       const int m = std::min(kFsnBatch, n - p0);
       HIPCHK(hipMemcpyAsync(fsn_pts.p, pts + 4 * (size_t)p0, sizeof(double) * 4 * m, hipMemcpyHostToDevice, stream));
-      if (grad)
-        hipLaunchKernelGGL(k_fstatnet_part<true>, dim3(m, q.nchunk), dim3(FSN_NT), 0, stream, q, fsn_pts.p,
-                           g.xDat.p, g.DetSSB.p, g.cosmodf.p, g.sinmodf.p, fsn_part.p);
-      else
-        hipLaunchKernelGGL(k_fstatnet_part<false>, dim3(m, q.nchunk), dim3(FSN_NT), 0, stream, q, fsn_pts.p,
-                           g.xDat.p, g.DetSSB.p, g.cosmodf.p, g.sinmodf.p, fsn_part.p);
-      hipLaunchKernelGGL(k_fstatnet_final, dim3((m + 63) / 64), dim3(64), 0, stream, m, q.nchunk, fsn_part.p,
-                         fsn_pts.p, fsn_out.p, grad ? fsn_grad.p : nullptr);
-      if (hipGetLastError() != hipSuccess) return PGW_ERR_DEVICE;
+      if ((st = fsn_enqueue(g, q, fsn_pts.p, m, fsn_part.p, fsn_out.p, grad ? fsn_grad.p : nullptr))) return st;
       HIPCHK(hipMemcpyAsync(out + 11 * (size_t)p0, fsn_out.p, sizeof(double) * 11 * m, hipMemcpyDeviceToHost, stream));
       if (grad)
         HIPCHK(hipMemcpyAsync(grad + 4 * (size_t)p0, fsn_grad.p, sizeof(double) * 4 * m, hipMemcpyDeviceToHost,
                               stream));
       HIPCHK(hipStreamSynchronize(stream));  // the next batch reuses the device buffers
     }
+    HIPCHK(hipEventRecord(ev1, stream));
+    HIPCHK(hipEventSynchronize(ev1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, ev0, ev1));
+    last_ms = t;
+    return PGW_OK;
+  }
+
+  // Fstatnet on caller-owned device buffers with the partial sums in this device's `scratch`: no
+  // allocation, one batch, synchronous on the handle's stream.
+  int fstatnet_noalloc(int sgi, const double* d_pts, int n, double* d_out, double* d_grad) override {
+    if (sgi < 0 || sgi >= (int)segs.size() || n < 0 || (n && (!d_pts || !d_out))) return PGW_ERR_ARG;
+    if (n == 0) return PGW_OK;
+    const Seg& g = *segs[sgi];
+    const FsnParams q = fsn_params(g);
+    if ((size_t)n * q.nchunk * FSN_NSUM > kScratchDoubles) return PGW_ERR_SCRATCH;
+    if (!fsn_scratch) return PGW_ERR_DEVICE;
+    HIPCHK(hipEventRecord(ev0, stream));
+    // every (point, chunk) slot of the partial sums is written before it is read: no clearing
+    int st;
+    if ((st = fsn_enqueue(g, q, d_pts, n, fsn_scratch, d_out, d_grad))) return st;
     HIPCHK(hipEventRecord(ev1, stream));
     HIPCHK(hipEventSynchronize(ev1));
     float t = 0.f;
@@ -1319,6 +1387,11 @@ int pgw_fstatnet(pgw_handle* h, int seg, const double* pts, int npts, double* ou
   return h->fstatnet(seg, pts, npts, out, grad);
 }
 
+int pgw_fstatnet_noalloc(pgw_handle* h, int seg, const double* d_pts, int npts, double* d_out, double* d_grad) {
+  if (!h) return PGW_ERR_ARG;
+  return h->fstatnet_noalloc(seg, d_pts, npts, d_out, d_grad);
+}
+
 int pgw_copy_bench(size_t bytes, int reps, double* ms) {
   if (!ms || bytes < 32 || reps < 1) return PGW_ERR_ARG;
   const size_t n = bytes / sizeof(double4);
@@ -1372,6 +1445,7 @@ const char* pgw_strerror(int status) {
     case PGW_ERR_UNSUPPORTED: return "unsupported size";
     case PGW_ERR_NODEVICE: return "no HIP device";
     case PGW_ERR_IO: return "trigger or state file I/O error";
+    case PGW_ERR_SCRATCH: return "points x chunks exceed the allocation-free entry point's scratch";
     default: return "unknown status";
   }
 }

@@ -49,6 +49,8 @@ def lib():
         L.pgw_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts),
                                    _ip, _ip, _ip]
         L.pgw_fstatnet.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp]
+        # device addresses as integers (caller-owned device buffers)
+        L.pgw_fstatnet_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.pgw_job_spotlight.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, C.c_int, C.POINTER(abi.Opts),
                                         _ip, _dp, C.c_int, _ip]
         L.pgw_get_settings.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.Settings)]
@@ -69,7 +71,8 @@ def lib():
 
 
 EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_job_core", "pgw_job_batch",
-            "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_fstatnet", "pgw_get_settings",
+            "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_fstatnet", "pgw_fstatnet_noalloc",
+            "pgw_get_settings",
             "pgw_search", "pgw_read_checkpoint", "pgw_job_spotlight",
             "pgw_debug_fstat", "pgw_debug_resampled", "pgw_time_stages", "pgw_copy_bench", "pgw_last_gpu_ms",
             "pgw_kernel_times", "pgw_destroy", "pgw_strerror"]
