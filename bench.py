@@ -116,16 +116,21 @@ def measured_traffic(kernel, prec, workload, L):
     return None
 
 
-def measured_valu(kernel, prec):
-    """VALU busy fraction of a kernel from the committed rocprofv3 counter summary
-    (profiles/*.md lines 'bench-valu: {json}'), or None."""
+def measured_valu(kernel, prec, workload=None):
+    """VALU busy fraction of a kernel from the committed rocprofv3 counter summaries
+    (profiles/*.md lines 'bench-valu: {json}'): the line of this workload if there is one, else
+    one without a workload, else None."""
+    generic = None
     for f in sorted((ROOT / "profiles").glob("*.md")):
         for line in f.read_text().splitlines():
             if line.startswith("bench-valu:"):
                 d = json.loads(line.split(":", 1)[1])
                 if d.get("kernel") == kernel and d.get("precision") == prec:
-                    return d["valu_busy"]
-    return None
+                    if d.get("workload") == workload:
+                        return d["valu_busy"]
+                    if "workload" not in d and generic is None:
+                        generic = d["valu_busy"]
+    return generic
 
 
 def measured_flops(kernel):
@@ -189,7 +194,7 @@ def roofline(kt, segw, prec_bytes, workload, tuned=None, spins=512):
                 # counter-measured HBM bytes over this run's kernel duration
                 traffic_GBps=(round(tr * n / (kt[k + "_ms"] * 1e-3) / 1e9, 1) if tr else None),
                 # counter-measured fraction of VALU issue cycles (SQ_INSTS_VALU x 4 / SIMD cycles)
-                valu_busy=measured_valu(kname, prec),
+                valu_busy=measured_valu(kname, prec, workload),
                 share_of_template_time=round(kt[k + "_ms"] / stage_ms, 4),
                 other_pass={"kernel": other, "achieved_GBps": round(ach_other, 1),
                             "frac": round(ach_other / HBM_PEAK_GBS, 4),

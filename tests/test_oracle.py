@@ -330,6 +330,70 @@ def test_spindown_range_property(orc, abi, synth):
             assert sd.min() >= -seg.sett["Smax"] - step and sd.max() <= seg.sett["Smin"] + step
 
 
+@pytest.mark.parametrize("sign", [-1.0, 1.0])
+@pytest.mark.parametrize("s0", [False, True])
+def test_spindown_range_endpoints(orc, abi, synth, sign, s0):
+    """Endpoints of jobcore.c:424-439 at hand-derived values: with M5 = sign * 2^-30 and the sky
+    terms nn M9 + mm M13 = 1.25 * 2^-30 (nn = 3, mm = 1), Smin = 3.7 * 2^-30 and Smax = 12.3 * 2^-30
+    (every quotient exact up to the rounding of 3.7 and 12.3), the raw bounds are
+    (Smin - 1.25) / M5 = -2.45 sign and -(1.25 + Smax) / M5 = -13.55 sign; truncation toward zero
+    gives -2 and 13 for M5 < 0, and 2 and -13 swapped into [-13, 2] for M5 > 0 (floor would give
+    -3 or -14); --nospindown keeps only the upper one."""
+    seg = synth.make_segment(N=8192, dets=("H1",), inject=False, nsky=(1, 1), nspin=1)
+    u = 2.0 ** -30
+    seg.M[5], seg.M[9], seg.M[13] = sign * u, 0.25 * u, 0.5 * u
+    seg.sett["Smin"], seg.sett["Smax"] = 3.7 * u, 12.3 * u
+    sett = abi.make_settings(seg)
+    rngs = abi.make_range(seg)
+    opts = abi.make_opts(use_range=False, s0=s0)
+    lo, hi = C.c_int(), C.c_int()
+    orc.lib().orc_spindown_range(C.byref(sett), 1, 3, C.byref(rngs), C.byref(opts), C.byref(lo), C.byref(hi))
+    want = (-2, 13) if sign < 0 else (-13, 2)
+    if s0:
+        want = (want[1], want[1])
+    assert (lo.value, hi.value) == want
+
+
+def test_het0_sign_at_band_edge(orc, abi, synth):
+    """het0 = fmod(nn M8 + mm M12, M0) keeps the sign of its argument (jobcore.c:264).  A noise-free
+    signal on the template of bin nmin of a sky point whose argument is negative (het0 in (-M0, 0))
+    is found at bin nmin with the template's frequency 2 pi nmin / L + het0 + het1; with het0
+    wrapped into [0, M0) its peak would fall at nmin - 1, outside [nmin, nmax), and the run's
+    record at nmin would carry a frequency one bin (M0) higher."""
+    seg = synth.make_segment(N=32768, dets=("H1", "L1"), sigmas=(1.0, 1.0), fftpad=2, seed=21, inject=False,
+                             nsky=(1, 1), nspin=3)
+    s, M = seg.sett, seg.M
+    pm, mm, nn, ss = 1, 0, -1, 4
+    arg = nn * M[8] + mm * M[12]
+    het0 = math.fmod(arg, M[0])
+    assert arg < 0 and -M[0] < het0 < 0
+    oms = s["oms"]
+    al1, al2 = nn * M[10] + mm * M[14], nn * M[11] + mm * M[15]
+    sv = [C.c_double() for _ in range(4)]
+    orc.lib().orc_lin2ast(al1 / oms, al2 / oms, pm, math.sin(seg.epsm), math.cos(seg.epsm), *[C.byref(v) for v in sv])
+    sinal, cosal, sindel, cosdel = (v.value for v in sv)
+    het1 = math.fmod(ss * M[4], M[0]) % M[0]
+    ii = s["nmin"]
+    f = 2 * math.pi * ii / s["nfftf"] + het0 + het1
+    fdot = ss * M[5] + nn * M[9] + mm * M[13]
+    t = np.arange(s["N"], dtype=float)
+    cm, sm = np.cos(s["omr"] * t), np.sin(s["omr"] * t)
+    for k in range(2):
+        aa, bb = synth.modvir(sinal, cosal, sindel, cosdel, math.sin(seg.phir[k]), math.cos(seg.phir[k]),
+                              seg.amod[k], cm, sm)
+        shift = seg.DetSSB[k].reshape(-1, 3) @ np.array([cosal * cosdel, sinal * cosdel, sindel])
+        ph = f * t + fdot * t * t + (oms + f + 2 * fdot * t) * shift
+        seg.xDat[k] = 0.08 * (aa * np.cos(ph) + bb * np.sin(ph))
+        seg.sig2[k] = 1.0
+    seg.range = dict(spndr=[ss, ss], nr=[nn, nn], mr=[mm, mm], pmr=[pm, pm])
+    recs, fn = orc.Search(seg, nthreads=4).job_core(pm, mm, nn, abi.make_range(seg), abi.make_opts(trl=5.0, white=True))
+    assert fn == 1 and len(recs) > 0
+    best = recs[np.argmax(recs[:, 4])]
+    assert abs(best[0] - f) < 1e-12
+    # the record's bin is nmin: its frequency minus the template's offsets is 2 pi nmin / L
+    assert round((best[0] - het0 - het1) * s["nfftf"] / (2 * math.pi)) == ii
+
+
 # ----------------------------------------------------------- composition --
 def test_openmp_equals_serial(orc, abi):
     from cases import injected_sky, segment
