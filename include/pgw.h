@@ -110,6 +110,8 @@ typedef struct pgw_output {
   const char *qname;     /* checkpoint state file (opts->qname); NULL: no checkpointing */
   int checkpoint_every;  /* sky points between checkpoints and trigger flushes (<= 0: 64) */
   int max_sky;           /* stop after this many sky points, state saved (<= 0: no limit) */
+  int float_records;     /* nonzero: records of 5 floats, as a COMP_FLOAT build writes them
+                            (FLOAT_TYPE = float, search/network/src-cpu/auxi.h:16-23) */
 } pgw_output;
 
 /* search_settings() + the size part of plan_fftw()
@@ -145,6 +147,15 @@ int pgw_job_core(pgw_handle *h, int pm, int mm, int nn, pgw_range *s_range, cons
  * works on the next. */
 int pgw_job_batch(pgw_handle *h, const int *sky, int nsky, pgw_range *s_range, const pgw_opts *opts,
                   int *sgnlc, double *sgnlv, int sgnlv_cap, int *FNum);
+
+/* pgw_job_core and pgw_job_batch for a driver built with COMP_FLOAT (FLOAT_TYPE = float,
+ * search/network/src-cpu/auxi.h:16-23): records of PGW_NPAR floats, each field the double the
+ * double-record entry points write rounded once to float, as job_core's assignment to its
+ * FLOAT_TYPE sgnlv does (jobcore.c:686-687).  Same semantics and status codes otherwise. */
+int pgw_job_core_f(pgw_handle *h, int pm, int mm, int nn, pgw_range *s_range, const pgw_opts *opts,
+                   int *sgnlc, float *sgnlv, int sgnlv_cap, int *FNum);
+int pgw_job_batch_f(pgw_handle *h, const int *sky, int nsky, pgw_range *s_range, const pgw_opts *opts,
+                    int *sgnlc, float *sgnlv, int sgnlv_cap, int *FNum);
 
 /* The spotlight search (search/spotlight/src/jobcore.c:51-135 and its job_core, :137-640): sky
  * positions (mm[i], nn[i]) of hemisphere pm, each searched over its own list of noss[i] spindowns
@@ -242,7 +253,8 @@ int pgw_get_settings(const pgw_handle *h, int seg, pgw_settings *sett);
 /* search(), search/network/src-cpu/jobcore.c:51-184: hemispheres pm = pst..pmr[1], then
  * mm = mst..mr[1], nn = nst..nr[1] (the starts fall back to mr[0], nr[0] and sst to spndr[0]
  * after the first), every sky point through pgw_job_core's semantics.  Records are appended to
- * <prefix>/triggers_<ident %03d>_<band %04d><label>_<pm>.bin (5 doubles each) under an fcntl
+ * <prefix>/triggers_<ident %03d>_<band %04d><label>_<pm>.bin (5 doubles each, 5 floats with
+ * out->float_records) under an fcntl
  * write lock whenever more than nfft are buffered and at the end of each hemisphere (the file is
  * created even when empty), as the reference does.  Sky points run in batches of up to
  * checkpoint_every through the batched engine.  With a state file, after each batch the buffered

@@ -54,7 +54,8 @@ class Range(C.Structure):
 
 class Output(C.Structure):
     _fields_ = [("prefix", C.c_char_p), ("ident", C.c_int), ("band", C.c_int), ("label", C.c_char_p),
-                ("qname", C.c_char_p), ("checkpoint_every", C.c_int), ("max_sky", C.c_int)]
+                ("qname", C.c_char_p), ("checkpoint_every", C.c_int), ("max_sky", C.c_int),
+                ("float_records", C.c_int)]
 
 
 class Opts(C.Structure):

@@ -14,8 +14,9 @@
 
 namespace {
 
-// Appends nrec records to `path` under a whole-file fcntl write lock (jobcore.c:127-145).
-int append_records(const std::string& path, const double* v, size_t nrec) {
+// Appends nrec records of PGW_NPAR values of `esize` bytes (8: double, 4: float) to `path` under a
+// whole-file fcntl write lock (jobcore.c:127-145).
+int append_records(const std::string& path, const void* v, size_t nrec, size_t esize) {
   const int fd = open(path.c_str(), O_WRONLY | O_CREAT | O_APPEND, S_IRUSR | S_IWUSR | S_IRGRP | S_IROTH);
   if (fd < 0) return PGW_ERR_IO;
   struct flock lck = {};
@@ -27,7 +28,7 @@ int append_records(const std::string& path, const double* v, size_t nrec) {
   while (fcntl(fd, F_SETLKW, &lck) < 0)
     if (errno != EINTR) { st = PGW_ERR_IO; break; }
   const char* p = reinterpret_cast<const char*>(v);
-  size_t left = nrec * PGW_NPAR * sizeof(double);
+  size_t left = nrec * PGW_NPAR * esize;
   while (!st && left > 0) {
     const ssize_t w = write(fd, p, left);
     if (w < 0) {
@@ -91,8 +92,11 @@ int pgw_search(pgw_handle* h, pgw_range* r, const pgw_opts* opts, const pgw_outp
   if (st) return st;
   const int chunk = out->checkpoint_every > 0 ? out->checkpoint_every : 64;
   const char* label = out->label ? out->label : "";
-  std::vector<double> buf;                        // records not yet in the trigger file
-  std::vector<double> part((size_t)PGW_NPAR * 2 * S.nfft);
+  // records as doubles, or as floats for a FLOAT_TYPE = float driver (search/network/src-cpu/auxi.h:16-23)
+  const bool fl = out->float_records != 0;
+  const size_t es = fl ? sizeof(float) : sizeof(double);
+  std::vector<char> buf;                          // records not yet in the trigger file (bytes)
+  std::vector<char> part((size_t)PGW_NPAR * 2 * S.nfft * es);
   int done = 0;
   for (int pm = r->pst; pm <= r->pmr[1]; ++pm) {
     char name[1024];
@@ -108,7 +112,7 @@ int pgw_search(pgw_handle* h, pgw_range* r, const pgw_opts* opts, const pgw_outp
     size_t k = 0;
     while (k < sky.size()) {
       if (out->max_sky > 0 && done >= out->max_sky)  // the state (if any) already points at sky[k]
-        return append_records(path, buf.data(), buf.size() / PGW_NPAR);
+        return append_records(path, buf.data(), buf.size() / (PGW_NPAR * es), es);
       // the first sky point after a resume keeps its own first spindown (sst), the rest start
       // at spndr[0], as search() resets sst after each job_core
       size_t n = r->sst != r->spndr[0] ? 1 : std::min(sky.size() - k, (size_t)chunk);
@@ -116,30 +120,31 @@ int pgw_search(pgw_handle* h, pgw_range* r, const pgw_opts* opts, const pgw_outp
       const int FN0 = *FNum;
       int cnt = 0;
       for (;;) {
-        const int room = (int)(part.size() / PGW_NPAR);
-        st = pgw_job_batch(h, &sky[k].pm, (int)n, r, opts, &cnt, part.data(), room, FNum);
+        const int room = (int)(part.size() / (PGW_NPAR * es));
+        st = fl ? pgw_job_batch_f(h, &sky[k].pm, (int)n, r, opts, &cnt, reinterpret_cast<float*>(part.data()), room, FNum)
+                : pgw_job_batch(h, &sky[k].pm, (int)n, r, opts, &cnt, reinterpret_cast<double*>(part.data()), room, FNum);
         if (st != PGW_ERR_CAPACITY) break;
         if (cnt <= room) return PGW_ERR_CAPACITY;  // no larger count to make room for: do not spin
-        part.resize((size_t)PGW_NPAR * cnt);       // room for every record of the batch, then again
+        part.resize((size_t)PGW_NPAR * cnt * es);  // room for every record of the batch, then again
         cnt = 0;
         *FNum = FN0;
       }
       if (st) return st;
       r->sst = r->spndr[0];
-      buf.insert(buf.end(), part.begin(), part.begin() + (size_t)PGW_NPAR * cnt);
+      buf.insert(buf.end(), part.begin(), part.begin() + (size_t)PGW_NPAR * cnt * es);
       k += n;
       done += (int)n;
       const bool last = k == sky.size();
       // flush rule of jobcore.c:127 (more than nfft buffered), and before every state update
-      if (buf.size() / PGW_NPAR > (size_t)S.nfft || (out->qname && !last)) {
-        if ((st = append_records(path, buf.data(), buf.size() / PGW_NPAR))) return st;
+      if (buf.size() / (PGW_NPAR * es) > (size_t)S.nfft || (out->qname && !last)) {
+        if ((st = append_records(path, buf.data(), buf.size() / (PGW_NPAR * es), es))) return st;
         buf.clear();
       }
       if (out->qname && !last && (st = write_state(out->qname, pm, sky[k].mm, sky[k].nn, r->sst, *FNum)))
         return st;
     }
     // leftover of the hemisphere (jobcore.c:152-166); creates the file even when empty
-    if ((st = append_records(path, buf.data(), buf.size() / PGW_NPAR))) return st;
+    if ((st = append_records(path, buf.data(), buf.size() / (PGW_NPAR * es), es))) return st;
     buf.clear();
     r->pst = pm + 1;
     if (out->qname && pm < r->pmr[1] && (st = write_state(out->qname, pm + 1, r->mr[0], r->nr[0], r->sst, *FNum)))

@@ -120,9 +120,15 @@ template <typename T> int upload_tw(DevBuf<cplx<T>>& buf, int P, int stride_exp 
 template <typename T> struct Plan {
   int L = 0, L1 = 0, L2 = 0, twshift = 0;
   DevBuf<cplx<T>> twc, twr;
-  int init(int len) {
+  // l1max > 0: rows of at most l1max points where the column length stays <= 256 (shorter rows,
+  // longer columns)
+  int init(int len, int l1max = 0) {
     L = len;
     if (!split_len(L, L1, L2)) return PGW_ERR_UNSUPPORTED;
+    while (l1max >= 1024 && L1 > l1max && L2 < 256) {
+      L1 /= 2;
+      L2 *= 2;
+    }
     twshift = 20 - ilog2(L);
     int st = upload_tw<T>(twc, L2);
     if (!st) st = upload_tw<T>(twr, L1);
@@ -300,8 +306,11 @@ struct pgw_handle {
   virtual ~pgw_handle() {}
   // ssoff/sslist (optional): work item i searches spindowns sslist[ssoff[i] .. ssoff[i+1]) instead
   // of the range's
-  virtual int run(const int* work, int n, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap,
-                  int* rec_work, int* FNum, const int* ssoff = nullptr, const int* sslist = nullptr) = 0;
+  // sgnlv: records of PGW_NPAR doubles, or of PGW_NPAR floats when rec_float (the reference's
+  // FLOAT_TYPE = float build)
+  virtual int run(const int* work, int n, pgw_range* r, const pgw_opts* o, int* sgnlc, void* sgnlv, int cap,
+                  int* rec_work, int* FNum, const int* ssoff = nullptr, const int* sslist = nullptr,
+                  bool rec_float = false) = 0;
   virtual int debug_fstat(int pm, int mm, int nn, int ss, const pgw_opts* o, double* F) = 0;
   virtual int debug_resampled(int pm, int mm, int nn, int det, double* a, double* b) = 0;
   virtual int time_stages(int seg, int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps,
@@ -378,6 +387,7 @@ template <typename T> struct Engine final : pgw_handle {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evd[2] = {nullptr, nullptr};
   int TB = 128, TBalloc = 0;
   int cand_cap = kCandCap;  // candidates per group in the device buffer (PGW_TUNE candcap= lowers it)
+  int l1max = 0;            // FFT row length cap (0: the split_len table)
   int group = kSlots;  // sky points per group
   int row_na = 1;
   bool col_seq = true;  // column pass: the two spectra share one LDS buffer in turn
@@ -454,7 +464,7 @@ template <typename T> struct Engine final : pgw_handle {
     auto it = plans.find(L);
     if (it != plans.end()) return it->second.get();
     auto p = std::make_unique<Plan<T>>();
-    st = p->init(L);
+    st = p->init(L, l1max);
     if (st) return nullptr;
     Plan<T>* q = p.get();
     plans[L] = std::move(p);
@@ -540,8 +550,6 @@ template <typename T> struct Engine final : pgw_handle {
   int init(const pgw_settings* s, const pgw_detector* ifo, int nseg, int ndet) {
     nifo = ndet;
     int st;
-    for (int k = 0; k < nseg; k++)
-      if ((st = add_segment(&s[k], ifo + (size_t)k * ndet))) return st;
     // kernel variants; PGW_TUNE="row_na=2,col_seq=1,tb=64,group=4,spl_k=4,half=1" overrides (used when
     // measuring variants)
     if (const char* tune = getenv("PGW_TUNE")) {
@@ -555,7 +563,11 @@ template <typename T> struct Engine final : pgw_handle {
       if (const char* q = strstr(tune, "row8=")) if (sscanf(q, "row8=%d", &v) == 1 && v >= 0 && v <= 2) row8 = v;
       if (const char* q = strstr(tune, "thr_fuse=")) if (sscanf(q, "thr_fuse=%d", &v) == 1) thr_fuse = v != 0;
       if (const char* q = strstr(tune, "candcap=")) if (sscanf(q, "candcap=%d", &v) == 1 && v >= 1 && v <= kCandCap) cand_cap = v;
+      if (const char* q = strstr(tune, "l1max=")) if (sscanf(q, "l1max=%d", &v) == 1 && (v == 1024 || v == 2048 || v == 4096)) l1max = v;
     }
+    // segments after the variants: their FFT plans follow l1max
+    for (int k = 0; k < nseg; k++)
+      if ((st = add_segment(&s[k], ifo + (size_t)k * ndet))) return st;
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     // this device's copy of the allocation-free Fstatnet's scratch (the device is current here)
     if ((st = scratch_address(&fsn_scratch))) return st;
@@ -846,7 +858,8 @@ template <typename T> struct Engine final : pgw_handle {
   }
 
   // Waits for a group's candidates and appends its records (jobcore.c:659-695) in (sky point, ss, bin) order.
-  int finish_group(const Group& gr0, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap, int* rec_work) {
+  int finish_group(const Group& gr0, const pgw_opts* o, int* sgnlc, void* sgnlv, int cap, int* rec_work,
+                   bool rec_float) {
 #pragma clang fp contract(off)
     HIPCHK(hipEventSynchronize(evd[gr0.buf]));
     int nc = hn_pin.p[gr0.buf];  // the device count keeps counting past the buffer's capacity
@@ -905,7 +918,12 @@ template <typename T> struct Engine final : pgw_handle {
         (*sgnlc)++;
         continue;
       }
-      memcpy(sgnlv + (size_t)PGW_NPAR * (*sgnlc), r, sizeof(r));
+      if (rec_float) {  // FLOAT_TYPE = float: each field rounded once, as the reference's assignment does
+        float* q = static_cast<float*>(sgnlv) + (size_t)PGW_NPAR * (*sgnlc);
+        for (int f = 0; f < PGW_NPAR; f++) q[f] = (float)r[f];
+      } else {
+        memcpy(static_cast<double*>(sgnlv) + (size_t)PGW_NPAR * (*sgnlc), r, sizeof(r));
+      }
       if (rec_work) rec_work[*sgnlc] = it.work;
       (*sgnlc)++;
     }
@@ -914,8 +932,8 @@ template <typename T> struct Engine final : pgw_handle {
 
   // Work items (seg, pm, mm, nn) in order, grouped; the host turns group g-1 into records while
   // the device works on group g.
-  int run(const int* work, int n, pgw_range* r, const pgw_opts* o, int* sgnlc, double* sgnlv, int cap, int* rec_work,
-          int* FNum, const int* ssoff, const int* sslist) override {
+  int run(const int* work, int n, pgw_range* r, const pgw_opts* o, int* sgnlc, void* sgnlv, int cap, int* rec_work,
+          int* FNum, const int* ssoff, const int* sslist, bool rec_float) override {
     HIPCHK(hipEventRecord(ev0, stream));
     Group prev;
     bool have_prev = false;
@@ -956,14 +974,14 @@ template <typename T> struct Engine final : pgw_handle {
       }
       if (st || gr.items.empty()) continue;
       if ((st = launch_group(gr, o))) break;
-      if (have_prev) st = finish_group(prev, o, sgnlc, sgnlv, cap, rec_work);
+      if (have_prev) st = finish_group(prev, o, sgnlc, sgnlv, cap, rec_work, rec_float);
       prev = std::move(gr);
       have_prev = true;
       buf ^= 1;
     }
     HIPCHK(hipEventRecord(ev1, stream));
     if (have_prev) {
-      const int st2 = finish_group(prev, o, sgnlc, sgnlv, cap, rec_work);
+      const int st2 = finish_group(prev, o, sgnlc, sgnlv, cap, rec_work, rec_float);
       if (!st) st = st2;
     }
     HIPCHK(hipEventSynchronize(ev1));
@@ -1181,7 +1199,7 @@ template <typename T> struct Engine final : pgw_handle {
     ktime = 0;
     auto per_template = [&](double& ms) {
       int sgnlc = 0, fn = 0;
-      const int st = run(work.data(), NI, &rq, &oq, &sgnlc, nullptr, 0, nullptr, &fn, ssoff.data(), sslist.data());
+      const int st = run(work.data(), NI, &rq, &oq, &sgnlc, nullptr, 0, nullptr, &fn, ssoff.data(), sslist.data(), false);
       if (st) return st;
       if (fn == 0) return PGW_ERR_ARG;  // outside the sky disk
       ms = last_ms / fn;
@@ -1333,6 +1351,27 @@ int pgw_job_batch(pgw_handle* h, const int* sky, int nsky, pgw_range* s_range, c
     work[4 * i + 3] = sky[3 * i + 2];
   }
   return h->run(work.data(), nsky, s_range, opts, sgnlc, sgnlv, sgnlv_cap, nullptr, FNum);
+}
+
+int pgw_job_core_f(pgw_handle* h, int pm, int mm, int nn, pgw_range* s_range, const pgw_opts* opts, int* sgnlc,
+                   float* sgnlv, int sgnlv_cap, int* FNum) {
+  if (!h || !s_range || !opts || !sgnlc || !FNum || (sgnlv_cap > 0 && !sgnlv)) return PGW_ERR_ARG;
+  const int work[4] = {0, pm, mm, nn};
+  return h->run(work, 1, s_range, opts, sgnlc, sgnlv, sgnlv_cap, nullptr, FNum, nullptr, nullptr, true);
+}
+
+int pgw_job_batch_f(pgw_handle* h, const int* sky, int nsky, pgw_range* s_range, const pgw_opts* opts, int* sgnlc,
+                    float* sgnlv, int sgnlv_cap, int* FNum) {
+  if (!h || (!sky && nsky > 0) || nsky < 0 || !s_range || !opts || !sgnlc || !FNum || (sgnlv_cap > 0 && !sgnlv))
+    return PGW_ERR_ARG;
+  std::vector<int> work((size_t)4 * nsky);
+  for (int i = 0; i < nsky; i++) {
+    work[4 * i] = 0;
+    work[4 * i + 1] = sky[3 * i];
+    work[4 * i + 2] = sky[3 * i + 1];
+    work[4 * i + 3] = sky[3 * i + 2];
+  }
+  return h->run(work.data(), nsky, s_range, opts, sgnlc, sgnlv, sgnlv_cap, nullptr, FNum, nullptr, nullptr, true);
 }
 
 int pgw_job_batch_multi(pgw_handle* h, const int* work, int nwork, pgw_range* s_range, const pgw_opts* opts,

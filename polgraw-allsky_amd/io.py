@@ -202,11 +202,12 @@ def band_lines(frame: "Frame", dtaprefix, narrowdown: float = 0.5 * math.pi, vet
     return out
 
 
-def read_triggers(path) -> np.ndarray:
-    """triggers_*.bin: (n, 5) records (frequency, spindown, delta, alpha, SNR)."""
-    a = np.fromfile(path, dtype=np.float64)
+def read_triggers(path, dtype=np.float64) -> np.ndarray:
+    """triggers_*.bin: (n, 5) records (frequency, spindown, delta, alpha, SNR); dtype float32 for
+    the files of a COMP_FLOAT build (FLOAT_TYPE = float, search/network/src-cpu/auxi.h:16-23)."""
+    a = np.fromfile(path, dtype=dtype)
     if len(a) % 5:
-        raise ValueError(f"{path}: not a whole number of 5-double records")
+        raise ValueError(f"{path}: not a whole number of 5-value records")
     return a.reshape(-1, 5)
 
 

@@ -40,6 +40,11 @@ def lib():
                                    C.POINTER(abi.Opts), _ip, _dp, C.c_int, _ip]
         L.pgw_job_batch.argtypes = [C.c_void_p, _ip, C.c_int, C.POINTER(abi.Range), C.POINTER(abi.Opts),
                                     _ip, _dp, C.c_int, _ip]
+        _fp = C.POINTER(C.c_float)
+        L.pgw_job_core_f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Range),
+                                     C.POINTER(abi.Opts), _ip, _fp, C.c_int, _ip]
+        L.pgw_job_batch_f.argtypes = [C.c_void_p, _ip, C.c_int, C.POINTER(abi.Range), C.POINTER(abi.Opts),
+                                      _ip, _fp, C.c_int, _ip]
         L.pgw_debug_fstat.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts), _dp]
         L.pgw_debug_resampled.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.pgw_init_multi.argtypes = [C.POINTER(C.c_void_p), C.POINTER(abi.Settings), C.POINTER(abi.Detector),
@@ -71,6 +76,7 @@ def lib():
 
 
 EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_job_core", "pgw_job_batch",
+            "pgw_job_core_f", "pgw_job_batch_f",
             "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_fstatnet", "pgw_fstatnet_noalloc",
             "pgw_get_settings",
             "pgw_search", "pgw_read_checkpoint", "pgw_job_spotlight",
@@ -171,6 +177,27 @@ class Handle:
             raise PGWError(st, "pgw_job_batch")
         return sgnlv[:abi.NPAR * sgnlc.value].reshape(-1, abi.NPAR).copy(), fnum.value
 
+    def job_core_f(self, pm, mm, nn, rng, opts, cap=1 << 16):
+        """job_core with float records (COMP_FLOAT drivers): (records (n, 5) float32, FNum)."""
+        sgnlv = np.empty(abi.NPAR * cap, np.float32)
+        sgnlc, fnum = C.c_int(0), C.c_int(0)
+        st = lib().pgw_job_core_f(self.h, pm, mm, nn, C.byref(rng), C.byref(opts), C.byref(sgnlc),
+                                  sgnlv.ctypes.data_as(C.POINTER(C.c_float)), cap, C.byref(fnum))
+        if st != 0:
+            raise PGWError(st, "pgw_job_core_f")
+        return sgnlv[:abi.NPAR * sgnlc.value].reshape(-1, abi.NPAR).copy(), fnum.value
+
+    def job_batch_f(self, sky, rng, opts, cap=1 << 18):
+        """job_batch with float records (COMP_FLOAT drivers): (records (n, 5) float32, FNum)."""
+        sky = np.ascontiguousarray(np.asarray(sky, dtype=np.int32).reshape(-1, 3))
+        sgnlv = np.empty(abi.NPAR * cap, np.float32)
+        sgnlc, fnum = C.c_int(0), C.c_int(0)
+        st = lib().pgw_job_batch_f(self.h, sky.ctypes.data_as(_ip), len(sky), C.byref(rng), C.byref(opts),
+                                   C.byref(sgnlc), sgnlv.ctypes.data_as(C.POINTER(C.c_float)), cap, C.byref(fnum))
+        if st != 0:
+            raise PGWError(st, "pgw_job_batch_f")
+        return sgnlv[:abi.NPAR * sgnlc.value].reshape(-1, abi.NPAR).copy(), fnum.value
+
     def job_spotlight(self, pm, sky, opts, cap=1 << 18):
         """sky: [(mm, nn, [ss, ...]), ...] as a spotlight file lists them; returns (records, FNum)."""
         mm = np.ascontiguousarray([x[0] for x in sky], dtype=np.int32)
@@ -211,10 +238,12 @@ class Handle:
             raise PGWError(st, "pgw_fstatnet")
         return out, g
 
-    def search(self, rng, opts, prefix, ident=1, band=666, label="", qname=None, every=64, max_sky=0, FNum=0):
+    def search(self, rng, opts, prefix, ident=1, band=666, label="", qname=None, every=64, max_sky=0, FNum=0,
+               float_records=False):
         """search(): trigger files under prefix, optional checkpoint state; returns FNum."""
         out = abi.Output(prefix=str(prefix).encode(), ident=ident, band=band, label=label.encode(),
-                         qname=str(qname).encode() if qname else None, checkpoint_every=every, max_sky=max_sky)
+                         qname=str(qname).encode() if qname else None, checkpoint_every=every, max_sky=max_sky,
+                         float_records=int(float_records))
         fn = C.c_int(FNum)
         st = lib().pgw_search(self.h, C.byref(rng), C.byref(opts), C.byref(out), C.byref(fn))
         if st != 0:

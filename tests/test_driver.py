@@ -108,3 +108,35 @@ def test_search_resume(pgw, abi, tmp_path):
     assert fn == fn_full
     for pm in (1, 2):
         _same(_read(part / f"triggers_001_0666_{pm}.bin"), _read(full / f"triggers_001_0666_{pm}.bin"))
+
+
+@pytest.mark.gpu
+def test_float_records(pgw, abi, tmp_path):
+    """COMP_FLOAT drivers (FLOAT_TYPE = float, cpu/auxi.h:16-23): pgw_job_core_f, pgw_job_batch_f and
+    float trigger files carry each field of the double records rounded once to float, with the
+    same record count, order and FNum."""
+    from polgraw_allsky_amd import io
+    seg = segment("net-2det-32k")
+    opts = abi.make_opts(trl=12.0)
+    r = _range(abi, seg)
+    h = pgw.Handle(seg, abi.PREC_FP64)
+    sky = [(1, mm, nn) for mm in range(r.mr[0], r.mr[1] + 1) for nn in range(r.nr[0], r.nr[1] + 1)]
+    d, fd = h.job_batch(sky, r, opts)
+    f, ff = h.job_batch_f(sky, r, opts)
+    assert ff == fd and f.dtype == np.float32 and len(d) > 0
+    assert np.array_equal(f, d.astype(np.float32))
+    d1, fd1 = h.job_core(*sky[4], r, opts)
+    f1, ff1 = h.job_core_f(*sky[4], r, opts)
+    assert ff1 == fd1 and np.array_equal(f1, d1.astype(np.float32))
+    # a record buffer one short: the count needed, as with double records
+    with pytest.raises(pgw.PGWError) as e:
+        h.job_batch_f(sky, r, opts, cap=len(d) - 1)
+    assert e.value.status == abi.PGW_ERR_CAPACITY
+    for fl, sub in ((False, "d"), (True, "f")):
+        (tmp_path / sub).mkdir()
+        h.search(_range(abi, seg), opts, tmp_path / sub, every=4, float_records=fl)
+    for pm in (1, 2):
+        dd = io.read_triggers(tmp_path / "d" / f"triggers_001_0666_{pm}.bin")
+        ff_ = io.read_triggers(tmp_path / "f" / f"triggers_001_0666_{pm}.bin", dtype=np.float32)
+        assert np.array_equal(ff_, dd.astype(np.float32))
+    print("records", len(d), flush=True)

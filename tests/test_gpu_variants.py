@@ -3,7 +3,7 @@ records: row pass with 16 or 8 elements per thread, with and without the half-sp
 both spectra together, column pass with shared LDS, spline chunk lengths, launch and group
 sizes, thresholds folded into the next column launch or launched on their own, a device
 candidate buffer smaller than a group's candidates (the group runs again into a buffer of the
-needed size).  FP64 to 1e-9 relative in F; FP32 against the FP64 oracle by the rule of
+needed size), FFT rows capped at 2048 or 1024 points (longer columns).  FP64 to 1e-9 relative in F; FP32 against the FP64 oracle by the rule of
 cases.fp32_record_diff."""
 import os
 
@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 VARIANTS = ["", "row8=0", "row8=1", "row8=2", "half=0", "row8=0,half=0", "row8=2,half=0", "row_na=2,row8=0",
             "col_seq=1", "spl_k=8", "spl_k=16", "tb=32,group=3", "thr_fuse=0", "tb=8", "candcap=16",
-            "candcap=16,group=1"]
+            "candcap=16,group=1", "l1max=2048", "l1max=2048,row8=0", "l1max=1024"]
 CASES = ["plain-1det-32k", "net-2det-32k", "net-2det-86k"]
 
 
@@ -70,7 +70,8 @@ def test_variant_fp64(pgw, abi, oracle_runs, name, tune):
     assert np.max(np.abs(Fg - Fr) / Fr) <= 1e-9
 
 
-@pytest.mark.parametrize("tune", ["", "row8=1", "row8=2", "half=0", "row_na=2", "spl_k=8", "thr_fuse=0"])
+@pytest.mark.parametrize("tune", ["", "row8=1", "row8=2", "half=0", "row_na=2", "spl_k=8", "thr_fuse=0",
+                                  "l1max=2048", "l1max=1024"])
 @pytest.mark.parametrize("name", CASES)
 def test_variant_fp32(pgw, abi, oracle_runs, name, tune):
     sky, ref, fr = oracle_runs[name]
