@@ -32,7 +32,8 @@ _ref = None
 def lib():
     global _lib
     if _lib is None:
-        path = HERE / "liborc.so"
+        # ORC_LIB: another build of the oracle (the sanitizer builds of oracle/Makefile)
+        path = Path(os.environ.get("ORC_LIB", HERE / "liborc.so"))
         if not path.exists():
             raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
         L = C.CDLL(str(path))
