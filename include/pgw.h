@@ -191,6 +191,12 @@ int pgw_job_batch_multi(pgw_handle *h, const int *work, int nwork, pgw_range *s_
  * Nelder-Mead and MADS, followup.c:537-1173).  FP64 on the device. */
 int pgw_fstatnet(pgw_handle *h, int seg, const double *pts, int npts, double *out, double *grad);
 
+/* pgw_fstatnet for points of several segments in one call: point i (pts[4*i..], out[11*i..],
+ * grad[4*i..]) is evaluated over segment seg[i].  The followup's candidates of many time frames
+ * (or many injections) go to the device together; no counterpart in the reference, which follows
+ * up one frame per process. */
+int pgw_fstatnet_multi(pgw_handle *h, const int *seg, const double *pts, int npts, double *out, double *grad);
+
 /* Allocation-free pgw_fstatnet.  No counterpart in the reference: for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  pts (4 * npts), out (11 * npts) and grad (4 * npts, optional) are DEVICE addresses

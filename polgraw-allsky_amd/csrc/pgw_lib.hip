@@ -319,6 +319,7 @@ struct pgw_handle {
   virtual int kernel_times(int enable, double* out) = 0;
   virtual int fstatnet(int seg, const double* pts, int n, double* out, double* grad) = 0;
   virtual int fstatnet_noalloc(int seg, const double* d_pts, int n, double* d_out, double* d_grad) = 0;
+  virtual int fstatnet_multi(const int* seg, const double* pts, int n, double* out, double* grad) = 0;
   virtual int settings(int seg, pgw_settings* out) const = 0;
   double last_ms = 0.0;
 };
@@ -1087,6 +1088,59 @@ template <typename T> struct Engine final : pgw_handle {
     return PGW_OK;
   }
 
+  // Fstatnet of points spread over segments (point i in segment seg[i]): the points are grouped by
+  // segment on the host (stable), uploaded once, each segment's two kernels queued back to back
+  // over its own slice of the buffers, the outputs copied back once, in batches of kFsnBatch.
+  int fstatnet_multi(const int* seg, const double* pts, int n, double* out, double* grad) override {
+    constexpr int kFsnBatch = 4096;
+    if (n < 0 || (n && (!seg || !pts || !out))) return PGW_ERR_ARG;
+    for (int i = 0; i < n; i++)
+      if (seg[i] < 0 || seg[i] >= (int)segs.size()) return PGW_ERR_ARG;
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return seg[a] < seg[b]; });
+    size_t maxchunk = 0;
+    for (const auto& g : segs) maxchunk = std::max(maxchunk, (size_t)((g->sett.N + FSN_CHUNK - 1) / FSN_CHUNK));
+    const int nb = std::min(n, kFsnBatch);
+    int st;
+    if (nb > 0 && ((st = fsn_pts.alloc_min((size_t)4 * nb)) || (st = fsn_part.alloc_min((size_t)nb * maxchunk * FSN_NSUM)) ||
+                   (st = fsn_out.alloc_min((size_t)11 * nb)) || (st = fsn_grad.alloc_min((size_t)4 * nb))))
+      return st;
+    std::vector<double> hp((size_t)4 * nb), ho((size_t)11 * nb), hg((size_t)4 * nb);
+    HIPCHK(hipEventRecord(ev0, stream));
+    for (int p0 = 0; p0 < n; p0 += kFsnBatch) {
+      const int m = std::min(kFsnBatch, n - p0);
+      for (int j = 0; j < m; j++) memcpy(&hp[4 * (size_t)j], pts + 4 * (size_t)order[p0 + j], 4 * sizeof(double));
+      HIPCHK(hipMemcpyAsync(fsn_pts.p, hp.data(), sizeof(double) * 4 * m, hipMemcpyHostToDevice, stream));
+      size_t partoff = 0;
+      for (int j0 = 0; j0 < m;) {  // runs of one segment
+        const int sgi = seg[order[p0 + j0]];
+        int j1 = j0 + 1;
+        while (j1 < m && seg[order[p0 + j1]] == sgi) j1++;
+        const Seg& g = *segs[sgi];
+        const FsnParams q = fsn_params(g);
+        if ((st = fsn_enqueue(g, q, fsn_pts.p + 4 * (size_t)j0, j1 - j0, fsn_part.p + partoff, fsn_out.p + 11 * (size_t)j0,
+                              grad ? fsn_grad.p + 4 * (size_t)j0 : nullptr)))
+          return st;
+        partoff += (size_t)(j1 - j0) * q.nchunk * FSN_NSUM;
+        j0 = j1;
+      }
+      HIPCHK(hipMemcpyAsync(ho.data(), fsn_out.p, sizeof(double) * 11 * m, hipMemcpyDeviceToHost, stream));
+      if (grad) HIPCHK(hipMemcpyAsync(hg.data(), fsn_grad.p, sizeof(double) * 4 * m, hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));  // the next batch reuses the buffers
+      for (int j = 0; j < m; j++) {
+        memcpy(out + 11 * (size_t)order[p0 + j], &ho[11 * (size_t)j], 11 * sizeof(double));
+        if (grad) memcpy(grad + 4 * (size_t)order[p0 + j], &hg[4 * (size_t)j], 4 * sizeof(double));
+      }
+    }
+    HIPCHK(hipEventRecord(ev1, stream));
+    HIPCHK(hipEventSynchronize(ev1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, ev0, ev1));
+    last_ms = t;
+    return PGW_OK;
+  }
+
   // Fstatnet on caller-owned device buffers with the partial sums in this device's `scratch`: no
   // allocation, one batch, synchronous on the handle's stream.
   int fstatnet_noalloc(int sgi, const double* d_pts, int n, double* d_out, double* d_grad) override {
@@ -1424,6 +1478,11 @@ int pgw_job_spotlight(pgw_handle* h, int pm, const int* mm, const int* nn, const
 int pgw_fstatnet(pgw_handle* h, int seg, const double* pts, int npts, double* out, double* grad) {
   if (!h) return PGW_ERR_ARG;
   return h->fstatnet(seg, pts, npts, out, grad);
+}
+
+int pgw_fstatnet_multi(pgw_handle* h, const int* seg, const double* pts, int npts, double* out, double* grad) {
+  if (!h) return PGW_ERR_ARG;
+  return h->fstatnet_multi(seg, pts, npts, out, grad);
 }
 
 int pgw_fstatnet_noalloc(pgw_handle* h, int seg, const double* d_pts, int npts, double* d_out, double* d_grad) {

@@ -67,8 +67,8 @@ class FStatNet:
 
 class MultiFStatNet:
     """Fstatnet for candidates spread over the segments of one pgw.Handle: candidate k lives in
-    segment seg_of[k].  The searches below pass each point's candidate (`owner`); the points of
-    each segment go to the device in one call."""
+    segment seg_of[k].  The searches below pass each point's candidate (`owner`); all points go to
+    the device in one call (pgw_fstatnet_multi)."""
 
     def __init__(self, handle: pgw.Handle, seg_of):
         self.h = handle
@@ -81,61 +81,7 @@ class MultiFStatNet:
                 raise ValueError("points of several segments need their owners")
             owner = np.arange(len(pts))
         seg = self.seg_of[np.asarray(owner, dtype=np.int64)]
-        out = np.zeros((len(pts), 11))
-        g = np.zeros((len(pts), 4)) if grad else None
-        for sgi in np.unique(seg):
-            sel = np.nonzero(seg == sgi)[0]
-            o, gg = self.h.fstatnet(pts[sel], seg=int(sgi), grad=grad)
-            out[sel] = o
-            if grad:
-                g[sel] = gg
-        return out, g
-
-
-def grid_point(M, oms: float, point, epsm: float):
-    """Hemisphere and grid coordinates g (M g = (omega, omegadot, al1, al2)) of a point
-    (omega, omegadot, delta, alpha), with al = (oms + omega) x the linear sky coordinates (the
-    followup's mapping, followup/src/followup.c:1482-1490)."""
-    from .synth import ast2lin
-    om, omd, de, al = (float(v) for v in point)
-    pm, be = ast2lin(al, de, epsm)
-    Mm = np.asarray(M, dtype=np.float64).reshape(4, 4).T
-    return pm, np.linalg.solve(Mm, np.array([om, omd, be[0] * (oms + om), be[1] * (oms + om)]))
-
-
-def point_of_grid(M, oms: float, g, pm: int, epsm: float):
-    """The inverse of grid_point: (omega, omegadot, delta, alpha) at grid coordinates g."""
-    from .synth import lin2ast
-    Mm = np.asarray(M, dtype=np.float64).reshape(4, 4).T
-    om, omd, a1, a2 = Mm @ np.asarray(g, dtype=np.float64)
-    de, al = lin2ast(a1 / (oms + om), a2 / (oms + om), pm, epsm)
-    return np.array([om, omd, de, al])
-
-
-def _corner(M, oms, g, step, pm, epsm):
-    """The point at grid coordinates g + step, or at g - step where that corner lies beyond the
-    sky disk's edge."""
-    try:
-        return point_of_grid(M, oms, g + step, pm, epsm)
-    except ValueError:
-        return point_of_grid(M, oms, g - step, pm, epsm)
-
-
-def cell_diagonal(M, oms: float, point, epsm: float) -> np.ndarray:
-    """|theta(g + 1) - theta(g)| per parameter between two opposite corners of the grid cell at
-    `point` (all four grid indices + 1): twice this is the followup's first simplex step
-    (followup/src/followup.c:1555-1567)."""
-    pm, g = grid_point(M, oms, point, epsm)
-    a = point_of_grid(M, oms, g, pm, epsm)
-    return np.abs(_corner(M, oms, g, np.ones(4), pm, epsm) - a)
-
-
-def cell_width(M, oms: float, point, epsm: float) -> np.ndarray:
-    """Width of the grid cell at `point` along each parameter: sum over the four grid directions
-    of |theta(g + e_j) - theta(g)| (the cell's bounding box, to first order)."""
-    pm, g = grid_point(M, oms, point, epsm)
-    a = point_of_grid(M, oms, g, pm, epsm)
-    return sum(np.abs(_corner(M, oms, g, e, pm, epsm) - a) for e in np.eye(4))
+        return self.h.fstatnet_multi(seg, pts, grad=grad)
 
 
 def _torch_fn():

@@ -54,6 +54,7 @@ def lib():
         L.pgw_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts),
                                    _ip, _ip, _ip]
         L.pgw_fstatnet.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp]
+        L.pgw_fstatnet_multi.argtypes = [C.c_void_p, _ip, _dp, C.c_int, _dp, _dp]
         # device addresses as integers (caller-owned device buffers)
         L.pgw_fstatnet_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.pgw_job_spotlight.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, C.c_int, C.POINTER(abi.Opts),
@@ -77,7 +78,7 @@ def lib():
 
 EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_job_core", "pgw_job_batch",
             "pgw_job_core_f", "pgw_job_batch_f",
-            "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_fstatnet", "pgw_fstatnet_noalloc",
+            "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_fstatnet", "pgw_fstatnet_multi", "pgw_fstatnet_noalloc",
             "pgw_get_settings",
             "pgw_search", "pgw_read_checkpoint", "pgw_job_spotlight",
             "pgw_debug_fstat", "pgw_debug_resampled", "pgw_time_stages", "pgw_copy_bench", "pgw_last_gpu_ms",
@@ -236,6 +237,20 @@ class Handle:
                                 g.ctypes.data_as(_dp) if grad else None)
         if st != 0:
             raise PGWError(st, "pgw_fstatnet")
+        return out, g
+
+    def fstatnet_multi(self, seg, pts, grad=True):
+        """Fstatnet at points (n, 4) of segments seg (n,) in one call: (out (n, 11), dF/dpts or None)."""
+        pts = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(-1, 4))
+        seg = np.ascontiguousarray(np.asarray(seg, dtype=np.int32).reshape(-1))
+        if len(seg) != len(pts):
+            raise ValueError("one segment per point")
+        out = np.zeros((len(pts), 11))
+        g = np.zeros((len(pts), 4)) if grad else None
+        st = lib().pgw_fstatnet_multi(self.h, seg.ctypes.data_as(_ip), pts.ctypes.data_as(_dp), len(pts),
+                                      out.ctypes.data_as(_dp), g.ctypes.data_as(_dp) if grad else None)
+        if st != 0:
+            raise PGWError(st, "pgw_fstatnet_multi")
         return out, g
 
     def search(self, rng, opts, prefix, ident=1, band=666, label="", qname=None, every=64, max_sky=0, FNum=0,

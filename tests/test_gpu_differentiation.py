@@ -84,3 +84,20 @@ def test_ascent_vs_simplex(work, fs):
     # the simplex ends as high in F but wanders further along the ridges of F (up to ~2 cells)
     assert np.all(np.median(ea, axis=0) <= 1.0)
     assert np.all(np.median(es, axis=0) <= 3.0)
+
+
+def test_fstatnet_multi_equals_per_segment(abi):
+    """pgw_fstatnet_multi over points of three segments in shuffled order equals pgw_fstatnet per
+    segment, bit for bit, with and without the gradient."""
+    from polgraw_allsky_amd import pgw, synth
+    segs, truth, pts, cells, seg_of, h0 = synth.differentiation_workload(seed=2024, nseg=3, per_seg=8)
+    h = pgw.Handle(segs, abi.PREC_FP64)
+    perm = np.random.default_rng(3).permutation(len(pts))
+    out, g = h.fstatnet_multi(seg_of[perm], pts[perm])
+    out2, _ = h.fstatnet_multi(seg_of[perm], pts[perm], grad=False)
+    for k in range(3):
+        sel = np.nonzero(seg_of[perm] == k)[0]
+        ro, rg = h.fstatnet(pts[perm][sel], seg=k)
+        assert np.array_equal(out[sel], ro, equal_nan=True) and np.array_equal(g[sel], rg)
+        assert np.array_equal(out2[sel], ro, equal_nan=True)
+    h.close()
