@@ -212,9 +212,11 @@ int pgw_fstatnet_noalloc(pgw_handle *h, int seg, const double *pts, int npts, do
  * (pm, mm, nn) of segment `seg` (which must lie inside the sky disk; opts->trl is not used) for
  * every kernel variant (templates per launch 64/128/256;
  * row pass with 16 elements per thread and both spectra together or in turn, or, in the FP64
- * form, with 8 elements per thread; column pass LDS sharing) and keeps the fastest for this handle.  The
+ * form, with 8 elements per thread; column pass LDS sharing; in the FP32 form at column lengths
+ * of 128 and 256, loads-first demodulation) and keeps the fastest for this handle.  The
  * chosen values are returned (pointers may be NULL): *row_na is 1 or 2 (16 elements per thread,
- * spectra in turn or together) or 8 (8 elements per thread). */
+ * spectra in turn or together) or 8 (8 elements per thread); the loads-first choice is kept by
+ * the handle and not returned. */
 int pgw_autotune(pgw_handle *h, int seg, int pm, int mm, int nn, const pgw_opts *opts, int *tb,
                  int *row_na, int *col_seq);
 

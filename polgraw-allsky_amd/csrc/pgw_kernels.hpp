@@ -412,6 +412,7 @@ struct SpinTab {
 
 struct SpinParams {
   int L1, twshift;
+  int rev;  // FP32 form: phase accumulated in revolutions (PhaseRev) instead of radians
 };
 
 template <typename T> struct PhaseRot;
@@ -432,6 +433,21 @@ template <> struct PhaseRot<float> {
     const double rr = fma(-k, 2.4492935982947064e-16, r);
     const float rev = (float)(rr * 0.15915494309189533577);
     return make_float2(__builtin_amdgcn_cosf(rev), -__builtin_amdgcn_sinf(rev));
+  }
+};
+
+// FP32 form, phase in revolutions: het1 / 2 pi and sgnlt1 / 2 pi once per thread, then per
+// sample u = het1 n / 2 pi + sgnlt1 _tmp1 / 2 pi in FP64 and its fraction u - rint(u) (exact in
+// FP64 for |u| < 2^52) to the hardware sin/cos, which take revolutions: four FP64 operations per
+// sample fewer than the radians and two-term reduction of PhaseRot<float>.
+struct PhaseRev {
+  double h, s;  // het1 / 2 pi, sgnlt1 / 2 pi
+  __device__ __forceinline__ PhaseRev(double het1, double sgnlt1)
+      : h(het1 * 0.15915494309189533577), s(sgnlt1 * 0.15915494309189533577) {}
+  __device__ __forceinline__ float2 eval(double dn, double tmp1) const {
+    const double u = fma(h, dn, s * tmp1);
+    const float f = (float)(u - rint(u));
+    return make_float2(__builtin_amdgcn_cosf(f), -__builtin_amdgcn_sinf(f));
   }
 };
 

@@ -175,15 +175,21 @@ void launch_row(const Plan<T>& pl, cplx<T>* Z, int nbatch, hipStream_t st) {
 template <typename T>
 void launch_spin_col(const Plan<T>& pl, const SlotBases& sb, const SpinTab& tab, const TmplDesc* desc, cplx<T>* Z,
                      int nt, int Nl, const cplx<T>* hi, const cplx<T>* lo, bool seq, const ThrFuse<T>& tf,
-                     hipStream_t st) {
-  const SpinParams sp{pl.L1, pl.twshift};
+                     hipStream_t st, bool rev = false, bool lf = false) {
+  const SpinParams sp{pl.L1, pl.twshift, rev ? 1 : 0};
+  // loads-first demodulation: FP32 with 16 register rows (elsewhere the default kernel runs)
+  const bool lf_ = lf && std::is_same<T, float>::value;
 #define PGW_SCOL_NZ(P2, SQ, U, NZ)                                                                        \
   do {                                                                                                    \
     const long gx_ = pl.L1 / col_tile<T, P2>();                                                           \
     ThrFuse<T> tf_ = tf;                                                                                  \
     tf_.rows = tf.nt > 0 ? (int)(((long)tf.p.nparts * tf.nt + 4 * gx_ - 1) / (4 * gx_)) : 0;             \
-    hipLaunchKernelGGL((k_spin_col2<T, P2, SQ, U, NZ>), dim3(gx_, nt + tf_.rows), dim3(256), 0, st, sp, sb, tab, \
-                       desc, Z, pl.twc.p, hi, lo, tf_);                                                   \
+    if (lf_)                                                                                              \
+      hipLaunchKernelGGL((k_spin_col2<T, P2, SQ, U, NZ, std::is_same<T, float>::value && ColE<T, P2>::E == 16>), \
+                         dim3(gx_, nt + tf_.rows), dim3(256), 0, st, sp, sb, tab, desc, Z, pl.twc.p, hi, lo, tf_); \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_spin_col2<T, P2, SQ, U, NZ>), dim3(gx_, nt + tf_.rows), dim3(256), 0, st, sp, sb, tab, \
+                         desc, Z, pl.twc.p, hi, lo, tf_);                                                 \
   } while (0)
 #define PGW_SCOL_GO(P2, SQ, U)                                                                           \
   do {                                                                                                   \
@@ -389,6 +395,10 @@ template <typename T> struct Engine final : pgw_handle {
   int TB = 128, TBalloc = 0;
   int cand_cap = kCandCap;  // candidates per group in the device buffer (PGW_TUNE candcap= lowers it)
   int l1max = 0;            // FFT row length cap (0: the split_len table)
+  // FP32 form: demodulation phase in revolutions (PhaseRev; the batching column pass 1 % faster,
+  // identical candidates); rev=0 in PGW_TUNE restores the radians form
+  bool phase_rev = std::is_same<T, float>::value;
+  bool col_lf = false;      // FP32 column pass: loads-first demodulation (autotuned)
   int group = kSlots;  // sky points per group
   int row_na = 1;
   bool col_seq = true;  // column pass: the two spectra share one LDS buffer in turn
@@ -565,6 +575,8 @@ template <typename T> struct Engine final : pgw_handle {
       if (const char* q = strstr(tune, "thr_fuse=")) if (sscanf(q, "thr_fuse=%d", &v) == 1) thr_fuse = v != 0;
       if (const char* q = strstr(tune, "candcap=")) if (sscanf(q, "candcap=%d", &v) == 1 && v >= 1 && v <= kCandCap) cand_cap = v;
       if (const char* q = strstr(tune, "l1max=")) if (sscanf(q, "l1max=%d", &v) == 1 && (v == 1024 || v == 2048 || v == 4096)) l1max = v;
+      if (const char* q = strstr(tune, "rev=")) if (sscanf(q, "rev=%d", &v) == 1) phase_rev = v != 0;
+      if (const char* q = strstr(tune, "col_lf=")) if (sscanf(q, "col_lf=%d", &v) == 1) col_lf = v != 0;
     }
     // segments after the variants: their FFT plans follow l1max
     for (int k = 0; k < nseg; k++)
@@ -797,7 +809,7 @@ template <typename T> struct Engine final : pgw_handle {
       thr_prev.nt = 0;
     }
     launch_spin_col<T>(*g.ps, sb, tab, d, Z.p, nt, uni ? segs[uni->seg]->sett.N : Nl, hi.p, lo.p, col_seq, tf,
-                       stream);
+                       stream, phase_rev, col_lf);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));
     RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2)};
     launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, row_half, row8, stream);
@@ -1261,26 +1273,31 @@ template <typename T> struct Engine final : pgw_handle {
     };
     double best = 1e30;
     int btb = TB, bna = row_na, br8 = row8, st = PGW_OK;
-    bool bseq = col_seq;
+    bool bseq = col_seq, blf = col_lf;
+    // loads-first column pass: FP32 at column lengths of 128 and 256 (16 register rows)
+    const int nlf = std::is_same<T, float>::value && segs[sgi]->ps->L2 >= 128 ? 2 : 1;
     for (int tb : tbs) {
       if ((st = ensure_tb(tb))) break;
       for (int rv = 0; rv < nrv && !st; rv++)
-        for (int seq = 0; seq <= 1 && !st; seq++) {
-          TB = tb;
-          row_na = rv_na[rv];
-          row8 = rv_r8[rv];
-          col_seq = seq != 0;
-          double w, t1, t2;  // one untimed search, then the faster of two
-          if ((st = per_template(w)) || (st = per_template(t1)) || (st = per_template(t2))) break;
-          const double per = std::min(t1, t2);
-          if (per < best) {
-            best = per;
-            btb = tb;
-            bna = rv_na[rv];
-            br8 = rv_r8[rv];
-            bseq = seq != 0;
+        for (int seq = 0; seq <= 1 && !st; seq++)
+          for (int lf = 0; lf < nlf && !st; lf++) {
+            TB = tb;
+            row_na = rv_na[rv];
+            row8 = rv_r8[rv];
+            col_seq = seq != 0;
+            col_lf = lf != 0;
+            double w, t1, t2;  // one untimed search, then the faster of two
+            if ((st = per_template(w)) || (st = per_template(t1)) || (st = per_template(t2))) break;
+            const double per = std::min(t1, t2);
+            if (per < best) {
+              best = per;
+              btb = tb;
+              bna = rv_na[rv];
+              br8 = rv_r8[rv];
+              bseq = seq != 0;
+              blf = lf != 0;
+            }
           }
-        }
       if (st) break;
     }
     ktime = kt_saved;
@@ -1288,6 +1305,7 @@ template <typename T> struct Engine final : pgw_handle {
     row_na = bna;
     row8 = br8;
     col_seq = bseq;
+    col_lf = blf;
     if (st) return st;
     if (tb_out) *tb_out = TB;
     if (na_out) *na_out = row8 ? 8 : row_na;

@@ -3,8 +3,10 @@ records: row pass with 16 or 8 elements per thread, with and without the half-sp
 both spectra together, column pass with shared LDS, spline chunk lengths, launch and group
 sizes, thresholds folded into the next column launch or launched on their own, a device
 candidate buffer smaller than a group's candidates (the group runs again into a buffer of the
-needed size), FFT rows capped at 2048 or 1024 points (longer columns).  FP64 to 1e-9 relative in F; FP32 against the FP64 oracle by the rule of
-cases.fp32_record_diff."""
+needed size), FFT rows capped at 2048 or 1024 points (longer columns); for FP32 also the
+demodulation phase in radians (rev=0) and the loads-first column pass (col_lf=1, built for 16
+register rows: with l1max=1024 the columns of 128 points have them).  FP64 to 1e-9 relative in
+F; FP32 against the FP64 oracle by the rule of cases.fp32_record_diff."""
 import os
 
 import numpy as np
@@ -71,7 +73,9 @@ def test_variant_fp64(pgw, abi, oracle_runs, name, tune):
 
 
 @pytest.mark.parametrize("tune", ["", "row8=1", "row8=2", "half=0", "row_na=2", "spl_k=8", "thr_fuse=0",
-                                  "l1max=2048", "l1max=1024"])
+                                  "l1max=2048", "l1max=1024", "rev=0", "rev=0,tb=8",
+                                  "col_lf=1", "col_lf=1,rev=0", "col_lf=1,tb=8",
+                                  "col_lf=1,l1max=1024", "col_lf=1,l1max=1024,rev=0"])
 @pytest.mark.parametrize("name", CASES)
 def test_variant_fp32(pgw, abi, oracle_runs, name, tune):
     sky, ref, fr = oracle_runs[name]
