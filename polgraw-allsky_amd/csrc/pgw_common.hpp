@@ -174,6 +174,14 @@ template <int R, int DIR, typename T2> __device__ __forceinline__ T2 rot_r(T2 a,
   }
 }
 
+// LDS written by some lanes of a wave made visible to the wave's other lanes (no workgroup
+// barrier: only the wave's own accesses are ordered)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 template <int R, int DIR, typename T2> __device__ __forceinline__ void dftR(T2* v) {
   if constexpr (R == 2) dft2<DIR>(v);
   else if constexpr (R == 4) dft4<DIR>(v);

@@ -285,4 +285,81 @@ __device__ __forceinline__ void row_fft8(T2 (&x)[8], T2* lds, const T2* __restri
   }
 }
 
+// Row FFT with 8 elements per thread in which only one exchange crosses waves (NT = P1/8 = 64 R
+// threads, R = P1/512 waves; thread t = 64 w + l).  Four-step with P1 = 8 x NT: x[m] = row element
+// n = t + NT m on entry, so the 8-point DFT over m runs in registers (k1), times W_P1^{t k1}.
+// One block-wide exchange (barrier) hands the NT values of slice k1 to wave k1 / (8/R), slot
+// s = k1 mod (8/R).  Each wave then transforms its 8/R slices of length 64 R alone, in its own
+// region of lds: radix R over j (n2 = l + 64 j) in registers, times W_{64R}^{l k1''}, leaving
+// 8 sequences of 64 values over the lanes (register d = s R + k1''); these are 8 x 8 four-step
+// DFTs with two exchanges inside the region (padded q + q/8: conflict-free strided reads), so
+// no workgroup barrier follows the first.
+// On exit x[m] = X[row8w_k0<P1>(t) + NT m].  Region of wave w: lds[576 w, 576 w + 576); lds must
+// hold 576 R elements.  The caller must barrier before lds is written by other waves again.
+template <int P1> struct Row8wShape {
+  static constexpr int NT = P1 / 8;
+  static constexpr int R = P1 / 512;            // waves; radix of the in-wave first step
+  static constexpr int SPW = 8 / R;             // slices per wave
+  static constexpr int REG = 576;               // elements per wave region (512 + 64 padding)
+  static constexpr int LDSN = REG * R;
+  static_assert(R == 2 || R == 4 || R == 8, "row length 1024, 2048 or 4096");
+};
+
+// first output index of thread t: X[k0 + NT m] is its register m
+template <int P1> __device__ __forceinline__ int row8w_k0(int t) {
+  using S = Row8wShape<P1>;
+  const int w = t >> 6, l = t & 63, d = l >> 3, ka = l & 7;
+  const int s = d / S::R, k1pp = d % S::R;
+  return w * S::SPW + s + 8 * (k1pp + S::R * ka);
+}
+
+template <typename T2, int P1, int DIR>
+__device__ __forceinline__ void row_fft8w(T2 (&x)[8], T2* lds, const T2* __restrict__ twr) {
+  using S = Row8wShape<P1>;
+  constexpr int NT = S::NT, R = S::R, SPW = S::SPW, REG = S::REG;
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  // twiddle bases up front: W_P1^t, W_{64R}^l = W_P1^{8 l}, W_64^a = W_P1^{a P1/64}
+  const T2 w1 = twr[t];
+  const T2 w2 = twr[8 * l];
+  const T2 w3 = twr[(l & 7) * (P1 / 64)];
+  // step 1: 8-point DFT over n1 = m for n2 = t, times W_P1^{t k1}
+  dftR<8, DIR>(x);
+  twiddle_run<8, DIR>(x, w1);
+  // slice k1 -> wave k1 / SPW, slot k1 % SPW, position n2 = t
+#pragma unroll
+  for (int k1 = 0; k1 < 8; k1++) lds[(k1 / SPW) * REG + (k1 % SPW) * NT + t] = x[k1];
+  __syncthreads();
+  T2* reg = lds + w * REG;
+#pragma unroll
+  for (int s = 0; s < SPW; s++)
+#pragma unroll
+    for (int j = 0; j < R; j++) x[s * R + j] = reg[s * NT + l + 64 * j];
+  wave_lds_sync();
+  // radix R over j for each slot, times W_{64R}^{l k1''}: register d = s R + k1'' holds element l
+  // of the d-th 64-point sequence
+#pragma unroll
+  for (int s = 0; s < SPW; s++) {
+    dftR<R, DIR>(x + s * R);
+    twiddle_run<R, DIR>(x + s * R, w2);
+  }
+  auto q = [](int i) { return i + (i >> 3); };
+  // 64-point DFTs over l = a + 8 b: lane (d, a) gathers b = 0..7
+#pragma unroll
+  for (int d = 0; d < 8; d++) reg[q(64 * d + l)] = x[d];
+  wave_lds_sync();
+  const int dd = l >> 3, a = l & 7;
+#pragma unroll
+  for (int b = 0; b < 8; b++) x[b] = reg[q(64 * dd + a + 8 * b)];
+  wave_lds_sync();
+  dftR<8, DIR>(x);
+  twiddle_run<8, DIR>(x, w3);  // W_64^{a ka}
+  // lane (d, ka) gathers a = 0..7
+#pragma unroll
+  for (int ka = 0; ka < 8; ka++) reg[q(64 * dd + 8 * ka + a)] = x[ka];
+  wave_lds_sync();
+#pragma unroll
+  for (int aa = 0; aa < 8; aa++) x[aa] = reg[q(64 * dd + 8 * a + aa)];
+  dftR<8, DIR>(x);
+}
+
 }  // namespace pgw

@@ -416,11 +416,50 @@ struct SpinParams {
 };
 
 template <typename T> struct PhaseRot;
+// sin and cos of x in FP64 without a large-argument path: k = rint(2x / pi), x - k pi/2 by three
+// FMAs against pi/2 split in three doubles (the first two products exact inside the FMA, so the
+// reduced angle is off by ~1e-16 for |x| < 2^50), then Taylor series to degree 17 (sin) and 18
+// (cos) on |r| <= pi/4 (truncation < 1e-19), and the quadrant k mod 4.  Absolute error ~1.2e-16
+// (against 200-bit arithmetic, |x| up to 1e12); demodulation phases here are below 1e7.
+__device__ __forceinline__ void sincos_cw(double x, double& s, double& c) {
+  constexpr double P1 = 1.5707963267948966, P2 = 6.123233995736766e-17, P3 = -1.4973849048591698e-33;
+  const double k = rint(x * 0.6366197723675814);
+  double r = fma(-k, P1, x);
+  r = fma(-k, P2, r);
+  r = fma(-k, P3, r);
+  const double r2 = r * r;
+  // sin r = r + r^3 sum_{j>=1} (-1)^j r^(2j-2) / (2j+1)!,  cos r = 1 + r^2 sum_{j>=1} (-1)^j r^(2j-2) / (2j)!
+  double ps = 2.8114572543455206e-15;                 // 1/17!
+  ps = fma(ps, r2, -7.647163731819816e-13);           // -1/15!
+  ps = fma(ps, r2, 1.6059043836821613e-10);           // 1/13!
+  ps = fma(ps, r2, -2.505210838544172e-08);           // -1/11!
+  ps = fma(ps, r2, 2.7557319223985893e-06);           // 1/9!
+  ps = fma(ps, r2, -0.0001984126984126984);           // -1/7!
+  ps = fma(ps, r2, 0.008333333333333333);             // 1/5!
+  ps = fma(ps, r2, -0.16666666666666666);             // -1/3!
+  const double sn = fma(r * r2, ps, r);
+  double pc = 1.5619206968586225e-16;                 // 1/18!
+  pc = fma(pc, r2, -4.779477332387385e-14);           // -1/16!
+  pc = fma(pc, r2, 1.1470745597729725e-11);           // 1/14!
+  pc = fma(pc, r2, -2.08767569878681e-09);            // -1/12!
+  pc = fma(pc, r2, 2.755731922398589e-07);            // 1/10!
+  pc = fma(pc, r2, -2.48015873015873e-05);            // -1/8!
+  pc = fma(pc, r2, 0.001388888888888889);             // 1/6!
+  pc = fma(pc, r2, -0.041666666666666664);            // -1/4!
+  pc = fma(pc, r2, 0.5);                              // 1/2! (sign below)
+  const double cs = fma(-r2, pc, 1.0);
+  // quadrant k mod 4 (k is an integer in FP64; exact for |k| < 2^53)
+  const int q = (int)fma(-4.0, floor(k * 0.25), k);
+  const double s0 = (q & 1) ? cs : sn, c0 = (q & 1) ? sn : cs;
+  s = (q & 2) ? -s0 : s0;
+  c = ((q + 1) & 2) ? -c0 : c0;
+}
+
 template <> struct PhaseRot<double> {
   // exp(-i phase) in FP64
   __device__ static __forceinline__ double2 eval(double ph) {
     double s, c;
-    sincos(ph, &s, &c);
+    sincos_cw(ph, s, c);
     return make_double2(c, -s);
   }
 };
@@ -476,13 +515,6 @@ __device__ __forceinline__ double part_mean(const double sum, int part, const Th
 }
 
 constexpr int THR_MAXP = 256;  // FStat blocks per template at most
-
-// LDS written by some lanes of a wave made visible to the wave's other lanes
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
 
 // One wave (lane = 0..63) thresholds FStat block `part` of template t; mu: THR_MAXP doubles of LDS
 // owned by this wave.  TJ, QB: loads in flight per batch in the flagged path (bins, block means).

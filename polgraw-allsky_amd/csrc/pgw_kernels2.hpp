@@ -357,7 +357,10 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
 // Row pass with 8 elements per thread (row_fft8; P1/8 threads, two workgroups per CU within 128
 // VGPRs): the FP64 form of k_spin_row2, same outputs.  The spectra are transformed in turn through
 // one LDS array, the second row's loads in flight during the first transform (PREF).
-template <typename T, int P1, bool HALF, bool PREF>
+// WL: row_fft8w (one block-wide exchange per transform, the others inside each wave; 6 workgroup
+// barriers per row instead of 14); its outputs are a permutation of the threads', so F reaches
+// global memory through LDS (padded by one element per P1/64: conflict-free scattered writes).
+template <typename T, int P1, bool HALF, bool PREF, bool WL = false>
 __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const double* __restrict__ aabb_slots,
                                                          const TmplDesc* __restrict__ desc,
                                                          const cplx<T>* __restrict__ Z,
@@ -366,7 +369,7 @@ __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const doub
   using T2 = cplx<T>;
   using S = Row8Shape<P1>;
   constexpr int NT = S::NT;
-  __shared__ T2 lds[S::PADN];
+  __shared__ T2 lds[WL ? Row8wShape<P1>::LDSN : S::PADN];
   // templates in reverse launch order: the column pass wrote the last templates' spectra last, so
   // the first row workgroups find them still in the last-level cache (MALL)
   const int k2 = blockIdx.x, t = gridDim.y - 1 - blockIdx.y, tid = threadIdx.x;
@@ -387,34 +390,59 @@ __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const doub
 #pragma unroll
     for (int m = 0; m < 8; m++) y[m] = Zr.load<T2>(vr, rb + mb * m);
   }
-  row_fft8<T2, P1, -1>(x, lds, twr);
+  auto fft = [&]() {
+    if constexpr (WL) row_fft8w<T2, P1, -1>(x, lds, twr);
+    else row_fft8<T2, P1, -1>(x, lds, twr);
+  };
+  fft();
 #pragma unroll
   for (int m = 0; m < MO; m++) F[m] = (x[m].x * x[m].x + x[m].y * x[m].y) * raa;
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < 8; m++) x[m] = PREF ? y[PREF ? m : 0] : Zr.load<T2>(vr, rb + mb * m);
-  row_fft8<T2, P1, -1>(x, lds, twr);
+  fft();
 #pragma unroll
   for (int m = 0; m < MO; m++) F[m] = F[m] + (x[m].x * x[m].x + x[m].y * x[m].y) * rbb;
   __syncthreads();
   T* Fs = reinterpret_cast<T*>(lds);
+  // Fs position of row bin k1 (WL: one padding element per P1/64)
+  auto fpos = [](int k1) { return WL ? k1 + k1 / (P1 / 64) : k1; };
   const BufRes Fb(Ft + (size_t)t * L, (uint32_t)(L * sizeof(T)));
   const uint32_t vf = (uint32_t)tid * (uint32_t)sizeof(T), fo = (uint32_t)(k2 * P1) * (uint32_t)sizeof(T);
+  if constexpr (WL) {
+    // register m holds bin k0 + NT m: through LDS to the thread that stores bin tid + NT m
+    const int k0 = row8w_k0<P1>(tid);
 #pragma unroll
-  for (int m = 0; m < 8; m++) {
-    const int k1 = tid + NT * m;
-    if (m < MO) {
+    for (int m = 0; m < MO; m++) {
+      const int k1 = k0 + NT * m;
       const int k = p.L2 * k1 + k2;
-      const bool ok = (k >= p.nmin && k < p.nmax);
-      if (ok) Fb.store(F[m < MO ? m : 0], vf, fo + (uint32_t)(NT * m * sizeof(T)));
-      Fs[k1] = ok ? F[m < MO ? m : 0] : (T)0;
+      Fs[fpos(k1)] = (k >= p.nmin && k < p.nmax) ? F[m] : (T)0;
     }
-    // (HALF, m >= MO: the upper half of Fs, k1 >= P1/2, holds no bin of [nmin, nmax) and is not read)
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < MO; m++) {
+      const int k1 = tid + NT * m;
+      const int k = p.L2 * k1 + k2;
+      if (k >= p.nmin && k < p.nmax) Fb.store(Fs[fpos(k1)], vf, fo + (uint32_t)(NT * m * sizeof(T)));
+    }
+    // (the partials below read Fs and write rs, which the stores above do not touch: no barrier)
+  } else {
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      const int k1 = tid + NT * m;
+      if (m < MO) {
+        const int k = p.L2 * k1 + k2;
+        const bool ok = (k >= p.nmin && k < p.nmax);
+        if (ok) Fb.store(F[m < MO ? m : 0], vf, fo + (uint32_t)(NT * m * sizeof(T)));
+        Fs[k1] = ok ? F[m < MO ? m : 0] : (T)0;
+      }
+      // (HALF, m >= MO: the upper half of Fs, k1 >= P1/2, holds no bin of [nmin, nmax) and is not read)
+    }
+    __syncthreads();
   }
-  __syncthreads();
   // FStat block partials of this row as in k_spin_row2, with runs of 8 consecutive bins per thread
   // (a block spans 4096 / L2 >= 16 consecutive k1, so a run falls into at most two blocks).
-  T* rs = Fs + P1;   // [2][NT] sums, then [2][NT] maxima
+  T* rs = Fs + fpos(P1);   // [2][NT] sums, then [2][NT] maxima
   // HALF: threads whose 8 bins lie in the upper half (tid >= NT/2) have nothing to sum, and no
   // block's bin range reaches their entries of rs (its bins end below nmax <= L/2)
   if (!HALF || tid < NT / 2) {
@@ -426,7 +454,7 @@ __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const doub
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const int j = j0 + ((i + tid) & 7);  // rotated start: lanes spread over the LDS banks
-      const T f = Fs[j];
+      const T f = Fs[fpos(j)];
       if (j < jsplit) { s0 += f; m0 = f > m0 ? f : m0; }
       else { s1 += f; m1 = f > m1 ? f : m1; }
     }

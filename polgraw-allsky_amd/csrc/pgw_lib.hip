@@ -237,18 +237,23 @@ void launch_spin_row(const Plan<T>& pl, const RowParams& rp, const double* aabb,
       if (half) PGW_SROW_GO(P1, 1, true); else PGW_SROW_GO(P1, 1, false);                                 \
     }                                                                                                     \
     break;
-#define PGW_SROW8_GO(P1, H, PF)                                                                          \
-  hipLaunchKernelGGL((k_spin_row8<T, P1, H, PF>), dim3(pl.L2, nt), dim3(P1 / 8), 0, st, rp, aabb, desc, Z, pl.twr.p, \
+#define PGW_SROW8_GO(P1, H, PF, WL)                                                                      \
+  hipLaunchKernelGGL((k_spin_row8<T, P1, H, PF, WL>), dim3(pl.L2, nt), dim3(P1 / 8), 0, st, rp, aabb, desc, Z, pl.twr.p, \
                      Ft, psum, pmax)
+#define PGW_SROW8_H(P1, PF, WL)                                                                           \
+  do {                                                                                                    \
+    if (half) PGW_SROW8_GO(P1, true, PF, WL); else PGW_SROW8_GO(P1, false, PF, WL);                       \
+  } while (0)
 #define PGW_SROW8_CASE(P1)                                                                                \
   case P1:                                                                                                \
-    if (row8 == 1) {                                                                                      \
-      if (half) PGW_SROW8_GO(P1, true, true); else PGW_SROW8_GO(P1, false, true);                         \
-    } else {                                                                                              \
-      if (half) PGW_SROW8_GO(P1, true, false); else PGW_SROW8_GO(P1, false, false);                       \
-    }                                                                                                     \
+    if (row8 == 1) PGW_SROW8_H(P1, true, false);                                                          \
+    else if (row8 == 2) PGW_SROW8_H(P1, false, false);                                                    \
+    else if (row8 == 3) PGW_SROW8_H(P1, true, true);                                                      \
+    else PGW_SROW8_H(P1, false, true);                                                                    \
     break;
-  if (row8) {  // 8 elements per thread (1: second row prefetched, 2: not)
+  // 8 elements per thread: 1 second row prefetched, 2 not; 3 and 4 likewise with the
+  // transforms' later exchanges inside each wave (row_fft8w)
+  if (row8) {
     switch (pl.L1) {
       PGW_SROW8_CASE(1024)
       PGW_SROW8_CASE(2048)
@@ -262,6 +267,7 @@ void launch_spin_row(const Plan<T>& pl, const RowParams& rp, const double* aabb,
     }
   }
 #undef PGW_SROW8_CASE
+#undef PGW_SROW8_H
 #undef PGW_SROW8_GO
 #undef PGW_SROW_CASE
 #undef PGW_SROW_GO
@@ -404,9 +410,11 @@ template <typename T> struct Engine final : pgw_handle {
   bool col_seq = true;  // column pass: the two spectra share one LDS buffer in turn
   int spl_k = 4;  // spline outputs per thread (4, 8 or 16)
   bool row_half = true;  // row pass forms only the outputs that can fall in [nmin, nmax)
-  // row pass with 8 elements per thread (1 with the second row prefetched, 2 without): the FP64
-  // default (two workgroups per CU instead of one); FP32 keeps 16 elements per thread
-  int row8 = std::is_same<T, double>::value ? 2 : 0;
+  // row pass with 8 elements per thread (1 with the second row prefetched, 2 without; 3 and 4
+  // likewise with the transforms' later exchanges inside each wave): the FP64 default, 3 (two
+  // workgroups per CU instead of one, 6 workgroup barriers per row instead of 14); FP32 keeps 16
+  // elements per thread
+  int row8 = std::is_same<T, double>::value ? 3 : 0;
   // each launch's threshold folded into the next launch's column pass (the group's last one runs
   // on its own); thr_prev: the threshold still to run (nt = 0: none)
   bool thr_fuse = true;
@@ -571,7 +579,7 @@ template <typename T> struct Engine final : pgw_handle {
       if (const char* q = strstr(tune, "group=")) if (sscanf(q, "group=%d", &v) == 1 && v >= 1 && v <= kSlots) group = v;
       if (const char* q = strstr(tune, "spl_k=")) if (sscanf(q, "spl_k=%d", &v) == 1 && (v == 4 || v == 8 || v == 16)) spl_k = v;
       if (const char* q = strstr(tune, "half=")) if (sscanf(q, "half=%d", &v) == 1) row_half = v != 0;
-      if (const char* q = strstr(tune, "row8=")) if (sscanf(q, "row8=%d", &v) == 1 && v >= 0 && v <= 2) row8 = v;
+      if (const char* q = strstr(tune, "row8=")) if (sscanf(q, "row8=%d", &v) == 1 && v >= 0 && v <= 4) row8 = v;
       if (const char* q = strstr(tune, "thr_fuse=")) if (sscanf(q, "thr_fuse=%d", &v) == 1) thr_fuse = v != 0;
       if (const char* q = strstr(tune, "candcap=")) if (sscanf(q, "candcap=%d", &v) == 1 && v >= 1 && v <= kCandCap) cand_cap = v;
       if (const char* q = strstr(tune, "l1max=")) if (sscanf(q, "l1max=%d", &v) == 1 && (v == 1024 || v == 2048 || v == 4096)) l1max = v;
@@ -1242,9 +1250,11 @@ template <typename T> struct Engine final : pgw_handle {
     if (sgi < 0 || sgi >= (int)segs.size()) return PGW_ERR_ARG;
     const int tbs[3] = {64, 128, 256};
     // row pass variants: 16 elements per thread with the spectra in turn (na 1), 8 elements per
-    // thread without / with the second row prefetched (row8 2 / 1), 16 elements per thread with
-    // the spectra together (na 2).  FP32 keeps the first: the others are slower over a search.
-    const int rv_na[4] = {1, 1, 1, 2}, rv_r8[4] = {0, 2, 1, 0};
+    // thread with the transforms' later exchanges inside each wave, without / with the second row
+    // prefetched (row8 4 / 3), 16 elements per thread with the spectra together (na 2).  FP32
+    // keeps the first: the others are slower over a search.  (row8 1 and 2, all exchanges
+    // block-wide, are slower than 3 and 4 at every measured shape; PGW_TUNE still selects them.)
+    const int rv_na[4] = {1, 1, 1, 2}, rv_r8[4] = {0, 4, 3, 0};
     const int nrv = std::is_same<T, double>::value ? 4 : 1;
     constexpr int NI = 4, NS = 512;
     std::vector<int> work(4 * NI), ssoff(NI + 1), sslist((size_t)NI * NS);

@@ -1,5 +1,6 @@
 """Kernel variants (chosen per process with PGW_TUNE, or by the autotuner) give the oracle's
-records: row pass with 16 or 8 elements per thread, with and without the half-spectrum pruning,
+records: row pass with 16 or 8 elements per thread (the latter also with the transforms' later
+exchanges inside each wave, row8=3/4), with and without the half-spectrum pruning,
 both spectra together, column pass with shared LDS, spline chunk lengths, launch and group
 sizes, thresholds folded into the next column launch or launched on their own, a device
 candidate buffer smaller than a group's candidates (the group runs again into a buffer of the
@@ -18,7 +19,8 @@ pytestmark = pytest.mark.gpu
 
 VARIANTS = ["", "row8=0", "row8=1", "row8=2", "half=0", "row8=0,half=0", "row8=2,half=0", "row_na=2,row8=0",
             "col_seq=1", "spl_k=8", "spl_k=16", "tb=32,group=3", "thr_fuse=0", "tb=8", "candcap=16",
-            "candcap=16,group=1", "l1max=2048", "l1max=2048,row8=0", "l1max=1024"]
+            "candcap=16,group=1", "l1max=2048", "l1max=2048,row8=0", "l1max=1024", "row8=3", "row8=4",
+            "row8=3,half=0", "row8=4,half=0", "row8=3,l1max=2048", "row8=3,l1max=1024", "row8=4,tb=8"]
 CASES = ["plain-1det-32k", "net-2det-32k", "net-2det-86k"]
 
 
@@ -75,7 +77,8 @@ def test_variant_fp64(pgw, abi, oracle_runs, name, tune):
 @pytest.mark.parametrize("tune", ["", "row8=1", "row8=2", "half=0", "row_na=2", "spl_k=8", "thr_fuse=0",
                                   "l1max=2048", "l1max=1024", "rev=0", "rev=0,tb=8",
                                   "col_lf=1", "col_lf=1,rev=0", "col_lf=1,tb=8",
-                                  "col_lf=1,l1max=1024", "col_lf=1,l1max=1024,rev=0"])
+                                  "col_lf=1,l1max=1024", "col_lf=1,l1max=1024,rev=0", "row8=3", "row8=4",
+                                  "row8=3,half=0", "row8=3,l1max=1024"])
 @pytest.mark.parametrize("name", CASES)
 def test_variant_fp32(pgw, abi, oracle_runs, name, tune):
     sky, ref, fr = oracle_runs[name]
