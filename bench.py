@@ -258,6 +258,7 @@ def tune_handle(args, h, segi, seg, opts):
         return None
     pm, mm, nn = seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2])
     choice = h.autotune(segi, pm, mm, nn, opts)
+    choice["variants"] = h.tune_string()
     print(f"autotune: {choice}", file=sys.stderr, flush=True)
     return choice
 

@@ -213,12 +213,20 @@ int pgw_fstatnet_noalloc(pgw_handle *h, int seg, const double *pts, int npts, do
  * every kernel variant (templates per launch 64/128/256;
  * row pass with 16 elements per thread and both spectra together or in turn, or, in the FP64
  * form, with 8 elements per thread; column pass LDS sharing; in the FP32 form at column lengths
- * of 128 and 256, loads-first demodulation) and keeps the fastest for this handle.  The
- * chosen values are returned (pointers may be NULL): *row_na is 1 or 2 (16 elements per thread,
- * spectra in turn or together) or 8 (8 elements per thread); the loads-first choice is kept by
- * the handle and not returned. */
+ * of 128 and 256, loads-first demodulation; spectra written with or without non-temporal
+ * stores) and keeps the fastest for this handle.  The chosen values are returned (pointers may
+ * be NULL): *row_na is 1 or 2 (16 elements per thread, spectra in turn or together) or 8 (8
+ * elements per thread); the loads-first and store choices are kept by the handle and not
+ * returned. */
 int pgw_autotune(pgw_handle *h, int seg, int pm, int mm, int nn, const pgw_opts *opts, int *tb,
                  int *row_na, int *col_seq);
+
+/* The handle's kernel variants (no counterpart in the reference) as a PGW_TUNE string, e.g.
+ * "tb=128,row_na=1,row8=0,col_seq=1,col_lf=0,col_nt=0,rev=1,half=1,spl_k=4,group=16,thr_fuse=1,
+ * l1max=0,candcap=1048576": after pgw_autotune, the choice it made; setting PGW_TUNE to it before
+ * pgw_init reproduces it.  buf receives a NUL-terminated string; PGW_ERR_ARG when len is too
+ * small (256 bytes always suffice). */
+int pgw_tune_string(const pgw_handle *h, char *buf, int len);
 
 /* Debug tap: the normalised F-statistic F[nmin..nmax) of one template (pm, mm, nn, ss),
  * as job_core holds it just before thresholding (jobcore.c:635-651).  F has nmax-nmin entries. */

@@ -67,6 +67,11 @@ struct BufRes {
     using U = uint32_t __attribute__((ext_vector_type(2)));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U, v), r, (int)voff, (int)soff, 0);
   }
+  // non-temporal (cache policy NT): streamed out without keeping the line in L2
+  __device__ __forceinline__ void store_nt(float2 v, uint32_t voff, uint32_t soff) const {
+    using U = uint32_t __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U, v), r, (int)voff, (int)soff, 2);
+  }
 };
 
 template <typename T2> __device__ __forceinline__ T2 cadd(T2 a, T2 b) { return mk<T2>(a.x + b.x, a.y + b.y); }

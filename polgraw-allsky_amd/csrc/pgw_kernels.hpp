@@ -413,6 +413,7 @@ struct SpinTab {
 struct SpinParams {
   int L1, twshift;
   int rev;  // FP32 form: phase accumulated in revolutions (PhaseRev) instead of radians
+  int nt;   // spectra written with non-temporal stores
 };
 
 template <typename T> struct PhaseRot;

@@ -169,8 +169,21 @@ __global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, c
   const uint32_t voff = (uint32_t)(n1 + g * S::Q * L1) * (uint32_t)sizeof(T2);
   const uint32_t rowb = (uint32_t)L1 * (uint32_t)sizeof(T2), specb = (uint32_t)(L * sizeof(T2));
   auto put = [&](T2 v, uint32_t soff) {
-    if constexpr (sizeof(T2) == 8) Zb.store(v, voff, soff);
-    else *reinterpret_cast<T2*>(reinterpret_cast<char*>(Zt) + voff + soff) = v;
+    if constexpr (sizeof(T2) == 8) {
+      if (p.nt) Zb.store_nt(v, voff, soff);
+      else Zb.store(v, voff, soff);
+    } else {
+      T2* q = reinterpret_cast<T2*>(reinterpret_cast<char*>(Zt) + voff + soff);
+      if (p.nt) {
+        using V = double __attribute__((ext_vector_type(2)));
+        V w;
+        w.x = v.x;
+        w.y = v.y;
+        __builtin_nontemporal_store(w, reinterpret_cast<V*>(q));
+      } else {
+        *q = v;
+      }
+    }
   };
   T2 w[E];
   col_out_twiddles<P2, E>(tw_big(hi, lo, (uint32_t)n1 << p.twshift), w);

@@ -175,8 +175,8 @@ void launch_row(const Plan<T>& pl, cplx<T>* Z, int nbatch, hipStream_t st) {
 template <typename T>
 void launch_spin_col(const Plan<T>& pl, const SlotBases& sb, const SpinTab& tab, const TmplDesc* desc, cplx<T>* Z,
                      int nt, int Nl, const cplx<T>* hi, const cplx<T>* lo, bool seq, const ThrFuse<T>& tf,
-                     hipStream_t st, bool rev = false, bool lf = false) {
-  const SpinParams sp{pl.L1, pl.twshift, rev ? 1 : 0};
+                     hipStream_t st, bool rev = false, bool lf = false, bool nts = false) {
+  const SpinParams sp{pl.L1, pl.twshift, rev ? 1 : 0, nts ? 1 : 0};
   // loads-first demodulation: FP32 with 16 register rows (elsewhere the default kernel runs)
   const bool lf_ = lf && std::is_same<T, float>::value;
 #define PGW_SCOL_NZ(P2, SQ, U, NZ)                                                                        \
@@ -328,6 +328,7 @@ struct pgw_handle {
   virtual int time_stages(int seg, int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps,
                           double* ms) = 0;
   virtual int autotune(int seg, int pm, int mm, int nn, const pgw_opts* o, int* tb, int* row_na, int* col_seq) = 0;
+  virtual int tune_string(char* buf, int len) const = 0;
   virtual int kernel_times(int enable, double* out) = 0;
   virtual int fstatnet(int seg, const double* pts, int n, double* out, double* grad) = 0;
   virtual int fstatnet_noalloc(int seg, const double* d_pts, int n, double* d_out, double* d_grad) = 0;
@@ -405,6 +406,9 @@ template <typename T> struct Engine final : pgw_handle {
   // identical candidates); rev=0 in PGW_TUNE restores the radians form
   bool phase_rev = std::is_same<T, float>::value;
   bool col_lf = false;      // FP32 column pass: loads-first demodulation (autotuned)
+  // column pass: spectra written with non-temporal stores (autotuned: faster where a launch's
+  // spectra outgrow the last-level cache, slower where the row pass would find them there)
+  bool col_nt = false;
   int group = kSlots;  // sky points per group
   int row_na = 1;
   bool col_seq = true;  // column pass: the two spectra share one LDS buffer in turn
@@ -585,6 +589,7 @@ template <typename T> struct Engine final : pgw_handle {
       if (const char* q = strstr(tune, "l1max=")) if (sscanf(q, "l1max=%d", &v) == 1 && (v == 1024 || v == 2048 || v == 4096)) l1max = v;
       if (const char* q = strstr(tune, "rev=")) if (sscanf(q, "rev=%d", &v) == 1) phase_rev = v != 0;
       if (const char* q = strstr(tune, "col_lf=")) if (sscanf(q, "col_lf=%d", &v) == 1) col_lf = v != 0;
+      if (const char* q = strstr(tune, "col_nt=")) if (sscanf(q, "col_nt=%d", &v) == 1) col_nt = v != 0;
     }
     // segments after the variants: their FFT plans follow l1max
     for (int k = 0; k < nseg; k++)
@@ -817,7 +822,7 @@ template <typename T> struct Engine final : pgw_handle {
       thr_prev.nt = 0;
     }
     launch_spin_col<T>(*g.ps, sb, tab, d, Z.p, nt, uni ? segs[uni->seg]->sett.N : Nl, hi.p, lo.p, col_seq, tf,
-                       stream, phase_rev, col_lf);
+                       stream, phase_rev, col_lf, col_nt);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));
     RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2)};
     launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, row_half, row8, stream);
@@ -1246,6 +1251,17 @@ template <typename T> struct Engine final : pgw_handle {
   // Times every kernel variant on a small search of sky point (pm, mm, nn) of segment sgi (four
   // copies of it with spindowns 0..511, so launches run back to back as in a search; threshold
   // out of reach so no records) and keeps the fastest.
+  // the kernel variants in PGW_TUNE's syntax (setting PGW_TUNE to it reproduces them)
+  int tune_string(char* buf, int len) const override {
+    if (!buf || len <= 0) return PGW_ERR_ARG;
+    const int n = snprintf(buf, (size_t)len,
+                           "tb=%d,row_na=%d,row8=%d,col_seq=%d,col_lf=%d,col_nt=%d,rev=%d,half=%d,spl_k=%d,"
+                           "group=%d,thr_fuse=%d,l1max=%d,candcap=%d",
+                           TB, row_na, row8, col_seq ? 1 : 0, col_lf ? 1 : 0, col_nt ? 1 : 0, phase_rev ? 1 : 0,
+                           row_half ? 1 : 0, spl_k, group, thr_fuse ? 1 : 0, l1max, cand_cap);
+    return n >= 0 && n < len ? PGW_OK : PGW_ERR_ARG;
+  }
+
   int autotune(int sgi, int pm, int mm, int nn, const pgw_opts* o, int* tb_out, int* na_out, int* seq_out) override {
     if (sgi < 0 || sgi >= (int)segs.size()) return PGW_ERR_ARG;
     const int tbs[3] = {64, 128, 256};
@@ -1283,31 +1299,34 @@ template <typename T> struct Engine final : pgw_handle {
     };
     double best = 1e30;
     int btb = TB, bna = row_na, br8 = row8, st = PGW_OK;
-    bool bseq = col_seq, blf = col_lf;
+    bool bseq = col_seq, blf = col_lf, bnt = col_nt;
     // loads-first column pass: FP32 at column lengths of 128 and 256 (16 register rows)
     const int nlf = std::is_same<T, float>::value && segs[sgi]->ps->L2 >= 128 ? 2 : 1;
     for (int tb : tbs) {
       if ((st = ensure_tb(tb))) break;
       for (int rv = 0; rv < nrv && !st; rv++)
         for (int seq = 0; seq <= 1 && !st; seq++)
-          for (int lf = 0; lf < nlf && !st; lf++) {
-            TB = tb;
-            row_na = rv_na[rv];
-            row8 = rv_r8[rv];
-            col_seq = seq != 0;
-            col_lf = lf != 0;
-            double w, t1, t2;  // one untimed search, then the faster of two
-            if ((st = per_template(w)) || (st = per_template(t1)) || (st = per_template(t2))) break;
-            const double per = std::min(t1, t2);
-            if (per < best) {
-              best = per;
-              btb = tb;
-              bna = rv_na[rv];
-              br8 = rv_r8[rv];
-              bseq = seq != 0;
-              blf = lf != 0;
+          for (int lf = 0; lf < nlf && !st; lf++)
+            for (int nts = 0; nts <= 1 && !st; nts++) {
+              TB = tb;
+              row_na = rv_na[rv];
+              row8 = rv_r8[rv];
+              col_seq = seq != 0;
+              col_lf = lf != 0;
+              col_nt = nts != 0;
+              double w, t1, t2;  // one untimed search, then the faster of two
+              if ((st = per_template(w)) || (st = per_template(t1)) || (st = per_template(t2))) break;
+              const double per = std::min(t1, t2);
+              if (per < best) {
+                best = per;
+                btb = tb;
+                bna = rv_na[rv];
+                br8 = rv_r8[rv];
+                bseq = seq != 0;
+                blf = lf != 0;
+                bnt = nts != 0;
+              }
             }
-          }
       if (st) break;
     }
     ktime = kt_saved;
@@ -1316,6 +1335,7 @@ template <typename T> struct Engine final : pgw_handle {
     row8 = br8;
     col_seq = bseq;
     col_lf = blf;
+    col_nt = bnt;
     if (st) return st;
     if (tb_out) *tb_out = TB;
     if (na_out) *na_out = row8 ? 8 : row_na;
@@ -1468,6 +1488,11 @@ int pgw_autotune(pgw_handle* h, int seg, int pm, int mm, int nn, const pgw_opts*
                  int* col_seq) {
   if (!h || !opts) return PGW_ERR_ARG;
   return h->autotune(seg, pm, mm, nn, opts, tb, row_na, col_seq);
+}
+
+int pgw_tune_string(const pgw_handle* h, char* buf, int len) {
+  if (!h) return PGW_ERR_ARG;
+  return h->tune_string(buf, len);
 }
 
 int pgw_debug_fstat(pgw_handle* h, int pm, int mm, int nn, int ss, const pgw_opts* opts, double* F) {

@@ -53,6 +53,7 @@ def lib():
                                           _ip, _dp, C.c_int, _ip, _ip]
         L.pgw_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts),
                                    _ip, _ip, _ip]
+        L.pgw_tune_string.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
         L.pgw_fstatnet.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp]
         L.pgw_fstatnet_multi.argtypes = [C.c_void_p, _ip, _dp, C.c_int, _dp, _dp]
         # device addresses as integers (caller-owned device buffers)
@@ -78,7 +79,7 @@ def lib():
 
 EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_job_core", "pgw_job_batch",
             "pgw_job_core_f", "pgw_job_batch_f",
-            "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_fstatnet", "pgw_fstatnet_multi", "pgw_fstatnet_noalloc",
+            "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_tune_string", "pgw_fstatnet", "pgw_fstatnet_multi", "pgw_fstatnet_noalloc",
             "pgw_get_settings",
             "pgw_search", "pgw_read_checkpoint", "pgw_job_spotlight",
             "pgw_debug_fstat", "pgw_debug_resampled", "pgw_time_stages", "pgw_copy_bench", "pgw_last_gpu_ms",
@@ -271,6 +272,14 @@ class Handle:
         if st != 0:
             raise PGWError(st, "pgw_autotune")
         return dict(tb=tb.value, row_na=na.value, col_seq=seq.value)
+
+    def tune_string(self):
+        """The handle's kernel variants in PGW_TUNE syntax (pgw_tune_string)."""
+        buf = C.create_string_buffer(256)
+        st = lib().pgw_tune_string(self.h, buf, len(buf))
+        if st != 0:
+            raise PGWError(st, "pgw_tune_string")
+        return buf.value.decode()
 
     def fstat(self, pm, mm, nn, ss, opts):
         F = np.zeros(self.sett.nmax - self.sett.nmin)

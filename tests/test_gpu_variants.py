@@ -20,7 +20,8 @@ pytestmark = pytest.mark.gpu
 VARIANTS = ["", "row8=0", "row8=1", "row8=2", "half=0", "row8=0,half=0", "row8=2,half=0", "row_na=2,row8=0",
             "col_seq=1", "spl_k=8", "spl_k=16", "tb=32,group=3", "thr_fuse=0", "tb=8", "candcap=16",
             "candcap=16,group=1", "l1max=2048", "l1max=2048,row8=0", "l1max=1024", "row8=3", "row8=4",
-            "row8=3,half=0", "row8=4,half=0", "row8=3,l1max=2048", "row8=3,l1max=1024", "row8=4,tb=8"]
+            "row8=3,half=0", "row8=4,half=0", "row8=3,l1max=2048", "row8=3,l1max=1024", "row8=4,tb=8",
+            "col_nt=1", "col_nt=1,tb=8", "col_nt=1,col_seq=0", "col_nt=1,l1max=1024"]
 CASES = ["plain-1det-32k", "net-2det-32k", "net-2det-86k"]
 
 
@@ -78,7 +79,8 @@ def test_variant_fp64(pgw, abi, oracle_runs, name, tune):
                                   "l1max=2048", "l1max=1024", "rev=0", "rev=0,tb=8",
                                   "col_lf=1", "col_lf=1,rev=0", "col_lf=1,tb=8",
                                   "col_lf=1,l1max=1024", "col_lf=1,l1max=1024,rev=0", "row8=3", "row8=4",
-                                  "row8=3,half=0", "row8=3,l1max=1024"])
+                                  "row8=3,half=0", "row8=3,l1max=1024", "col_nt=1", "col_nt=1,tb=8",
+                                  "col_nt=1,col_lf=1,l1max=1024"])
 @pytest.mark.parametrize("name", CASES)
 def test_variant_fp32(pgw, abi, oracle_runs, name, tune):
     sky, ref, fr = oracle_runs[name]
@@ -184,3 +186,29 @@ def test_eight_detectors_and_whitenoise(pgw, abi, orc, white):
     assert np.array_equal(got[:, :4], ref[:, :4])
     Fg, Fr = got[:, 4] ** 2 / 2 + 2, ref[:, 4] ** 2 / 2 + 2
     assert np.max(np.abs(Fg - Fr) / Fr) <= 1e-9
+
+
+def test_tune_string(pgw, abi):
+    """pgw_tune_string reports the variants PGW_TUNE pinned, and a handle created with that string
+    reports the same string (defaults included); the autotuner's choice is reported in the same
+    form."""
+    seg = segment("net-2det-32k")
+    old = os.environ.get("PGW_TUNE")
+    try:
+        os.environ["PGW_TUNE"] = "tb=64,col_seq=0,row8=4,col_nt=1,rev=0"
+        s = pgw.Handle(seg, abi.PREC_FP32).tune_string()
+        kv = dict(x.split("=") for x in s.split(","))
+        assert (kv["tb"], kv["col_seq"], kv["row8"], kv["col_nt"], kv["rev"]) == ("64", "0", "4", "1", "0"), s
+        os.environ["PGW_TUNE"] = s
+        assert pgw.Handle(seg, abi.PREC_FP32).tune_string() == s
+    finally:
+        if old is None:
+            os.environ.pop("PGW_TUNE", None)
+        else:
+            os.environ["PGW_TUNE"] = old
+    h = pgw.Handle(seg, abi.PREC_FP64)
+    pm, mm, nn = injected_sky(seg)
+    choice = h.autotune(0, pm, mm, nn, abi.make_opts(trl=12.0))
+    kv = dict(x.split("=") for x in h.tune_string().split(","))
+    print(choice, kv, flush=True)
+    assert int(kv["tb"]) == choice["tb"] and int(kv["col_seq"]) == choice["col_seq"]
