@@ -47,13 +47,13 @@ struct BufRes {
   __amdgpu_buffer_rsrc_t r;
   __device__ __forceinline__ BufRes(const void* base, uint32_t bytes)
       : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000)) {}
-  // V = double, float2 or double2 (8 or 16 bytes)
-  template <typename V> __device__ __forceinline__ V load(uint32_t voff, uint32_t soff) const {
+  // V = double, float2 or double2 (8 or 16 bytes); AUX = cache policy (0, or 2: non-temporal)
+  template <typename V, int AUX = 0> __device__ __forceinline__ V load(uint32_t voff, uint32_t soff) const {
     if constexpr (sizeof(V) == 8) {
-      return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+      return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, AUX));
     } else {
       static_assert(sizeof(V) == 16, "8- or 16-byte loads");
-      return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+      return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, AUX));
     }
   }
   __device__ __forceinline__ void store(float v, uint32_t voff, uint32_t soff) const {

@@ -493,6 +493,7 @@ struct PhaseRev {
 
 struct RowParams {
   int L2, nmin, nmax, nblk, nparts, lgL2;  // L2 = 1 << lgL2
+  int nt;  // spectra read with non-temporal loads
 };
 
 struct Cand {

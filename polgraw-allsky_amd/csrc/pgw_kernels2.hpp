@@ -255,6 +255,7 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
   const BufRes Zr(Z + (size_t)t * 2 * L, (uint32_t)(2 * L * sizeof(T2)));
   const uint32_t vr = (uint32_t)tid * (uint32_t)sizeof(T2), mb = NT * (uint32_t)sizeof(T2);
   const uint32_t ra = (uint32_t)(k2 * P1) * (uint32_t)sizeof(T2), rb = ra + (uint32_t)(L * sizeof(T2));
+  auto ldz = [&](uint32_t off) { return p.nt ? Zr.load<T2, 2>(vr, off) : Zr.load<T2>(vr, off); };
   const double* aabb = aabb_slots + 2 * desc[t].slot;
   // F = |xa|^2/aa + |xb|^2/bb (jobcore.c:635-638) with the two reciprocals taken once: each
   // product differs from the quotient by at most one rounding
@@ -264,7 +265,7 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
   if constexpr (NA == 2) {
     T2 x[2][16];
 #pragma unroll
-    for (int m = 0; m < 16; m++) { x[0][m] = Zr.load<T2>(vr, ra + mb * m); x[1][m] = Zr.load<T2>(vr, rb + mb * m); }
+    for (int m = 0; m < 16; m++) { x[0][m] = ldz(ra + mb * m); x[1][m] = ldz(rb + mb * m); }
     row_fft<T2, P1, 2, -1>(x, lds, twr);
 #pragma unroll
     for (int m = 0; m < MO; m++)
@@ -273,10 +274,10 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
     constexpr bool PREF = RowOcc<T, P1, NA>::PREF;
     T2 x[1][16], y[1][PREF ? 16 : 1];
 #pragma unroll
-    for (int m = 0; m < 16; m++) x[0][m] = Zr.load<T2>(vr, ra + mb * m);
+    for (int m = 0; m < 16; m++) x[0][m] = ldz(ra + mb * m);
     if constexpr (PREF) {
 #pragma unroll
-      for (int m = 0; m < 16; m++) y[0][m] = Zr.load<T2>(vr, rb + mb * m);  // in flight during the first FFT
+      for (int m = 0; m < 16; m++) y[0][m] = ldz(rb + mb * m);  // in flight during the first FFT
     }
     // FP32: the stage-C twiddle powers of the first transform serve the second (no more
     // registers at the peak: they live while the prefetched row no longer does)
@@ -295,7 +296,7 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
     for (int m = 0; m < MO; m++) F[m] = (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * raa;
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < 16; m++) x[0][m] = PREF ? y[0][PREF ? m : 0] : Zr.load<T2>(vr, rb + mb * m);
+    for (int m = 0; m < 16; m++) x[0][m] = PREF ? y[0][PREF ? m : 0] : ldz(rb + mb * m);
     row_fft<T2, P1, 1, -1>(x, lds, twr, PC ? pc : nullptr, true, pbp);
 #pragma unroll
     for (int m = 0; m < MO; m++) F[m] = F[m] + (x[0][m].x * x[0][m].x + x[0][m].y * x[0][m].y) * rbb;
@@ -392,16 +393,17 @@ __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const doub
   const BufRes Zr(Z + (size_t)t * 2 * L, (uint32_t)(2 * L * sizeof(T2)));
   const uint32_t vr = (uint32_t)tid * (uint32_t)sizeof(T2), mb = NT * (uint32_t)sizeof(T2);
   const uint32_t ra = (uint32_t)(k2 * P1) * (uint32_t)sizeof(T2), rb = ra + (uint32_t)(L * sizeof(T2));
+  auto ldz = [&](uint32_t off) { return p.nt ? Zr.load<T2, 2>(vr, off) : Zr.load<T2>(vr, off); };
   const double* aabb = aabb_slots + 2 * desc[t].slot;
   const T raa = (T)(1.0 / ld_global(aabb, 0)), rbb = (T)(1.0 / ld_global(aabb, 1));
   constexpr int MO = HALF ? 4 : 8;  // output slots that can hold a bin of [nmin, nmax)
   T F[MO];
   T2 x[8], y[PREF ? 8 : 1];
 #pragma unroll
-  for (int m = 0; m < 8; m++) x[m] = Zr.load<T2>(vr, ra + mb * m);
+  for (int m = 0; m < 8; m++) x[m] = ldz(ra + mb * m);
   if constexpr (PREF) {
 #pragma unroll
-    for (int m = 0; m < 8; m++) y[m] = Zr.load<T2>(vr, rb + mb * m);
+    for (int m = 0; m < 8; m++) y[m] = ldz(rb + mb * m);
   }
   auto fft = [&]() {
     if constexpr (WL) row_fft8w<T2, P1, -1>(x, lds, twr);
@@ -412,7 +414,7 @@ __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const doub
   for (int m = 0; m < MO; m++) F[m] = (x[m].x * x[m].x + x[m].y * x[m].y) * raa;
   __syncthreads();
 #pragma unroll
-  for (int m = 0; m < 8; m++) x[m] = PREF ? y[PREF ? m : 0] : Zr.load<T2>(vr, rb + mb * m);
+  for (int m = 0; m < 8; m++) x[m] = PREF ? y[PREF ? m : 0] : ldz(rb + mb * m);
   fft();
 #pragma unroll
   for (int m = 0; m < MO; m++) F[m] = F[m] + (x[m].x * x[m].x + x[m].y * x[m].y) * rbb;

@@ -21,7 +21,8 @@ VARIANTS = ["", "row8=0", "row8=1", "row8=2", "half=0", "row8=0,half=0", "row8=2
             "col_seq=1", "spl_k=8", "spl_k=16", "tb=32,group=3", "thr_fuse=0", "tb=8", "candcap=16",
             "candcap=16,group=1", "l1max=2048", "l1max=2048,row8=0", "l1max=1024", "row8=3", "row8=4",
             "row8=3,half=0", "row8=4,half=0", "row8=3,l1max=2048", "row8=3,l1max=1024", "row8=4,tb=8",
-            "col_nt=1", "col_nt=1,tb=8", "col_nt=1,col_seq=0", "col_nt=1,l1max=1024"]
+            "col_nt=1", "col_nt=1,tb=8", "col_nt=1,col_seq=0", "col_nt=1,l1max=1024", "row_nt=0",
+            "row_nt=0,row8=0", "row_nt=0,row8=1"]
 CASES = ["plain-1det-32k", "net-2det-32k", "net-2det-86k"]
 
 
@@ -80,7 +81,7 @@ def test_variant_fp64(pgw, abi, oracle_runs, name, tune):
                                   "col_lf=1", "col_lf=1,rev=0", "col_lf=1,tb=8",
                                   "col_lf=1,l1max=1024", "col_lf=1,l1max=1024,rev=0", "row8=3", "row8=4",
                                   "row8=3,half=0", "row8=3,l1max=1024", "col_nt=1", "col_nt=1,tb=8",
-                                  "col_nt=1,col_lf=1,l1max=1024"])
+                                  "col_nt=1,col_lf=1,l1max=1024", "row_nt=0", "row_nt=0,row_na=2"])
 @pytest.mark.parametrize("name", CASES)
 def test_variant_fp32(pgw, abi, oracle_runs, name, tune):
     sky, ref, fr = oracle_runs[name]
@@ -195,10 +196,11 @@ def test_tune_string(pgw, abi):
     seg = segment("net-2det-32k")
     old = os.environ.get("PGW_TUNE")
     try:
-        os.environ["PGW_TUNE"] = "tb=64,col_seq=0,row8=4,col_nt=1,rev=0"
+        os.environ["PGW_TUNE"] = "tb=64,col_seq=0,row8=4,col_nt=1,row_nt=0,rev=0"
         s = pgw.Handle(seg, abi.PREC_FP32).tune_string()
         kv = dict(x.split("=") for x in s.split(","))
-        assert (kv["tb"], kv["col_seq"], kv["row8"], kv["col_nt"], kv["rev"]) == ("64", "0", "4", "1", "0"), s
+        assert (kv["tb"], kv["col_seq"], kv["row8"], kv["col_nt"], kv["row_nt"], kv["rev"]) == \
+            ("64", "0", "4", "1", "0", "0"), s
         os.environ["PGW_TUNE"] = s
         assert pgw.Handle(seg, abi.PREC_FP32).tune_string() == s
     finally:
