@@ -222,8 +222,9 @@ int pgw_autotune(pgw_handle *h, int seg, int pm, int mm, int nn, const pgw_opts 
                  int *row_na, int *col_seq);
 
 /* The handle's kernel variants (no counterpart in the reference) as a PGW_TUNE string, e.g.
- * "tb=128,row_na=1,row8=0,col_seq=1,col_lf=0,col_nt=0,row_nt=1,rev=1,half=1,spl_k=4,group=16,
- * thr_fuse=1,l1max=0,candcap=1048576": after pgw_autotune, the choice it made; setting PGW_TUNE to it before
+ * "tb=128,row_na=1,row8=0,col_seq=1,col_lf=0,col_nt=0,row_nt=1,f_nt=1,rev=1,half=1,spl_k=4,
+ * group=16,thr_fuse=1,l1max=0,candcap=1048576": after pgw_autotune, the choice it made; setting
+ * PGW_TUNE to it before
  * pgw_init reproduces it.  buf receives a NUL-terminated string; PGW_ERR_ARG when len is too
  * small (256 bytes always suffice). */
 int pgw_tune_string(const pgw_handle *h, char *buf, int len);

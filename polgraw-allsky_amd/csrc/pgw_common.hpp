@@ -56,12 +56,12 @@ struct BufRes {
       return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, AUX));
     }
   }
-  __device__ __forceinline__ void store(float v, uint32_t voff, uint32_t soff) const {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+  template <int AUX = 0> __device__ __forceinline__ void store(float v, uint32_t voff, uint32_t soff) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, AUX);
   }
-  __device__ __forceinline__ void store(double v, uint32_t voff, uint32_t soff) const {
+  template <int AUX = 0> __device__ __forceinline__ void store(double v, uint32_t voff, uint32_t soff) const {
     using U = uint32_t __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U, v), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U, v), r, (int)voff, (int)soff, AUX);
   }
   __device__ __forceinline__ void store(float2 v, uint32_t voff, uint32_t soff) const {
     using U = uint32_t __attribute__((ext_vector_type(2)));

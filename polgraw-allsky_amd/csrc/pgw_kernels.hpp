@@ -493,7 +493,8 @@ struct PhaseRev {
 
 struct RowParams {
   int L2, nmin, nmax, nblk, nparts, lgL2;  // L2 = 1 << lgL2
-  int nt;  // spectra read with non-temporal loads
+  int nt;   // spectra read with non-temporal loads
+  int fnt;  // F written with non-temporal stores
 };
 
 struct Cand {

@@ -305,13 +305,17 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
   T* Fs = reinterpret_cast<T*>(lds);
   const BufRes Fb(Ft + (size_t)t * L, (uint32_t)(L * sizeof(T)));
   const uint32_t vf = (uint32_t)tid * (uint32_t)sizeof(T), fo = (uint32_t)(k2 * P1) * (uint32_t)sizeof(T);
+  auto stf = [&](T v, uint32_t soff) {
+    if (p.fnt) Fb.template store<2>(v, vf, soff);
+    else Fb.store(v, vf, soff);
+  };
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const int k1 = tid + NT * m;
     if (m < MO) {
       const int k = p.L2 * k1 + k2;
       const bool ok = (k >= p.nmin && k < p.nmax);
-      if (ok) Fb.store(F[m < MO ? m : 0], vf, fo + (uint32_t)(NT * m * sizeof(T)));
+      if (ok) stf(F[m < MO ? m : 0], fo + (uint32_t)(NT * m * sizeof(T)));
       Fs[k1] = ok ? F[m < MO ? m : 0] : (T)0;
     }
     // (HALF, m >= MO: the upper half of Fs, k1 >= P1/2, holds no bin of [nmin, nmax) and is not read)
@@ -424,6 +428,10 @@ __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const doub
   auto fpos = [](int k1) { return WL ? k1 + k1 / (P1 / 64) : k1; };
   const BufRes Fb(Ft + (size_t)t * L, (uint32_t)(L * sizeof(T)));
   const uint32_t vf = (uint32_t)tid * (uint32_t)sizeof(T), fo = (uint32_t)(k2 * P1) * (uint32_t)sizeof(T);
+  auto stf = [&](T v, uint32_t soff) {
+    if (p.fnt) Fb.template store<2>(v, vf, soff);
+    else Fb.store(v, vf, soff);
+  };
   if constexpr (WL) {
     // register m holds bin k0 + NT m: through LDS to the thread that stores bin tid + NT m
     const int k0 = row8w_k0<P1>(tid);
@@ -438,7 +446,7 @@ __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const doub
     for (int m = 0; m < MO; m++) {
       const int k1 = tid + NT * m;
       const int k = p.L2 * k1 + k2;
-      if (k >= p.nmin && k < p.nmax) Fb.store(Fs[fpos(k1)], vf, fo + (uint32_t)(NT * m * sizeof(T)));
+      if (k >= p.nmin && k < p.nmax) stf(Fs[fpos(k1)], fo + (uint32_t)(NT * m * sizeof(T)));
     }
     // (the partials below read Fs and write rs, which the stores above do not touch: no barrier)
   } else {
@@ -448,7 +456,7 @@ __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const doub
       if (m < MO) {
         const int k = p.L2 * k1 + k2;
         const bool ok = (k >= p.nmin && k < p.nmax);
-        if (ok) Fb.store(F[m < MO ? m : 0], vf, fo + (uint32_t)(NT * m * sizeof(T)));
+        if (ok) stf(F[m < MO ? m : 0], fo + (uint32_t)(NT * m * sizeof(T)));
         Fs[k1] = ok ? F[m < MO ? m : 0] : (T)0;
       }
       // (HALF, m >= MO: the upper half of Fs, k1 >= P1/2, holds no bin of [nmin, nmax) and is not read)

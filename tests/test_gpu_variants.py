@@ -22,7 +22,7 @@ VARIANTS = ["", "row8=0", "row8=1", "row8=2", "half=0", "row8=0,half=0", "row8=2
             "candcap=16,group=1", "l1max=2048", "l1max=2048,row8=0", "l1max=1024", "row8=3", "row8=4",
             "row8=3,half=0", "row8=4,half=0", "row8=3,l1max=2048", "row8=3,l1max=1024", "row8=4,tb=8",
             "col_nt=1", "col_nt=1,tb=8", "col_nt=1,col_seq=0", "col_nt=1,l1max=1024", "row_nt=0",
-            "row_nt=0,row8=0", "row_nt=0,row8=1"]
+            "row_nt=0,row8=0", "row_nt=0,row8=1", "f_nt=0", "f_nt=0,row8=0"]
 CASES = ["plain-1det-32k", "net-2det-32k", "net-2det-86k"]
 
 
@@ -81,7 +81,7 @@ def test_variant_fp64(pgw, abi, oracle_runs, name, tune):
                                   "col_lf=1", "col_lf=1,rev=0", "col_lf=1,tb=8",
                                   "col_lf=1,l1max=1024", "col_lf=1,l1max=1024,rev=0", "row8=3", "row8=4",
                                   "row8=3,half=0", "row8=3,l1max=1024", "col_nt=1", "col_nt=1,tb=8",
-                                  "col_nt=1,col_lf=1,l1max=1024", "row_nt=0", "row_nt=0,row_na=2"])
+                                  "col_nt=1,col_lf=1,l1max=1024", "row_nt=0", "row_nt=0,row_na=2", "f_nt=0"])
 @pytest.mark.parametrize("name", CASES)
 def test_variant_fp32(pgw, abi, oracle_runs, name, tune):
     sky, ref, fr = oracle_runs[name]
