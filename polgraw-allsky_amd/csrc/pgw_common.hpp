@@ -47,7 +47,8 @@ struct BufRes {
   __amdgpu_buffer_rsrc_t r;
   __device__ __forceinline__ BufRes(const void* base, uint32_t bytes)
       : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000)) {}
-  // V = double, float2 or double2 (8 or 16 bytes); AUX = cache policy (0, or 2: non-temporal)
+  // V = double, float2 or double2 (8 or 16 bytes); AUX = cache policy (0; 2: non-temporal; 16:
+  // device scope (SC1), the load misses any line the L2 of this XCD may hold stale)
   template <typename V, int AUX = 0> __device__ __forceinline__ V load(uint32_t voff, uint32_t soff) const {
     if constexpr (sizeof(V) == 8) {
       return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, AUX));
@@ -66,6 +67,11 @@ struct BufRes {
   __device__ __forceinline__ void store(float2 v, uint32_t voff, uint32_t soff) const {
     using U = uint32_t __attribute__((ext_vector_type(2)));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U, v), r, (int)voff, (int)soff, 0);
+  }
+  // cache policy AUX (bits: 1 SC0, 2 NT, 16 SC1)
+  template <int AUX> __device__ __forceinline__ void store_c(float2 v, uint32_t voff, uint32_t soff) const {
+    using U = uint32_t __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U, v), r, (int)voff, (int)soff, AUX);
   }
   // non-temporal (cache policy NT): streamed out without keeping the line in L2
   __device__ __forceinline__ void store_nt(float2 v, uint32_t voff, uint32_t soff) const {

@@ -162,13 +162,15 @@ template <int P1> struct RowShape {
 // barrier before reusing lds).
 // pc (optional): the stage-C twiddle powers of this thread, computed on the first call (pc_have =
 // false) and reused on later calls of the same thread (the second spectrum of a row); pb likewise
-// the stage-B powers.
-template <typename T2, int P1, int NA, int DIR>
+// the stage-B powers.  t: the thread's index among the row's P1/16 threads (a workgroup may
+// transform several rows, one per group of P1/16 threads).
+// TAG: distinct instantiations for distinct kernels (a body shared between kernels is optimised once
+// before it is inlined, which changed the register allocation of the row pass)
+template <typename T2, int P1, int NA, int DIR, int TAG = 0>
 __device__ __forceinline__ void row_fft(T2 (&x)[NA][16], T2* lds, const T2* __restrict__ twr, T2* pc = nullptr,
-                                        bool pc_have = false, T2* pb = nullptr) {
+                                        bool pc_have = false, T2* pb = nullptr, int t = (int)threadIdx.x) {
   using S = RowShape<P1>;
   constexpr int NT = S::NT, R3 = S::R3, PADN = S::PADN;
-  const int t = threadIdx.x;
   // stage twiddle bases loaded up front so their latency overlaps stage A
   const T2 wB = twr[(t & 15) * (P1 / 256)];
   T2 wC[16 / R3];

@@ -521,7 +521,8 @@ constexpr int THR_MAXP = 256;  // FStat blocks per template at most
 
 // One wave (lane = 0..63) thresholds FStat block `part` of template t; mu: THR_MAXP doubles of LDS
 // owned by this wave.  TJ, QB: loads in flight per batch in the flagged path (bins, block means).
-template <typename T, int TJ = (sizeof(T) == 4 ? 32 : 16), int QB = 16>
+// SC: F and the partials read at device scope (written by other workgroups of the same launch).
+template <typename T, int TJ = (sizeof(T) == 4 ? 32 : 16), int QB = 16, bool SC = false>
 __device__ __forceinline__ void thr_wave(const ThrParams& p, int part, int t, int lane, double* mu,
                                          const T* __restrict__ Ft, const T* __restrict__ psum,
                                          const T* __restrict__ pmax, double* __restrict__ mu_out,
@@ -530,10 +531,14 @@ __device__ __forceinline__ void thr_wave(const ThrParams& p, int part, int t, in
   const T* F = Ft + (size_t)t * L;
   const T* ps0 = psum + (size_t)t * p.nparts * p.L2;
   const T* pm0 = pmax + (size_t)t * p.nparts * p.L2;
+  auto ld = [](const T* q) -> T {
+    if constexpr (SC) return __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *q;
+  };
   double sum = 0.0, mx = 0.0;
   for (int k2 = lane; k2 < p.L2; k2 += 64) {
-    sum += (double)ps0[(size_t)part * p.L2 + k2];
-    mx = fmax(mx, (double)pm0[(size_t)part * p.L2 + k2]);
+    sum += (double)ld(&ps0[(size_t)part * p.L2 + k2]);
+    mx = fmax(mx, (double)ld(&pm0[(size_t)part * p.L2 + k2]));
   }
   sum = wave_sum(sum);
   mx = wave_max(mx);
@@ -547,7 +552,7 @@ __device__ __forceinline__ void thr_wave(const ThrParams& p, int part, int t, in
   const int lok = p.nmin + PGW_NAV_DEV * part;
   const int hik = (part < p.nblk) ? lok + PGW_NAV_DEV : p.nmax;
   const int mL2 = p.L2 - 1;
-  auto at = [&](int k) { return (double)F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)]; };
+  auto at = [&](int k) { return (double)ld(&F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)]); };
   // (TJ loads per batch keep the kernel's register count, and so the occupancy of the far more
   // common unflagged waves, moderate)
   static_assert(NJ % TJ == 0, "whole batches");
@@ -559,7 +564,7 @@ __device__ __forceinline__ void thr_wave(const ThrParams& p, int part, int t, in
 #pragma unroll
     for (int j = 0; j < TJ; j++) {
       const int k = min(lok + lane + 64 * (j0 + j), hik - 1);
-      f[j] = F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)];
+      f[j] = ld(&F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)]);
     }
 #pragma unroll
     for (int j = 0; j < TJ; j++)
@@ -574,7 +579,7 @@ __device__ __forceinline__ void thr_wave(const ThrParams& p, int part, int t, in
     for (int k2 = lane; k2 < p.L2; k2 += 64) {
       T x[QB];
 #pragma unroll
-      for (int j = 0; j < QB; j++) x[j] = ps0[(size_t)min(q0 + j, p.nparts - 1) * p.L2 + k2];
+      for (int j = 0; j < QB; j++) x[j] = ld(&ps0[(size_t)min(q0 + j, p.nparts - 1) * p.L2 + k2]);
 #pragma unroll
       for (int j = 0; j < QB; j++) v[j] += (double)x[j];
     }

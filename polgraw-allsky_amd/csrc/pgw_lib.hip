@@ -27,6 +27,7 @@
 
 #include "../../include/pgw.h"
 #include "pgw_fstatnet.hpp"
+#include "pgw_fused.hpp"
 #include "pgw_kernels2.hpp"
 
 using namespace pgw;
@@ -273,6 +274,54 @@ void launch_spin_row(const Plan<T>& pl, const RowParams& rp, const double* aabb,
 #undef PGW_SROW_GO
 }
 
+// The template stage of a launch as one persistent kernel (pgw_fused.hpp), FP32 form.  a: every
+// field but the job layout and the shape-dependent counts.  Returns false (nothing launched) for
+// shapes it is not built for.  Grid: the workgroups that fit the device at once (at most one per
+// job).
+template <int P2, int P1, int NZ>
+void go_fused(FusedArgs& a, const SpinTab& tab, int ncu, hipStream_t st) {
+  static int per_cu = 0;
+  if (per_cu < 1) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spin_fused<P2, P1, NZ>, 256, 0) != hipSuccess || nb < 1)
+      nb = 1;
+    per_cu = nb;
+  }
+  const int grid = std::max(1, std::min(a.total, per_cu * ncu));
+  hipLaunchKernelGGL((k_spin_fused<P2, P1, NZ>), dim3(grid), dim3(256), 0, st, a, tab);
+}
+
+bool launch_fused(int L1, int L2, int Nl, FusedArgs& a, const SpinTab& tab, int lag, int ncu, hipStream_t st) {
+  const int nparts = a.tp.nparts;
+#define PGW_FUSED_GO(P2, P1)                                                                  \
+  do {                                                                                        \
+    constexpr int E_ = ColE<float, P2>::E;                                                    \
+    a.ct = P1 / ColShape<P2, E_>::C;                                                          \
+    a.rj = P2 / FusedShape<P2, P1>::RJ;                                                       \
+    a.tj = (nparts + 3) / 4;                                                                  \
+    a.D = std::min(lag, a.nt / 2);                                                            \
+    a.o1 = a.D * a.ct;                                                                        \
+    a.o2 = a.o1 + a.D * (a.ct + a.rj);                                                        \
+    a.o3 = a.o2 + (a.nt - 2 * a.D) * (a.ct + a.rj + a.tj);                                    \
+    a.o4 = a.o3 + a.D * (a.rj + a.tj);                                                        \
+    a.total = a.o4 + a.D * a.tj;                                                              \
+    const long span = (long)P1 * (P2 / E_);                                                   \
+    const long rows = (Nl + span - 1) / span;                                                 \
+    if (rows <= 3 * E_ / 8) go_fused<P2, P1, 3 * E_ / 8>(a, tab, ncu, st);                    \
+    else if (rows <= E_ / 2) go_fused<P2, P1, E_ / 2>(a, tab, ncu, st);                       \
+    else go_fused<P2, P1, E_>(a, tab, ncu, st);                                               \
+    return true;                                                                              \
+  } while (0)
+  if (L1 == 1024 && L2 == 16) PGW_FUSED_GO(16, 1024);
+  if (L1 == 1024 && L2 == 32) PGW_FUSED_GO(32, 1024);
+  if (L1 == 2048 && L2 == 32) PGW_FUSED_GO(32, 2048);
+  if (L1 == 2048 && L2 == 64) PGW_FUSED_GO(64, 2048);
+  if (L1 == 4096 && L2 == 64) PGW_FUSED_GO(64, 4096);
+  if (L1 == 4096 && L2 == 128) PGW_FUSED_GO(128, 4096);
+#undef PGW_FUSED_GO
+  return false;
+}
+
 // lin2ast, search/network/src-cpu/auxi.c:15-24 (host; same expression order)
 void lin2ast(double be1, double be2, int pm, double sepsm, double cepsm, double* sinal, double* cosal,
              double* sindel, double* cosdel) {
@@ -393,9 +442,9 @@ template <typename T> struct Engine final : pgw_handle {
   DevBuf<TmplDesc> desc;  // 2 x kDescCap
   PinBuf<TmplDesc> desc_pin;
   DevBuf<Cand> cand;      // 2 x kCandCap
-  DevBuf<int> ncand;      // 2
+  DevBuf<int> ncand;      // 2 x {candidates, error}
   PinBuf<Cand> hc_pin;    // 2 x kCandChunk
-  PinBuf<int> hn_pin;     // 2
+  PinBuf<int> hn_pin;     // 2 x {candidates, error}
   std::vector<Cand> hcand;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evd[2] = {nullptr, nullptr};
@@ -429,6 +478,19 @@ template <typename T> struct Engine final : pgw_handle {
   // on its own); thr_prev: the threshold still to run (nt = 0: none)
   bool thr_fuse = true;
   ThrFuse<T> thr_prev{};
+  // a group's launch sequence (sky stage, copies, template launches) captured into a HIP graph and
+  // launched as one; one executable graph per buffer set, updated in place while the sequence keeps
+  // its shape
+  bool graph = false;
+  bool capturing = false;
+  // the template stage of a launch as one persistent work-queue kernel (FP32 form; the separate
+  // kernels elsewhere); lag: stages between a template's column jobs and its row jobs
+  bool fused = false;
+  int fused_lag = 8;
+  int fused_spin = 1 << 20;  // polls of a dependency counter before the fused kernel gives up (~1 s)
+  DevBuf<FusedSync> fsync;
+  int ncu = 1;  // compute units of the handle's device
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
 
   // kernel timing over searches: 4 events per template launch (column, row, threshold; with the
   // threshold folded into the next column pass the last interval is empty)
@@ -483,6 +545,8 @@ template <typename T> struct Engine final : pgw_handle {
 
   ~Engine() override {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (hipGraphExec_t& x : gexec)
+      if (x) (void)hipGraphExecDestroy(x);
     for (hipEvent_t e : kpool) (void)hipEventDestroy(e);
     for (hipEvent_t e : {ev0, ev1, evd[0], evd[1]})
       if (e) (void)hipEventDestroy(e);
@@ -598,6 +662,10 @@ template <typename T> struct Engine final : pgw_handle {
       if (const char* q = strstr(tune, "col_nt=")) if (sscanf(q, "col_nt=%d", &v) == 1) col_nt = v != 0;
       if (const char* q = strstr(tune, "row_nt=")) if (sscanf(q, "row_nt=%d", &v) == 1) row_nt = v != 0;
       if (const char* q = strstr(tune, "f_nt=")) if (sscanf(q, "f_nt=%d", &v) == 1) f_nt = v != 0;
+      if (const char* q = strstr(tune, "graph=")) if (sscanf(q, "graph=%d", &v) == 1) graph = v != 0;
+      if (const char* q = strstr(tune, "fused=")) if (sscanf(q, "fused=%d", &v) == 1) fused = v != 0;
+      if (const char* q = strstr(tune, "flag=")) if (sscanf(q, "flag=%d", &v) == 1 && v >= 0 && v <= 64) fused_lag = v;
+      if (const char* q = strstr(tune, "fspin=")) if (sscanf(q, "fspin=%d", &v) == 1 && v >= 1 && v <= 24) fused_spin = 1 << v;
     }
     // segments after the variants: their FFT plans follow l1max
     for (int k = 0; k < nseg; k++)
@@ -624,9 +692,16 @@ template <typename T> struct Engine final : pgw_handle {
         (st = xm_slots.alloc((size_t)kSlots * slot_xm)) || (st = tmp1_slots.alloc((size_t)kSlots * slot_tmp1)) ||
         (st = aabb_slots.alloc((size_t)kSlots * 2)) || (st = desc.alloc((size_t)2 * kDescCap)) ||
         (st = desc_pin.alloc((size_t)2 * kDescCap)) || (st = cand.alloc((size_t)2 * kCandCap)) ||
-        (st = ncand.alloc(2)) || (st = hc_pin.alloc((size_t)2 * kCandChunk)) || (st = hn_pin.alloc(2)) ||
+        (st = ncand.alloc(4)) || (st = hc_pin.alloc((size_t)2 * kCandChunk)) || (st = hn_pin.alloc(4)) ||
+        (st = fsync.alloc(1)) ||
         (st = ensure_tb(TB)))
       return st;
+    HIPCHK(hipMemset(fsync.p, 0, sizeof(FusedSync)));  // the fused kernel's counters start at zero
+    {
+      int dev = 0;
+      HIPCHK(hipGetDevice(&dev));
+      HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
     // spline recursion coefficients (auxi.c:66-72): invp_i = 2/(c_{i-1}+4), c_i = -invp_i/2, c_0 = 0
     SplineTab tab;
     double c = 0.0;
@@ -821,6 +896,44 @@ template <typename T> struct Engine final : pgw_handle {
         for (int i = 0; i < nt; i++) spin_coeffs(q, uni->ss_at(j0 + i), tab.het1[i], tab.sgnlt1[i]);
       }
     }
+    RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2), row_nt ? 1 : 0, f_nt ? 1 : 0};
+    ThrParams tp{g.ps->L1, g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, o->white_flag, t0, ccap, ilog2(g.ps->L2), o->trl};
+    if constexpr (std::is_same<T, float>::value) {
+      // one persistent kernel for the whole template stage (its threshold included)
+      if (fused && phase_rev && !col_lf && row_half && row_na == 1 && nt <= kFusedTB &&
+          2L * g.sett.nmax <= (long)g.ps->L1 * g.ps->L2) {
+        if (int s = flush_thr()) return s;
+        FusedArgs fa{};
+        fa.sp = SpinParams{g.ps->L1, g.ps->twshift, 1, col_nt ? 1 : 0};
+        fa.sb = sb;
+        fa.rp = rp;
+        fa.tp = tp;
+        fa.desc = d;
+        fa.Z = Z.p;
+        fa.twc = g.ps->twc.p;
+        fa.twr = g.ps->twr.p;
+        fa.hi = hi.p;
+        fa.lo = lo.p;
+        fa.Ft = Ft.p;
+        fa.psum = psum.p;
+        fa.pmax = pmax.p;
+        fa.mu = mu.p;
+        fa.cand = cb;
+        fa.ncand = ncand.p + 2 * buf;
+        fa.err = ncand.p + 2 * buf + 1;
+        fa.sy = fsync.p;
+        fa.nt = nt;
+        fa.spin_limit = fused_spin;
+        if (launch_fused(g.ps->L1, g.ps->L2, uni ? segs[uni->seg]->sett.N : Nl, fa, tab, fused_lag, ncu, stream)) {
+          if (ev) {
+            HIPCHK(hipEventRecord(ev[1], stream));
+            HIPCHK(hipEventRecord(ev[2], stream));
+            HIPCHK(hipEventRecord(ev[3], stream));
+          }
+          return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
+        }
+      }
+    }
     // the pending threshold reads F and the FStat partials of the previous launch, which the row
     // pass below overwrites only after this column launch has finished
     ThrFuse<T> tf{};
@@ -832,15 +945,13 @@ template <typename T> struct Engine final : pgw_handle {
     launch_spin_col<T>(*g.ps, sb, tab, d, Z.p, nt, uni ? segs[uni->seg]->sett.N : Nl, hi.p, lo.p, col_seq, tf,
                        stream, phase_rev, col_lf, col_nt);
     if (ev) HIPCHK(hipEventRecord(ev[1], stream));
-    RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2), row_nt ? 1 : 0, f_nt ? 1 : 0};
     launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, row_half, row8, stream);
     if (ev) HIPCHK(hipEventRecord(ev[2], stream));
-    ThrParams tp{g.ps->L1, g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, o->white_flag, t0, ccap, ilog2(g.ps->L2), o->trl};
     if (fold) {
-      thr_prev = ThrFuse<T>{tp, Ft.p, psum.p, pmax.p, mu.p, cb, ncand.p + buf, nt, 0};
+      thr_prev = ThrFuse<T>{tp, Ft.p, psum.p, pmax.p, mu.p, cb, ncand.p + 2 * buf, nt, 0};
     } else {
       hipLaunchKernelGGL((k_threshold<T>), dim3(g.nparts, nt), dim3(64), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p,
-                         cb, ncand.p + buf);
+                         cb, ncand.p + 2 * buf);
     }
     if (ev) HIPCHK(hipEventRecord(ev[3], stream));
     return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
@@ -851,8 +962,9 @@ template <typename T> struct Engine final : pgw_handle {
            a.sett.fftpad == b.sett.fftpad && a.sett.nfft == b.sett.nfft;
   }
 
-  // Queues a group: sky stages into slots, descriptors, template launches, async candidate copy.
-  int launch_group(Group& gr, const pgw_opts* o) {
+  // Queues a group: sky stages into slots, descriptors, template launches, async candidate copy,
+  // and (rec_ev) the event finish_group waits on.
+  int launch_group(Group& gr, const pgw_opts* o, bool rec_ev = true) {
     int st = PGW_OK;
     if (!gr.cbuf) {
       gr.cbuf = cand.p + (size_t)gr.buf * kCandCap;
@@ -866,7 +978,7 @@ template <typename T> struct Engine final : pgw_handle {
     for (const Item& it : gr.items) fill_desc(it, o, hd);
     HIPCHK(hipMemcpyAsync(desc.p + (size_t)gr.buf * kDescCap, hd, sizeof(TmplDesc) * gr.ntempl, hipMemcpyHostToDevice,
                           stream));
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p + gr.buf, 1);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p + 2 * gr.buf, 2);
     const Seg& g0 = *segs[gr.items[0].seg];
     int Ng = 0;  // largest N of the group
     for (const Item& it : gr.items) Ng = std::max(Ng, segs[it.seg]->sett.N);
@@ -876,7 +988,7 @@ template <typename T> struct Engine final : pgw_handle {
       while (k + 1 < gr.items.size() && gr.items[k + 1].t0 <= t0) k++;
       const Item& it = gr.items[k];
       const bool one = t0 + nt <= it.t0 + it.ns();
-      const bool timed = ktime && kseq++ % ktime == 0;
+      const bool timed = !capturing && ktime && kseq++ % ktime == 0;
       if ((st = launch_templates(g0, gr.buf, gr.cbuf, gr.ccap, t0, nt, one ? &it : nullptr, t0 - it.t0, Ng, o,
                                  timed ? kquad(nt) : nullptr, thr_fuse))) {
         thr_prev.nt = 0;
@@ -884,9 +996,48 @@ template <typename T> struct Engine final : pgw_handle {
       }
     }
     if ((st = flush_thr())) return st;
-    HIPCHK(hipMemcpyAsync(hn_pin.p + gr.buf, ncand.p + gr.buf, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(hn_pin.p + 2 * gr.buf, ncand.p + 2 * gr.buf, 2 * sizeof(int), hipMemcpyDeviceToHost,
+                          stream));
     HIPCHK(hipMemcpyAsync(hc_pin.p + (size_t)gr.buf * kCandChunk, gr.cbuf, sizeof(Cand) * kCandChunk,
                           hipMemcpyDeviceToHost, stream));
+    if (rec_ev) HIPCHK(hipEventRecord(evd[gr.buf], stream));
+    return PGW_OK;
+  }
+
+  // launch_group's sequence captured from the stream into a graph, then launched as one.  The
+  // executable graph of the group's buffer set is updated with the new parameters when the
+  // sequence has the same shape, else instantiated again; its previous launch has finished (the
+  // group two before this one, same buffer set, was waited for by finish_group).
+  int launch_group_graph(Group& gr, const pgw_opts* o) {
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    capturing = true;
+    int st = launch_group(gr, o, false);
+    capturing = false;
+    hipGraph_t gph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(stream, &gph);
+    if (st || ec != hipSuccess || !gph) {
+      thr_prev.nt = 0;
+      if (gph) (void)hipGraphDestroy(gph);
+      return st ? st : PGW_ERR_DEVICE;
+    }
+    hipGraphExec_t& x = gexec[gr.buf];
+    if (x) {
+      hipGraphNode_t en = nullptr;
+      hipGraphExecUpdateResult r = hipGraphExecUpdateError;
+      if (hipGraphExecUpdate(x, gph, &en, &r) != hipSuccess || r != hipGraphExecUpdateSuccess) {
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(x);
+        x = nullptr;
+      }
+    }
+    hipError_t e = hipSuccess;
+    if (!x) e = hipGraphInstantiate(&x, gph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gph);
+    if (e != hipSuccess) {
+      x = nullptr;
+      return PGW_ERR_DEVICE;
+    }
+    HIPCHK(hipGraphLaunch(x, stream));
     HIPCHK(hipEventRecord(evd[gr.buf], stream));
     return PGW_OK;
   }
@@ -896,7 +1047,8 @@ template <typename T> struct Engine final : pgw_handle {
                    bool rec_float) {
 #pragma clang fp contract(off)
     HIPCHK(hipEventSynchronize(evd[gr0.buf]));
-    int nc = hn_pin.p[gr0.buf];  // the device count keeps counting past the buffer's capacity
+    if (hn_pin.p[2 * gr0.buf + 1]) return PGW_ERR_DEVICE;  // a fused launch's dependency wait ran out
+    int nc = hn_pin.p[2 * gr0.buf];  // the device count keeps counting past the buffer's capacity
     const Group* pg = &gr0;
     Group big;
     DevBuf<Cand> bigbuf;
@@ -912,7 +1064,8 @@ template <typename T> struct Engine final : pgw_handle {
       big.ccap = nc;
       if ((st = launch_group(big, o))) return st;
       HIPCHK(hipEventSynchronize(evd[big.buf]));
-      if (hn_pin.p[big.buf] != nc) return PGW_ERR_DEVICE;  // the same work found a different count
+      if (hn_pin.p[2 * big.buf + 1] || hn_pin.p[2 * big.buf] != nc)
+        return PGW_ERR_DEVICE;  // the same work found a different count
       if (hcand.size() < (size_t)nc) hcand.resize(nc);
       pg = &big;
     }
@@ -1007,7 +1160,7 @@ template <typename T> struct Engine final : pgw_handle {
         i++;
       }
       if (st || gr.items.empty()) continue;
-      if ((st = launch_group(gr, o))) break;
+      if ((st = graph ? launch_group_graph(gr, o) : launch_group(gr, o))) break;
       if (have_prev) st = finish_group(prev, o, sgnlc, sgnlv, cap, rec_work, rec_float);
       prev = std::move(gr);
       have_prev = true;
@@ -1238,7 +1391,7 @@ template <typename T> struct Engine final : pgw_handle {
       HIPCHK(hipEventElapsedTime(&t, ev0, ev1));
       acc[0] = t;
       for (int rr = 0; rr < reps && !st; rr++) {
-        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p, 1);
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p, 2);
         st = launch_templates(g, 0, cand.p, cand_cap, 0, nt, &it, 0, g.sett.N, o, ev);
         HIPCHK(hipStreamSynchronize(stream));
         for (int k = 0; k < 3; k++) {
@@ -1264,10 +1417,11 @@ template <typename T> struct Engine final : pgw_handle {
     if (!buf || len <= 0) return PGW_ERR_ARG;
     const int n = snprintf(buf, (size_t)len,
                            "tb=%d,row_na=%d,row8=%d,col_seq=%d,col_lf=%d,col_nt=%d,row_nt=%d,f_nt=%d,rev=%d,"
-                           "half=%d,spl_k=%d,group=%d,thr_fuse=%d,l1max=%d,candcap=%d",
+                           "half=%d,spl_k=%d,group=%d,thr_fuse=%d,l1max=%d,candcap=%d,graph=%d,fused=%d,flag=%d",
                            TB, row_na, row8, col_seq ? 1 : 0, col_lf ? 1 : 0, col_nt ? 1 : 0, row_nt ? 1 : 0,
                            f_nt ? 1 : 0, phase_rev ? 1 : 0,
-                           row_half ? 1 : 0, spl_k, group, thr_fuse ? 1 : 0, l1max, cand_cap);
+                           row_half ? 1 : 0, spl_k, group, thr_fuse ? 1 : 0, l1max, cand_cap, graph ? 1 : 0, fused ? 1 : 0,
+                           fused_lag);
     return n >= 0 && n < len ? PGW_OK : PGW_ERR_ARG;
   }
 

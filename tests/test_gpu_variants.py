@@ -4,7 +4,9 @@ exchanges inside each wave, row8=3/4), with and without the half-spectrum prunin
 both spectra together, column pass with shared LDS, spline chunk lengths, launch and group
 sizes, thresholds folded into the next column launch or launched on their own, a device
 candidate buffer smaller than a group's candidates (the group runs again into a buffer of the
-needed size), FFT rows capped at 2048 or 1024 points (longer columns); for FP32 also the
+needed size), FFT rows capped at 2048 or 1024 points (longer columns), groups launched as captured HIP
+graphs (graph=1); for FP32 also the template stage as one persistent work-queue kernel
+(fused=1, lag flag), the
 demodulation phase in radians (rev=0) and the loads-first column pass (col_lf=1, built for 16
 register rows: with l1max=1024 the columns of 128 points have them).  FP64 to 1e-9 relative in
 F; FP32 against the FP64 oracle by the rule of cases.fp32_record_diff."""
@@ -22,7 +24,7 @@ VARIANTS = ["", "row8=0", "row8=1", "row8=2", "half=0", "row8=0,half=0", "row8=2
             "candcap=16,group=1", "l1max=2048", "l1max=2048,row8=0", "l1max=1024", "row8=3", "row8=4",
             "row8=3,half=0", "row8=4,half=0", "row8=3,l1max=2048", "row8=3,l1max=1024", "row8=4,tb=8",
             "col_nt=1", "col_nt=1,tb=8", "col_nt=1,col_seq=0", "col_nt=1,l1max=1024", "row_nt=0",
-            "row_nt=0,row8=0", "row_nt=0,row8=1", "f_nt=0", "f_nt=0,row8=0"]
+            "row_nt=0,row8=0", "row_nt=0,row8=1", "f_nt=0", "f_nt=0,row8=0", "graph=1", "graph=1,group=1,tb=8"]
 CASES = ["plain-1det-32k", "net-2det-32k", "net-2det-86k"]
 
 
@@ -81,7 +83,10 @@ def test_variant_fp64(pgw, abi, oracle_runs, name, tune):
                                   "col_lf=1", "col_lf=1,rev=0", "col_lf=1,tb=8",
                                   "col_lf=1,l1max=1024", "col_lf=1,l1max=1024,rev=0", "row8=3", "row8=4",
                                   "row8=3,half=0", "row8=3,l1max=1024", "col_nt=1", "col_nt=1,tb=8",
-                                  "col_nt=1,col_lf=1,l1max=1024", "row_nt=0", "row_nt=0,row_na=2", "f_nt=0"])
+                                  "col_nt=1,col_lf=1,l1max=1024", "row_nt=0", "row_nt=0,row_na=2", "f_nt=0",
+                                  "graph=1", "graph=1,group=1,tb=8", "fused=1", "fused=1,flag=0",
+                                  "fused=1,tb=8,flag=2", "fused=1,l1max=1024", "fused=1,col_nt=1,tb=256",
+                                  "fused=1,graph=1"])
 @pytest.mark.parametrize("name", CASES)
 def test_variant_fp32(pgw, abi, oracle_runs, name, tune):
     sky, ref, fr = oracle_runs[name]
@@ -187,6 +192,46 @@ def test_eight_detectors_and_whitenoise(pgw, abi, orc, white):
     assert np.array_equal(got[:, :4], ref[:, :4])
     Fg, Fr = got[:, 4] ** 2 / 2 + 2, ref[:, 4] ** 2 / 2 + 2
     assert np.max(np.abs(Fg - Fr) / Fr) <= 1e-9
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_graph_replays(pgw, abi, oracle_runs, prec):
+    """graph=1: each group's launch sequence is captured into a HIP graph and launched as one;
+    the handle keeps one executable graph per buffer set and updates it in place on later groups.
+    Five calls in a row on one handle (two groups each, so both graphs are replayed and updated),
+    with another handle's eager launches and synchronisations between them: every call's records
+    equal the oracle's."""
+    name = "net-2det-32k"
+    sky, ref, fr = oracle_runs[name]
+    seg = segment(name)
+    p = abi.PREC_FP64 if prec == "fp64" else abi.PREC_FP32
+    old = os.environ.get("PGW_TUNE")
+    os.environ["PGW_TUNE"] = "graph=1,group=1"
+    try:
+        h = pgw.Handle(seg, p)
+    finally:
+        if old is None:
+            del os.environ["PGW_TUNE"]
+        else:
+            os.environ["PGW_TUNE"] = old
+    eager = pgw.Handle(seg, p)
+    rng, opts = abi.make_range(seg), abi.make_opts(trl=12.0)
+    for call in range(5):
+        got, fg = h.job_batch(sky, rng, opts)
+        eager.job_batch(sky[::-1], rng, opts)
+        print(prec, "call", call, "records", len(got), len(ref), flush=True)
+        assert fg == fr
+        if prec == "fp64":
+            assert got.shape == ref.shape and len(ref) > 0
+            assert np.array_equal(got[:, :4], ref[:, :4])
+            Fg, Fr = got[:, 4] ** 2 / 2 + 2, ref[:, 4] ** 2 / 2 + 2
+            assert np.max(np.abs(Fg - Fr) / Fr) <= 1e-9
+        else:
+            rel, nmatch, allowed, viol = fp32_record_diff(got, ref, 12.0)
+            assert not viol and nmatch > 0, viol[:5]
+    assert "graph=1" in h.tune_string()
+    h.close()
+    eager.close()
 
 
 def test_tune_string(pgw, abi):
