@@ -68,12 +68,16 @@ __device__ __forceinline__ double lin2_nc(double a0, double b0, double a1, doubl
 // R[j][2d+1][i] = xDat bb e^{-i phase} for i < N, zero for N <= i < nfft; _tmp1 (slot), shftf
 // [j][d][Nstride]; block partial sums of aa^2, bb^2 at paa/pbb [j][d][block].
 template <typename T>
+// zero (optional): two ints cleared by the first workgroup (the group's candidate count and
+// error word, which the template stage of the same group counts up later in the stream)
 __global__ __launch_bounds__(NT) void k_sky_prologue(const SkyItem* __restrict__ items, cplx<T>* __restrict__ R,
                                                      double* __restrict__ shftf, size_t Nstride,
-                                                     double* __restrict__ paa, double* __restrict__ pbb) {
+                                                     double* __restrict__ paa, double* __restrict__ pbb,
+                                                     int* __restrict__ zero) {
   using T2 = cplx<T>;
   const int d = blockIdx.y, j = blockIdx.z;
   const int i = blockIdx.x * NT + threadIdx.x;
+  if (zero && (blockIdx.x | blockIdx.y | blockIdx.z) == 0 && threadIdx.x < 2) zero[threadIdx.x] = 0;
   const SkyItem& it = items[j];
   const SkyParams& p = it.p;
   const SkyDet& q = p.det[d];

@@ -36,8 +36,16 @@ template <> struct ColE<double, 256> { static constexpr int E = 16; static const
 // of them is used (indexes clamped into the series, rows past N add nothing), so a wave waits on
 // its loads once per detector instead of once per row.  Built for FP32 with E = 16 only: at
 // E = 8 the extra registers cost waves per SIMD and the pass is 12-15 % slower.
+// minimum waves per SIMD of the column pass (FP32 with 8 register rows: PGW_COL32_W)
+#ifndef PGW_COL32_W
+#define PGW_COL32_W 1
+#endif
+template <typename T, int P2> struct ColOcc {
+  static constexpr int W = (std::is_same<T, float>::value && ColE<T, P2>::E == 8) ? PGW_COL32_W : 1;
+};
+
 template <typename T, int P2, bool SEQ, bool UNI, int NZ = ColE<T, P2>::E, bool LF = false>
-__global__ __launch_bounds__(256) void k_spin_col2(SpinParams p, SlotBases sb, const SpinTab tab,
+__global__ __launch_bounds__(256, (ColOcc<T, P2>::W)) void k_spin_col2(SpinParams p, SlotBases sb, const SpinTab tab,
                                                    const TmplDesc* __restrict__ desc,
                                                    cplx<T>* __restrict__ Z, const cplx<T>* __restrict__ twc,
                                                    const cplx<T>* __restrict__ hi, const cplx<T>* __restrict__ lo,

@@ -45,6 +45,9 @@ namespace {
 
 constexpr int kCandCap = 1 << 20;   // candidates per group (device buffer)
 constexpr int kCandChunk = 4096;    // candidates copied back with the count, without a second sync
+// A group's device candidate buffer: a header Cand {t = count, ii = error word} at [0] (the count
+// keeps counting past the capacity; the error word is set by a fused launch whose dependency wait
+// ran out), the candidates from [1].  One copy brings the header and the first kCandChunk back.
 constexpr int kSlots = 16;          // sky points per group
 constexpr int kDescCap = 1 << 16;   // templates per group
 constexpr int kTBMax = 256;         // templates per launch, upper bound
@@ -415,8 +418,8 @@ template <typename T> struct Engine final : pgw_handle {
   struct Group {
     std::vector<Item> items;
     int ntempl = 0, buf = 0;
-    Cand* cbuf = nullptr;  // device candidate buffer (default: cand + buf * kCandCap)
-    int ccap = 0;          // its capacity (default: cand_cap)
+    Cand* cbuf = nullptr;  // device candidate buffer, header first (default: cand + buf * (kCandCap + 1))
+    int ccap = 0;          // its capacity in candidates (default: cand_cap)
   };
 
   std::vector<std::unique_ptr<Seg>> segs;
@@ -441,10 +444,8 @@ template <typename T> struct Engine final : pgw_handle {
   DevBuf<double> mu;
   DevBuf<TmplDesc> desc;  // 2 x kDescCap
   PinBuf<TmplDesc> desc_pin;
-  DevBuf<Cand> cand;      // 2 x kCandCap
-  DevBuf<int> ncand;      // 2 x {candidates, error}
-  PinBuf<Cand> hc_pin;    // 2 x kCandChunk
-  PinBuf<int> hn_pin;     // 2 x {candidates, error}
+  DevBuf<Cand> cand;      // 2 x (1 + kCandCap): header, candidates
+  PinBuf<Cand> hc_pin;    // 2 x (1 + kCandChunk)
   std::vector<Cand> hcand;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evd[2] = {nullptr, nullptr};
@@ -691,8 +692,8 @@ template <typename T> struct Engine final : pgw_handle {
         (st = sky_pin.alloc(2 * K)) ||
         (st = xm_slots.alloc((size_t)kSlots * slot_xm)) || (st = tmp1_slots.alloc((size_t)kSlots * slot_tmp1)) ||
         (st = aabb_slots.alloc((size_t)kSlots * 2)) || (st = desc.alloc((size_t)2 * kDescCap)) ||
-        (st = desc_pin.alloc((size_t)2 * kDescCap)) || (st = cand.alloc((size_t)2 * kCandCap)) ||
-        (st = ncand.alloc(4)) || (st = hc_pin.alloc((size_t)2 * kCandChunk)) || (st = hn_pin.alloc(4)) ||
+        (st = desc_pin.alloc((size_t)2 * kDescCap)) || (st = cand.alloc((size_t)2 * (kCandCap + 1))) ||
+        (st = hc_pin.alloc((size_t)2 * (kCandChunk + 1))) ||
         (st = fsync.alloc(1)) ||
         (st = ensure_tb(TB)))
       return st;
@@ -782,7 +783,8 @@ template <typename T> struct Engine final : pgw_handle {
   // Sky-point stage (jobcore.c:270-393) of G sky points at once: items[j] (host, prepared by
   // sky_item) are staged through pinned buffer `buf`; every kernel covers all G sky points.
   // Segments of one batch share nfft and Ninterp (compatible()).
-  int sky_batch(const Seg& g0, const SkyItem* items, int G, int buf) {
+  // zero (optional): the group's candidate header, cleared by the first kernel
+  int sky_batch(const Seg& g0, const SkyItem* items, int G, int buf, int* zero = nullptr) {
     const pgw_settings& S = g0.sett;
     SkyItem* hp = sky_pin.p + (size_t)buf * kSlots;
     memcpy(hp, items, sizeof(SkyItem) * G);
@@ -795,7 +797,7 @@ template <typename T> struct Engine final : pgw_handle {
     }
     const int nb = 2 * nifo;
     hipLaunchKernelGGL((k_sky_prologue<T>), dim3(g0.npro, nifo, G), dim3(NT), 0, stream, dp, R.p, shftf.p, Nmax,
-                       paa.p, pbb.p);
+                       paa.p, pbb.p, zero);
     // resampling: forward FFT (nfft), zero-insert, backward FFT (Ninterp), scale (jobcore.c:319-362)
     LoadPlain<T2> lp{R.p, S.nfft, S.nfft};
     launch_col<T, -1>(*g0.pf, lp, Zf.p, nb * G, hi.p, lo.p, stream);
@@ -918,9 +920,9 @@ template <typename T> struct Engine final : pgw_handle {
         fa.psum = psum.p;
         fa.pmax = pmax.p;
         fa.mu = mu.p;
-        fa.cand = cb;
-        fa.ncand = ncand.p + 2 * buf;
-        fa.err = ncand.p + 2 * buf + 1;
+        fa.cand = cb + 1;
+        fa.ncand = &cb->t;
+        fa.err = &cb->ii;
         fa.sy = fsync.p;
         fa.nt = nt;
         fa.spin_limit = fused_spin;
@@ -948,10 +950,10 @@ template <typename T> struct Engine final : pgw_handle {
     launch_spin_row<T>(*g.ps, rp, aabb_slots.p, d, Z.p, Ft.p, psum.p, pmax.p, nt, row_na, row_half, row8, stream);
     if (ev) HIPCHK(hipEventRecord(ev[2], stream));
     if (fold) {
-      thr_prev = ThrFuse<T>{tp, Ft.p, psum.p, pmax.p, mu.p, cb, ncand.p + 2 * buf, nt, 0};
+      thr_prev = ThrFuse<T>{tp, Ft.p, psum.p, pmax.p, mu.p, cb + 1, &cb->t, nt, 0};
     } else {
       hipLaunchKernelGGL((k_threshold<T>), dim3(g.nparts, nt), dim3(64), 0, stream, tp, Ft.p, psum.p, pmax.p, mu.p,
-                         cb, ncand.p + 2 * buf);
+                         cb + 1, &cb->t);
     }
     if (ev) HIPCHK(hipEventRecord(ev[3], stream));
     return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
@@ -967,18 +969,17 @@ template <typename T> struct Engine final : pgw_handle {
   int launch_group(Group& gr, const pgw_opts* o, bool rec_ev = true) {
     int st = PGW_OK;
     if (!gr.cbuf) {
-      gr.cbuf = cand.p + (size_t)gr.buf * kCandCap;
+      gr.cbuf = cand.p + (size_t)gr.buf * (kCandCap + 1);
       gr.ccap = cand_cap;
     }
     SkyItem si[kSlots];
     for (Item& it : gr.items)
       if (!sky_item(*segs[it.seg], it.pm, it.mm, it.nn, it.slot, it.sk, si[it.slot])) return PGW_ERR_DEVICE;
-    if ((st = sky_batch(*segs[gr.items[0].seg], si, (int)gr.items.size(), gr.buf))) return st;
+    if ((st = sky_batch(*segs[gr.items[0].seg], si, (int)gr.items.size(), gr.buf, &gr.cbuf->t))) return st;
     TmplDesc* hd = desc_pin.p + (size_t)gr.buf * kDescCap;
     for (const Item& it : gr.items) fill_desc(it, o, hd);
     HIPCHK(hipMemcpyAsync(desc.p + (size_t)gr.buf * kDescCap, hd, sizeof(TmplDesc) * gr.ntempl, hipMemcpyHostToDevice,
                           stream));
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p + 2 * gr.buf, 2);
     const Seg& g0 = *segs[gr.items[0].seg];
     int Ng = 0;  // largest N of the group
     for (const Item& it : gr.items) Ng = std::max(Ng, segs[it.seg]->sett.N);
@@ -996,9 +997,7 @@ template <typename T> struct Engine final : pgw_handle {
       }
     }
     if ((st = flush_thr())) return st;
-    HIPCHK(hipMemcpyAsync(hn_pin.p + 2 * gr.buf, ncand.p + 2 * gr.buf, 2 * sizeof(int), hipMemcpyDeviceToHost,
-                          stream));
-    HIPCHK(hipMemcpyAsync(hc_pin.p + (size_t)gr.buf * kCandChunk, gr.cbuf, sizeof(Cand) * kCandChunk,
+    HIPCHK(hipMemcpyAsync(hc_pin.p + (size_t)gr.buf * (kCandChunk + 1), gr.cbuf, sizeof(Cand) * (kCandChunk + 1),
                           hipMemcpyDeviceToHost, stream));
     if (rec_ev) HIPCHK(hipEventRecord(evd[gr.buf], stream));
     return PGW_OK;
@@ -1047,8 +1046,9 @@ template <typename T> struct Engine final : pgw_handle {
                    bool rec_float) {
 #pragma clang fp contract(off)
     HIPCHK(hipEventSynchronize(evd[gr0.buf]));
-    if (hn_pin.p[2 * gr0.buf + 1]) return PGW_ERR_DEVICE;  // a fused launch's dependency wait ran out
-    int nc = hn_pin.p[2 * gr0.buf];  // the device count keeps counting past the buffer's capacity
+    const Cand* hh = hc_pin.p + (size_t)gr0.buf * (kCandChunk + 1);  // header, then candidates
+    if (hh->ii) return PGW_ERR_DEVICE;  // a fused launch's dependency wait ran out
+    const int nc = hh->t;  // the device count keeps counting past the buffer's capacity
     const Group* pg = &gr0;
     Group big;
     DevBuf<Cand> bigbuf;
@@ -1058,21 +1058,20 @@ template <typename T> struct Engine final : pgw_handle {
       // rerun rebuilds this group's sky slots, which that group no longer needs.
       HIPCHK(hipStreamSynchronize(stream));
       int st;
-      if ((st = bigbuf.alloc((size_t)std::max(nc, kCandChunk)))) return st;
+      if ((st = bigbuf.alloc((size_t)std::max(nc, kCandChunk) + 1))) return st;
       big = gr0;
       big.cbuf = bigbuf.p;
       big.ccap = nc;
       if ((st = launch_group(big, o))) return st;
       HIPCHK(hipEventSynchronize(evd[big.buf]));
-      if (hn_pin.p[2 * big.buf + 1] || hn_pin.p[2 * big.buf] != nc)
-        return PGW_ERR_DEVICE;  // the same work found a different count
+      if (hh->ii || hh->t != nc) return PGW_ERR_DEVICE;  // the same work found a different count
       if (hcand.size() < (size_t)nc) hcand.resize(nc);
       pg = &big;
     }
     const Group& gr = *pg;
-    memcpy(hcand.data(), hc_pin.p + (size_t)gr.buf * kCandChunk, sizeof(Cand) * std::min(nc, kCandChunk));
+    memcpy(hcand.data(), hh + 1, sizeof(Cand) * std::min(nc, kCandChunk));
     if (nc > kCandChunk) {  // rare: the rest synchronously (the buffer is not reused before this returns)
-      HIPCHK(hipMemcpyAsync(hcand.data() + kCandChunk, gr.cbuf + kCandChunk, sizeof(Cand) * (nc - kCandChunk),
+      HIPCHK(hipMemcpyAsync(hcand.data() + kCandChunk, gr.cbuf + 1 + kCandChunk, sizeof(Cand) * (nc - kCandChunk),
                             hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
     }
@@ -1391,7 +1390,7 @@ template <typename T> struct Engine final : pgw_handle {
       HIPCHK(hipEventElapsedTime(&t, ev0, ev1));
       acc[0] = t;
       for (int rr = 0; rr < reps && !st; rr++) {
-        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, ncand.p, 2);
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, &cand.p->t, 2);
         st = launch_templates(g, 0, cand.p, cand_cap, 0, nt, &it, 0, g.sett.N, o, ev);
         HIPCHK(hipStreamSynchronize(stream));
         for (int k = 0; k < 3; k++) {
