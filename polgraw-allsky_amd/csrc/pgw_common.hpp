@@ -86,6 +86,11 @@ template <typename T2> __device__ __forceinline__ T2 cmul(T2 a, T2 b) {
   return mk<T2>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 template <typename T2> __device__ __forceinline__ T2 conjg(T2 a) { return mk<T2>(a.x, -a.y); }
+// acc + a e with two fused multiply-adds per component (cadd(acc, cmul(a, e)) takes a multiply, a
+// fused multiply-add and an add per component, and rounds once more)
+template <typename T2> __device__ __forceinline__ T2 cmac(T2 acc, T2 a, T2 e) {
+  return mk<T2>(fma(a.x, e.x, fma(-a.y, e.y, acc.x)), fma(a.x, e.y, fma(a.y, e.x, acc.y)));
+}
 // twiddle from a forward-sign table entry: DIR = -1 forward (as stored), +1 backward (conjugate)
 template <int DIR, typename T2> __device__ __forceinline__ T2 tw_dir(T2 w) { return DIR < 0 ? w : conjg(w); }
 // multiply by W_4 = exp(DIR * i pi/2): -i (forward) or +i (backward)

@@ -160,8 +160,8 @@ __device__ __forceinline__ void fused_col(const FusedArgs& a, const FusedTw& tw,
         const double tmp1 = t1b.load<double>(vt, dN * (uint32_t)sizeof(double));
         const float2 e = pr.eval(dn, tmp1);
         const T2 ex = mk<T2>(e.x, e.y);
-        xa = cadd(xa, cmul(xmb.load<T2>(vx, 2 * dN * (uint32_t)sizeof(T2)), ex));
-        xb = cadd(xb, cmul(xmb.load<T2>(vx, (2 * dN + (uint32_t)N) * (uint32_t)sizeof(T2)), ex));
+        xa = cmac(xa, xmb.load<T2>(vx, 2 * dN * (uint32_t)sizeof(T2)), ex);
+        xb = cmac(xb, xmb.load<T2>(vx, (2 * dN + (uint32_t)N) * (uint32_t)sizeof(T2)), ex);
       }
     }
     x[0][r] = xa;

@@ -116,8 +116,8 @@ __global__ __launch_bounds__(256, (ColOcc<T, P2>::W)) void k_spin_col2(SpinParam
             ex = mk<T2>(e.x, e.y);
           }
           else ex = PhaseRot<T>::eval(lin2_nc(het1, dn, sgnlt1, tmp1));
-          xa = cadd(xa, cmul(xmb.load<T2>(vx, 2 * dN * (uint32_t)sizeof(T2)), ex));
-          xb = cadd(xb, cmul(xmb.load<T2>(vx, (2 * dN + (uint32_t)N) * (uint32_t)sizeof(T2)), ex));
+          xa = cmac(xa, xmb.load<T2>(vx, 2 * dN * (uint32_t)sizeof(T2)), ex);
+          xb = cmac(xb, xmb.load<T2>(vx, (2 * dN + (uint32_t)N) * (uint32_t)sizeof(T2)), ex);
         }
       }
       x[0][r] = xa;
@@ -154,8 +154,8 @@ __global__ __launch_bounds__(256, (ColOcc<T, P2>::W)) void k_spin_col2(SpinParam
           } else {
             ex = PhaseRot<T>::eval(lin2_nc(het1, dn, sgnlt1, tv[r]));
           }
-          x[0][r] = cadd(x[0][r], cmul(av[r], ex));
-          x[1][r] = cadd(x[1][r], cmul(bv[r], ex));
+          x[0][r] = cmac(x[0][r], av[r], ex);
+          x[1][r] = cmac(x[1][r], bv[r], ex);
         }
       }
     }
