@@ -261,6 +261,12 @@ int pgw_kernel_times(pgw_handle *h, int enable, double out[5]);
  * hardware byte counters. */
 int pgw_copy_bench(size_t bytes, int reps, double *ms);
 
+/* Measurement helper (no counterpart in the reference): the HBM bandwidth this device reaches with
+ * simple streaming kernels over a buffer of `bytes` bytes.  kind 0: copy (bytes read and bytes
+ * written), 1: read only, 2: write only (non-temporal stores); the fastest of three grid sizes,
+ * `reps` launches each; *ms = milliseconds per launch. */
+int pgw_hbm_probe(int kind, size_t bytes, int reps, double *ms);
+
 /* Elapsed device time (ms) of the last pgw_job_core / pgw_job_batch call, from HIP events. */
 double pgw_last_gpu_ms(const pgw_handle *h);
 

@@ -669,4 +669,21 @@ __global__ __launch_bounds__(NT) void k_copy(const double4* __restrict__ a, doub
   for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) b[i] = a[i];
 }
 
+// HBM probes (grid-stride, 16 bytes per lane and step): read only (a sum that is stored only if it
+// takes a value the data cannot give, so the loads stay) and write only (non-temporal stores)
+__global__ __launch_bounds__(NT) void k_read_probe(const float4* __restrict__ a, size_t n, float* __restrict__ out) {
+  float s = 0.f;
+  for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) {
+    const float4 x = a[i];
+    s += (x.x + x.y) + (x.z + x.w);
+  }
+  if (s == -1.f) out[0] = s;  // the buffer holds zeros
+}
+__global__ __launch_bounds__(NT) void k_write_probe(float4* __restrict__ b, size_t n) {
+  using V = float __attribute__((ext_vector_type(4)));
+  V z = {0.f, 0.f, 0.f, 0.f};
+  for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT)
+    __builtin_nontemporal_store(z, reinterpret_cast<V*>(b + i));
+}
+
 }  // namespace pgw

@@ -1734,6 +1734,50 @@ int pgw_copy_bench(size_t bytes, int reps, double* ms) {
   return PGW_OK;
 }
 
+int pgw_hbm_probe(int kind, size_t bytes, int reps, double* ms) {
+  if (!ms || kind < 0 || kind > 2 || bytes < 4096 || reps < 1) return PGW_ERR_ARG;
+  const size_t n = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  float* o = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = PGW_OK;
+  auto fail = [&](int s) {
+    if (!st) st = s;
+  };
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess || hipMalloc(&o, 64) != hipSuccess)
+    fail(PGW_ERR_NOMEM);
+  if (!st && (hipMemset(a, 0, n * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+              hipEventCreate(&e1) != hipSuccess))
+    fail(PGW_ERR_DEVICE);
+  double best = 1e30;
+  // the fastest of three grid sizes (a few workgroups per CU up to many): copies and reads peak
+  // at small grids, non-temporal writes at large ones
+  for (int grid : {2048, 8192, 32768}) {
+    if (st) break;
+    auto go = [&]() {
+      if (kind == 0) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(NT), 0, 0, (const double4*)a, (double4*)b, n / 2);
+      else if (kind == 1) hipLaunchKernelGGL(k_read_probe, dim3(grid), dim3(NT), 0, 0, (const float4*)a, n, o);
+      else hipLaunchKernelGGL(k_write_probe, dim3(grid), dim3(NT), 0, 0, (float4*)b, n);
+    };
+    go();  // warm-up
+    if (hipEventRecord(e0, 0) != hipSuccess) { fail(PGW_ERR_DEVICE); break; }
+    for (int r = 0; r < reps; r++) go();
+    float t = 0.f;
+    if (hipEventRecord(e1, 0) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+      fail(PGW_ERR_DEVICE);
+    else
+      best = std::min(best, (double)t / reps);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  (void)hipFree(o);
+  if (!st) *ms = best;
+  return st;
+}
+
 double pgw_last_gpu_ms(const pgw_handle* h) { return h ? h->last_ms : 0.0; }
 
 int pgw_kernel_times(pgw_handle* h, int enable, double out[5]) {

@@ -67,6 +67,7 @@ def lib():
         L.pgw_time_stages.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.POINTER(abi.Opts), C.c_int, _dp]
         L.pgw_copy_bench.argtypes = [C.c_size_t, C.c_int, _dp]
+        L.pgw_hbm_probe.argtypes = [C.c_int, C.c_size_t, C.c_int, _dp]
         L.pgw_last_gpu_ms.restype = C.c_double
         L.pgw_last_gpu_ms.argtypes = [C.c_void_p]
         L.pgw_kernel_times.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -82,7 +83,8 @@ EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_j
             "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_tune_string", "pgw_fstatnet", "pgw_fstatnet_multi", "pgw_fstatnet_noalloc",
             "pgw_get_settings",
             "pgw_search", "pgw_read_checkpoint", "pgw_job_spotlight",
-            "pgw_debug_fstat", "pgw_debug_resampled", "pgw_time_stages", "pgw_copy_bench", "pgw_last_gpu_ms",
+            "pgw_debug_fstat", "pgw_debug_resampled", "pgw_time_stages", "pgw_copy_bench", "pgw_hbm_probe",
+            "pgw_last_gpu_ms",
             "pgw_kernel_times", "pgw_destroy", "pgw_strerror"]
 
 
@@ -117,6 +119,20 @@ def read_checkpoint(qname, rng):
     if st < 0:
         raise PGWError(st, "pgw_read_checkpoint")
     return st, fn.value
+
+
+def hbm_probe(nbytes: int = 1 << 30, reps: int = 5) -> dict:
+    """GB/s this device reaches with streaming kernels over nbytes: copy (read + write bytes), read
+    only and write only (pgw_hbm_probe)."""
+    out = {}
+    for kind, name in ((0, "copy"), (1, "read"), (2, "write")):
+        ms = C.c_double()
+        st = lib().pgw_hbm_probe(kind, nbytes, reps, C.byref(ms))
+        if st != 0:
+            raise PGWError(st, "pgw_hbm_probe")
+        moved = 2 * nbytes if kind == 0 else nbytes
+        out[name] = round(moved / ms.value / 1e6, 1)
+    return out
 
 
 def copy_bench(nbytes: int, reps: int = 20) -> float:
