@@ -63,9 +63,9 @@ template <int R, int DIR, typename T2> __device__ __forceinline__ void apply_tw_
 }
 
 // ----------------------------------------------------------- column pass --
-template <int P2, int E> struct ColShape {
+template <int P2, int E, int NTH = 256> struct ColShape {
   static constexpr int G = P2 / E;     // groups per column
-  static constexpr int C = 256 / G;    // columns per block
+  static constexpr int C = NTH / G;    // columns per block of NTH threads
   static constexpr int Q = E / G;      // kr values per thread in step 2
   static_assert(G >= 1 && G <= E && E % G == 0, "column split");
 };
@@ -75,10 +75,10 @@ template <int P2, int E> struct ColShape {
 // caller must barrier before reusing lds.
 // NZ < E: the inputs x[a][r], r >= NZ, are zero (pruned first DFT).  pw (optional): the step-1
 // twiddle powers of this thread, computed on the first call (pw_have = false), reused after.
-template <typename T2, int P2, int E, int NA, int DIR, int NZ = E>
+template <typename T2, int P2, int E, int NA, int DIR, int NZ = E, int NTH = 256>
 __device__ __forceinline__ void col_fft(T2 (&x)[NA][E], T2* lds, const T2* __restrict__ twc, T2* pw = nullptr,
                                         bool pw_have = false) {
-  using S = ColShape<P2, E>;
+  using S = ColShape<P2, E, NTH>;
   const int c = threadIdx.x % S::C, g = threadIdx.x / S::C;
   const T2 wg = twc[g];
 #pragma unroll
@@ -111,9 +111,9 @@ __device__ __forceinline__ void col_fft(T2 (&x)[NA][E], T2* lds, const T2* __res
 
 // Inter-pass twiddles W_L^{n1 k2} for the thread's E output slots from one table entry u = W_L^{n1}:
 // k2 = kr + E kg with kr = grp*Q + q, so W = u^{grp Q} * u^q * (u^E)^{kg}  (forward sign).
-template <int P2, int E, typename T2>
+template <int P2, int E, int NTH = 256, typename T2>
 __device__ __forceinline__ void col_out_twiddles(T2 u, T2* w) {
-  using S = ColShape<P2, E>;
+  using S = ColShape<P2, E, NTH>;
   const int g = threadIdx.x / S::C;
   T2 pu[E + 1];  // u^0..u^E
   tw_powers<E>(u, pu);

@@ -44,20 +44,22 @@ template <typename T, int P2> struct ColOcc {
   static constexpr int W = (std::is_same<T, float>::value && ColE<T, P2>::E == 8) ? PGW_COL32_W : 1;
 };
 
-template <typename T, int P2, bool SEQ, bool UNI, int NZ = ColE<T, P2>::E, bool LF = false>
-__global__ __launch_bounds__(256, (ColOcc<T, P2>::W)) void k_spin_col2(SpinParams p, SlotBases sb, const SpinTab tab,
+// NTH: threads per workgroup (256, or 512 for tiles of twice the columns: each store instruction
+// then writes row segments twice as long)
+template <typename T, int P2, bool SEQ, bool UNI, int NZ = ColE<T, P2>::E, bool LF = false, int NTH = 256>
+__global__ __launch_bounds__(NTH, (ColOcc<T, P2>::W)) void k_spin_col2(SpinParams p, SlotBases sb, const SpinTab tab,
                                                    const TmplDesc* __restrict__ desc,
                                                    cplx<T>* __restrict__ Z, const cplx<T>* __restrict__ twc,
                                                    const cplx<T>* __restrict__ hi, const cplx<T>* __restrict__ lo,
                                                    const ThrFuse<T> tf) {
   using T2 = cplx<T>;
   constexpr int E = ColE<T, P2>::E;
-  using S = ColShape<P2, E>;
+  using S = ColShape<P2, E, NTH>;
   __shared__ T2 lds[(SEQ ? 1 : 2) * P2 * S::C];
-  static_assert(sizeof(lds) >= 4 * THR_MAXP * sizeof(double), "threshold waves' mean tables fit the LDS");
+  static_assert(sizeof(lds) >= NTH / 64 * THR_MAXP * sizeof(double), "threshold waves' mean tables fit the LDS");
   if ((int)blockIdx.y < tf.rows) {  // whole workgroups: no barrier is split
     const int w = threadIdx.x >> 6;
-    const int item = ((int)blockIdx.y * (int)gridDim.x + (int)blockIdx.x) * 4 + w;
+    const int item = ((int)blockIdx.y * (int)gridDim.x + (int)blockIdx.x) * (NTH / 64) + w;
     // few loads in flight in the (rare) flagged path, so the column pass keeps its registers
     if (item < tf.p.nparts * tf.nt)
       thr_wave<T, 4, 2>(tf.p, item % tf.p.nparts, item / tf.p.nparts, threadIdx.x & 63,
@@ -194,9 +196,9 @@ __global__ __launch_bounds__(256, (ColOcc<T, P2>::W)) void k_spin_col2(SpinParam
     }
   };
   T2 w[E];
-  col_out_twiddles<P2, E>(tw_big(hi, lo, (uint32_t)n1 << p.twshift), w);
+  col_out_twiddles<P2, E, NTH>(tw_big(hi, lo, (uint32_t)n1 << p.twshift), w);
   if constexpr (!SEQ) {
-    col_fft<T2, P2, E, 2, -1, NZ>(x, lds, twc);
+    col_fft<T2, P2, E, 2, -1, NZ, NTH>(x, lds, twc);
 #pragma unroll
     for (int s = 0; s < E; s++) {
       const uint32_t k2c = col_out_k2c<P2, E>(s);
@@ -213,7 +215,7 @@ __global__ __launch_bounds__(256, (ColOcc<T, P2>::W)) void k_spin_col2(SpinParam
     for (int a = 0; a < 2; a++) {
       T2 (&xa)[1][E] = *reinterpret_cast<T2(*)[1][E]>(&x[a]);
       if (a) __syncthreads();
-      col_fft<T2, P2, E, 1, -1, NZ>(xa, lds, twc, PW ? pw : nullptr, a > 0);
+      col_fft<T2, P2, E, 1, -1, NZ, NTH>(xa, lds, twc, PW ? pw : nullptr, a > 0);
 #pragma unroll
       for (int s = 0; s < E; s++) {
         put(cmul(xa[0][s], w[s]), (uint32_t)a * specb + (uint32_t)col_out_k2c<P2, E>(s) * rowb);

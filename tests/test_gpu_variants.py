@@ -5,7 +5,7 @@ both spectra together, column pass with shared LDS, spline chunk lengths, launch
 sizes, thresholds folded into the next column launch or launched on their own, a device
 candidate buffer smaller than a group's candidates (the group runs again into a buffer of the
 needed size), FFT rows capped at 2048 or 1024 points (longer columns), groups launched as captured HIP
-graphs (graph=1); for FP32 also the template stage as one persistent work-queue kernel
+graphs (graph=1); for FP32 also column passes of 256 or 512 threads (col_wide), the template stage as one persistent work-queue kernel
 (fused=1, lag flag), the
 demodulation phase in radians (rev=0) and the loads-first column pass (col_lf=1, built for 16
 register rows: with l1max=1024 the columns of 128 points have them).  FP64 to 1e-9 relative in
@@ -86,7 +86,8 @@ def test_variant_fp64(pgw, abi, oracle_runs, name, tune):
                                   "col_nt=1,col_lf=1,l1max=1024", "row_nt=0", "row_nt=0,row_na=2", "f_nt=0",
                                   "graph=1", "graph=1,group=1,tb=8", "fused=1", "fused=1,flag=0",
                                   "fused=1,tb=8,flag=2", "fused=1,l1max=1024", "fused=1,col_nt=1,tb=256",
-                                  "fused=1,graph=1"])
+                                  "fused=1,graph=1", "col_wide=0", "col_wide=0,col_nt=1,tb=8", "col_wide=1,col_seq=0",
+                                  "col_wide=1,thr_fuse=0,tb=8"])
 @pytest.mark.parametrize("name", CASES)
 def test_variant_fp32(pgw, abi, oracle_runs, name, tune):
     sky, ref, fr = oracle_runs[name]
