@@ -423,9 +423,9 @@ struct SpinParams {
 template <typename T> struct PhaseRot;
 // sin and cos of x in FP64 without a large-argument path: k = rint(2x / pi), x - k pi/2 by three
 // FMAs against pi/2 split in three doubles (the first two products exact inside the FMA, so the
-// reduced angle is off by ~1e-16 for |x| < 2^50), then Taylor series to degree 17 (sin) and 18
-// (cos) on |r| <= pi/4 (truncation < 1e-19), and the quadrant k mod 4.  Absolute error ~1.2e-16
-// (against 200-bit arithmetic, |x| up to 1e12); demodulation phases here are below 1e7.
+// reduced angle is off by ~1e-16 for |x| < 2^50), then minimax polynomials of degree 13 (sin)
+// and 12 (cos) on |r| <= pi/4, and the quadrant k mod 4.  Absolute error ~1.1e-16 (against
+// 200-bit arithmetic, |x| up to 1e12); demodulation phases here are below 1e7.
 __device__ __forceinline__ void sincos_cw(double x, double& s, double& c) {
   constexpr double P1 = 1.5707963267948966, P2 = 6.123233995736766e-17, P3 = -1.4973849048591698e-33;
   const double k = rint(x * 0.6366197723675814);
@@ -433,25 +433,22 @@ __device__ __forceinline__ void sincos_cw(double x, double& s, double& c) {
   r = fma(-k, P2, r);
   r = fma(-k, P3, r);
   const double r2 = r * r;
-  // sin r = r + r^3 sum_{j>=1} (-1)^j r^(2j-2) / (2j+1)!,  cos r = 1 + r^2 sum_{j>=1} (-1)^j r^(2j-2) / (2j)!
-  double ps = 2.8114572543455206e-15;                 // 1/17!
-  ps = fma(ps, r2, -7.647163731819816e-13);           // -1/15!
-  ps = fma(ps, r2, 1.6059043836821613e-10);           // 1/13!
-  ps = fma(ps, r2, -2.505210838544172e-08);           // -1/11!
-  ps = fma(ps, r2, 2.7557319223985893e-06);           // 1/9!
-  ps = fma(ps, r2, -0.0001984126984126984);           // -1/7!
-  ps = fma(ps, r2, 0.008333333333333333);             // 1/5!
-  ps = fma(ps, r2, -0.16666666666666666);             // -1/3!
+  // sin r = r + r^3 S(r^2), cos r = 1 - r^2 C(r^2): S and C of degree 5, the minimax polynomials
+  // of sin and cos on |r| <= pi/4 (Remez exchange in 200-bit arithmetic, sincos_minimax.py;
+  // approximation error 1.6e-18 and 5.2e-17, below the FP64 rounding of the result)
+  double ps = 1.588858596376266e-10;
+  ps = fma(ps, r2, -2.5050600239083346e-08);
+  ps = fma(ps, r2, 2.7557312544777556e-06);
+  ps = fma(ps, r2, -0.0001984126982593413);
+  ps = fma(ps, r2, 0.008333333333316482);
+  ps = fma(ps, r2, -0.166666666666666);
   const double sn = fma(r * r2, ps, r);
-  double pc = 1.5619206968586225e-16;                 // 1/18!
-  pc = fma(pc, r2, -4.779477332387385e-14);           // -1/16!
-  pc = fma(pc, r2, 1.1470745597729725e-11);           // 1/14!
-  pc = fma(pc, r2, -2.08767569878681e-09);            // -1/12!
-  pc = fma(pc, r2, 2.755731922398589e-07);            // 1/10!
-  pc = fma(pc, r2, -2.48015873015873e-05);            // -1/8!
-  pc = fma(pc, r2, 0.001388888888888889);             // 1/6!
-  pc = fma(pc, r2, -0.041666666666666664);            // -1/4!
-  pc = fma(pc, r2, 0.5);                              // 1/2! (sign below)
+  double pc = -2.0633590346945927e-09;
+  pc = fma(pc, r2, 2.755529638055987e-07);
+  pc = fma(pc, r2, -2.480157901658819e-05);
+  pc = fma(pc, r2, 0.001388888887172768);
+  pc = fma(pc, r2, -0.04166666666650322);
+  pc = fma(pc, r2, 0.49999999999999484);
   const double cs = fma(-r2, pc, 1.0);
   // quadrant k mod 4 (k is an integer in FP64; exact for |k| < 2^53)
   const int q = (int)fma(-4.0, floor(k * 0.25), k);

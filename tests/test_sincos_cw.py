@@ -1,9 +1,10 @@
-"""The FP64 demodulation sin/cos of the column pass (sincos_cw in pgw_kernels.hpp): its constants
-against the values the mathematics fixes (pi/2 split in three doubles, 2/pi, +-1/n!), and its
-reduction and series, evaluated with exact FMAs on the same constants, against 200-bit sin and
-cos.  The device code itself is exercised by the FP64 parity tests; this pins what a plausible
-slip (a digit of a coefficient, a wrong sign, a dropped reduction term) would change by far less
-than their 1e-9 tolerance on F."""
+"""The FP64 demodulation sin/cos of the column pass (sincos_cw in pgw_kernels.hpp): its reduction
+constants against the values the mathematics fixes (pi/2 split in three doubles, 2/pi); its
+polynomials, evaluated in 200-bit arithmetic on their double coefficients, against sin and cos on
+the reduced range (the minimax error bound, which a slip in a leading digit of any coefficient
+or a dropped term exceeds); and the whole reduction and evaluation, with exact FMAs on the same
+constants, against 200-bit sin and cos.  The device code itself is exercised by the FP64 parity
+tests; this pins what a plausible slip would change by far less than their 1e-9 tolerance on F."""
 import math
 import random
 import re
@@ -43,9 +44,33 @@ def test_constants_are_the_mathematical_ones():
     assert p3 == float(half_pi - p1 - p2)
     assert abs(half_pi - p1 - p2 - p3) < mpmath.mpf(2) ** -160
     assert two_pi == float(2 / mpmath.pi)
-    # sin: highest term first, 1/17! ... -1/3!;  cos (sign applied by 1 - r^2 pc): 1/18! ... 1/2!
-    assert ps == [(-1) ** ((n - 1) // 2) / math.factorial(n) for n in (17, 15, 13, 11, 9, 7, 5, 3)]
-    assert pc == [(-1) ** (n // 2 + 1) / math.factorial(n) for n in (18, 16, 14, 12, 10, 8, 6, 4, 2)]
+
+
+def test_polynomials_on_the_reduced_range():
+    """sin r = r + r^3 S(r^2), cos r = 1 - r^2 C(r^2) on |r| <= pi/4, in exact arithmetic on the
+    kernel's double coefficients: within the minimax bounds (1.6e-18, 5.2e-17) plus the
+    coefficients' rounding.  A coefficient off in its 12th significant digit, or a term dropped,
+    moves the error far past them; S and C start at -1/3! and 1/2! (to 1e-13), as the series
+    fix."""
+    mpmath.mp.prec = 200
+    _, _, ps, pc = _constants()
+    assert len(ps) == 6 and len(pc) == 6
+    assert abs(ps[-1] + 1 / 6) < 1e-13 and abs(pc[-1] - 0.5) < 1e-13
+    es = ec = mpmath.mpf(0)
+    n = 2000
+    for i in range(n + 1):
+        r = mpmath.pi / 4 * i / n
+        z = r * r
+        S = mpmath.mpf(0)
+        for c in ps:
+            S = S * z + c
+        C = mpmath.mpf(0)
+        for c in pc:
+            C = C * z + c
+        es = max(es, abs(r + r * z * S - mpmath.sin(r)))
+        ec = max(ec, abs(1 - z * C - mpmath.cos(r)))
+    assert es < 1e-17, es
+    assert ec < 7e-17, ec
 
 
 def _fma(a, b, c):
@@ -78,6 +103,7 @@ def test_reduction_and_series_accuracy():
     rng = random.Random(7)
     xs = [0.0, 1e-300, math.pi / 4, math.pi / 2, -math.pi, 3 * math.pi / 4, 1e7, -1e7, 2.0 ** 40]
     xs += [rng.uniform(-sc, sc) for sc in (4, 1e3, 1e5, 1e7, 1e9, 1e12) for _ in range(150)]
+    xs += [k * math.pi / 4 + d for k in range(-8, 9) for d in (-1e-9, 0.0, 1e-9)]  # octant edges
     worst = 0.0
     for x in xs:
         s, c = _sincos(x, consts)
