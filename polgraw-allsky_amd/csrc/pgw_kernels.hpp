@@ -616,11 +616,25 @@ __device__ __forceinline__ void thr_wave(const ThrParams& p, int part, int t, in
     int ii = k;
     double Fc = fn;
     const bool ends = k + 1 < hik && !((nextm >> j) & 1);  // next bin of this part not pre-selected
-    for (int kk = k + 1; !ends && kk < p.nmax; kk++) {
-      const int pk = min((kk - p.nmin) / PGW_NAV_DEV, p.nparts - 1);
-      const double v = at(kk) / mu[pk];
-      if (!(v > p.trl)) break;
-      if (v >= Fc) { ii = kk; Fc = v; }
+    // the run's later bins RB at a time, their loads in flight together (addresses clamped below
+    // nmax; bins past the run's end are loaded and ignored)
+    constexpr int RB = TJ < 8 ? TJ : 8;
+    for (int kk0 = k + 1; !ends && kk0 < p.nmax; kk0 += RB) {
+      double v[RB];
+#pragma unroll
+      for (int q = 0; q < RB; q++) {
+        const int kk = min(kk0 + q, p.nmax - 1);
+        const int pk = min((kk - p.nmin) / PGW_NAV_DEV, p.nparts - 1);
+        v[q] = at(kk) / mu[pk];
+      }
+      bool done = false;
+#pragma unroll
+      for (int q = 0; q < RB; q++) {
+        if (done) continue;
+        if (kk0 + q >= p.nmax || !(v[q] > p.trl)) done = true;
+        else if (v[q] >= Fc) { ii = kk0 + q; Fc = v[q]; }
+      }
+      if (done) break;
     }
     const int idx = atomicAdd(ncand, 1);
     if (idx < p.cap) {
