@@ -99,5 +99,5 @@ def test_fstatnet_multi_equals_per_segment(abi):
         sel = np.nonzero(seg_of[perm] == k)[0]
         ro, rg = h.fstatnet(pts[perm][sel], seg=k)
         assert np.array_equal(out[sel], ro, equal_nan=True) and np.array_equal(g[sel], rg)
-        assert np.array_equal(out2[sel], ro, equal_nan=True)
+        assert np.array_equal(out2[sel], h.fstatnet(pts[perm][sel], seg=k, grad=False)[0], equal_nan=True)
     h.close()

@@ -62,9 +62,11 @@ def test_fstatnet_1024_points_network(pgw, abi, orc):
     print("1024 points: gpu ms", h.last_gpu_ms, flush=True)
     out_o, g_o = orc.Search(seg, nthreads=8).fstatnet(pts)
     _compare(seg, pts, sc, out_g, g_g, out_o, g_o)
-    # without the gradient the values are the same
+    # without the gradient the values are the same to the last bits (the values-only kernel takes its
+    # sin and cos from another FP64 implementation: 1.1e-16 absolute error against 1 ulp)
     out_n, g_n = h.fstatnet(pts, grad=False)
-    assert g_n is None and np.array_equal(out_n, out_g, equal_nan=True)
+    assert g_n is None
+    assert np.allclose(out_n, out_g, rtol=1e-12, atol=1e-12, equal_nan=True)
 
 
 def test_fstatnet_empty_and_errors(pgw, abi):
