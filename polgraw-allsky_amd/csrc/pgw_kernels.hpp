@@ -23,6 +23,7 @@ constexpr int NT = 256;          // threads per block everywhere
 constexpr int PGW_NAV_DEV = 4096; // FStat block length (NAVFSTAT, auxi.h:13)
 constexpr int SPL_W = 32;        // spline warm-up (|coefficient| ~ 0.268: 0.268^32 ~ 5e-19)
 constexpr int SPL_NTAB = 64;     // spline recursion coefficients tabulated below this index
+constexpr int SPL_SB = 8;        // spline steps per scheduling block (LDS reads issued ahead within one)
 
 struct SkyDet {
   double c[9];
@@ -223,15 +224,72 @@ struct SplineTab {
 };
 
 // Both recursions of the spline for spf_out(K) = NT K consecutive outputs of one series per
-// workgroup: the input window is read into LDS with coalesced loads, u (forward) and then y2
-// (backward) run in chunks of K per thread from LDS, each chunk starting SPL_W samples early from
-// a zero state, and y2 leaves through LDS with coalesced stores.  grid (ceil(n / spf_out(K)),
-// series).  LDS arrays are padded by one element per 16 so the lanes' chunk starts fall on
-// different banks.
+// workgroup: the input window is read into LDS with coalesced loads, u (forward, chunks of K + 1
+// per thread) and then y2 (backward, chunks of K) run from LDS, each chunk starting SPL_W samples
+// early from a zero state, and y2 leaves through LDS with coalesced stores.  grid (ceil(n /
+// spf_out(K)), series).  Both recursions run a fixed number of fully unrolled steps whose LDS
+// addresses are the lane's base plus a constant, so the reads of a block of SPL_SB steps issue
+// ahead of the dependent arithmetic.  Lane strides are odd in elements: K + 1 for the forward
+// reads of the unpadded y window, and K + 1 in the u array, which is padded by one element per K
+// (spk), for the backward reads of chunks of K.
 __host__ __device__ constexpr int spf_out(int K) { return NT * K; }            // y2 outputs per workgroup
 __host__ __device__ constexpr int spf_uw(int K) { return NT * K + SPL_W; }     // u values (outputs + warm-up above)
-__host__ __device__ constexpr int spf_yw(int K) { return NT * K + 2 * SPL_W + 2; }  // y samples
-__device__ __forceinline__ constexpr int spad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int spf_yw(int K) { return NT * (K + 1) + SPL_W + 2; }  // y samples the forward chunks read
+template <int K> __device__ __forceinline__ constexpr int spk(int m) { return m + m / K; }
+// coefficient tables of the first wave of a series' first workgroup, indexed along its chains:
+// invp at i + SPL_W for i in [-SPL_W, 64 (K + 1)), c at k for k in [0, 64 K + SPL_W)
+__host__ __device__ constexpr int spl_ti(int K) { return 64 * (K + 1) + SPL_W; }
+__host__ __device__ constexpr int spl_tc(int K) { return 64 * K + SPL_W; }
+
+// Forward recursion of one thread: u on [uf, uf + K + 1) within [A0, U1] into Us (at spk<K>(i - A0)),
+// from a zero state at uf - SPL_W.  Yw: y at uf - SPL_W - 1 onwards.  HEAD (first wave of a
+// series' first workgroup): invp from Ti (Ti[s] for step s), which is 0 for i < 1, so u stays 0
+// below the series' start (u_0 = 0, auxi.c:62), and the tabulated invp_i below SPL_NTAB.
+template <bool HEAD, typename T, int K>
+__device__ __forceinline__ void spline_fwd(const cplx<T>* Yw, double2* Us, const double* Ti, double invp_inf,
+                                           int uf, int A0, int U1) {
+  constexpr int KF = K + 1;
+  auto yat = [&](int m) {
+    const cplx<T> v = Yw[m];
+    return make_double2((double)v.x, (double)v.y);
+  };
+  double2 ym = yat(0), y0 = yat(1);
+  double ur = 0.0, ui = 0.0;
+#pragma unroll
+  for (int s = 0; s < SPL_W + KF; s++) {
+    const int i = uf - SPL_W + s;
+    const double2 yp = yat(s + 2);
+    const double invp = HEAD ? Ti[s] : invp_inf;
+    const double dr = ym.x - 2. * y0.x + yp.x, di = ym.y - 2. * y0.y + yp.y;
+    ur = (-.5 * ur + 3. * dr) * invp;
+    ui = (-.5 * ui + 3. * di) * invp;
+    if (s >= SPL_W && i <= U1) Us[spk<K>(i - A0)] = make_double2(ur, ui);
+    ym = y0;
+    y0 = yp;
+    if (s % SPL_SB == SPL_SB - 1) __builtin_amdgcn_sched_barrier(0);  // bounds the reads issued ahead
+  }
+}
+
+// Backward recursion of one thread: y2 on [a, a + K) into out, from a zero state at
+// a + K + SPL_W - 1.  Uw = Us + spk<K>(a - A0).  HEAD: c_k from Tc (Tc[m] for k = a + m).  TAIL
+// (last workgroups of a series): steps above n - 2 keep the state at 0 (y2_{n-1} = 0).
+template <bool HEAD, bool TAIL, int K>
+__device__ __forceinline__ void spline_bwd(const double2* Uw, const double* Tc, double c_inf, int a, int n,
+                                           double2 (&out)[K]) {
+  double zr = 0.0, zi = 0.0;
+#pragma unroll
+  for (int s = 0; s < SPL_W + K; s++) {
+    const int m = K + SPL_W - 1 - s;  // offset in the chunk: spk<K>(a - A0 + m) = spk<K>(a - A0) + spk<K>(m)
+    const double c = HEAD ? Tc[m] : c_inf;
+    const double2 uk = Uw[spk<K>(m)];
+    double nr = c * zr + uk.x, ni = c * zi + uk.y;
+    if (TAIL && a + m > n - 2) nr = ni = 0.0;
+    zr = nr;
+    zi = ni;
+    if (s >= SPL_W) out[K - 1 - (s - SPL_W)] = make_double2(zr, zi);
+    if (s % SPL_SB == SPL_SB - 1) __builtin_amdgcn_sched_barrier(0);
+  }
+}
 
 template <typename T, int K>
 __global__ __launch_bounds__(NT) void k_spline(const cplx<T>* __restrict__ y, size_t ystride,
@@ -239,62 +297,61 @@ __global__ __launch_bounds__(NT) void k_spline(const cplx<T>* __restrict__ y, si
                                                size_t stride, const SkyItem* __restrict__ items, int nb,
                                                const double* __restrict__ paa, const double* __restrict__ pbb,
                                                int npro) {
+  constexpr int KF = K + 1;
+  static_assert(NT * KF >= NT * K + SPL_W, "forward chunks cover the warm-up values");
+  static_assert(64 * KF >= SPL_NTAB + SPL_W && 64 * K >= SPL_NTAB, "tabulated coefficients stay in the first wave");
   constexpr int SPF_OUT = spf_out(K), SPF_UW = spf_uw(K), SPF_YW = spf_yw(K);
-  __shared__ cplx<T> Ys[spad(SPF_YW) + 1];
-  __shared__ double2 Us[spad(SPF_UW) + 1];
+  constexpr int TI = spl_ti(K), TC = spl_tc(K);
+  __shared__ cplx<T> Ys[SPF_YW];
+  __shared__ double2 Us[spk<K>(SPF_UW) + 1];
+  __shared__ double tsh[2 * NT > TI + TC ? 2 * NT : TI + TC];  // aa/bb reduction, then coefficient tables
   const int b = blockIdx.y, tid = threadIdx.x;
   const SkyItem& it = items[b / nb];
   // the sky point's aa, bb, by the first workgroup of its first series (saves a launch)
   if (blockIdx.x == 0 && b % nb == 0) {
-    __shared__ double red[2][NT];
     const size_t o = (size_t)(b / nb) * it.p.nifo * npro;
-    aabb_reduce(it.p, paa + o, pbb + o, npro, it.aabb, red);
+    aabb_reduce(it.p, paa + o, pbb + o, npro, it.aabb, reinterpret_cast<double (*)[NT]>(tsh));
   }
   const int n = it.nspl;
   const int A0 = blockIdx.x * SPF_OUT;
   if (A0 >= n) return;  // beyond a shorter sky point's knots (uniform across the workgroup)
   const int A1 = min(A0 + SPF_OUT, n);         // y2 outputs [A0, A1)
   const int U1 = min(A1 + SPL_W - 1, n - 2);   // u needed on [A0, U1]
-  const int i0 = max(1, A0 - SPL_W);           // first recursion index of any chunk
-  const int Ylo = i0 - 1, Yhi = U1 + 1;        // y window [Ylo, Yhi] (Yhi <= n - 1)
+  const int Ylo = A0 - SPL_W - 1, Yhi = U1 + 1;  // y window [Ylo, Yhi] (Yhi <= n - 1); below 0 only
+                                               // in the first workgroup, where invp = 0 voids it
   const cplx<T>* Y = y + (size_t)b * ystride;
   double2* Y2 = y2 + (size_t)b * stride;
-  // the window's loads all in flight before the LDS stores (addresses clamped to the window)
+  // the window's loads all in flight before the LDS stores (indices clamped to [0, Yhi]; entries
+  // beyond Yhi are read only by steps whose results are discarded)
   constexpr int NYL = (SPF_YW + NT - 1) / NT;
   cplx<T> yv[NYL];
 #pragma unroll
-  for (int r = 0; r < NYL; r++) yv[r] = Y[Ylo + min(tid + r * NT, Yhi - Ylo)];
+  for (int r = 0; r < NYL; r++) yv[r] = Y[min(max(Ylo + tid + r * NT, 0), Yhi)];
 #pragma unroll
   for (int r = 0; r < NYL; r++)
-    if (tid + r * NT <= Yhi - Ylo) Ys[spad(tid + r * NT)] = yv[r];
-  // recursion coefficients: the first SPL_NTAB from LDS, the limit from registers, so the
-  // recursions read no global memory
-  __shared__ double tinv[SPL_NTAB], tcf[SPL_NTAB];
-  if (tid < SPL_NTAB) {
-    tinv[tid] = tab->invp[tid];
-    tcf[tid] = tab->c[tid];
+    if (tid + r * NT < SPF_YW) Ys[tid + r * NT] = yv[r];
+  const bool head = A0 == 0;  // uniform across the workgroup
+  if (head) {
+    // invp_i, c_i (auxi.c:66-72): 0 at and below i = 0, tabulated below SPL_NTAB, the limit above
+    for (int j = tid; j < TI; j += NT) {
+      const int i = j - SPL_W;
+      tsh[j] = i < 1 ? 0.0 : i < SPL_NTAB ? tab->invp[i] : tab->invp_inf;
+    }
+    for (int k = tid; k < TC; k += NT) tsh[TI + k] = k < SPL_NTAB ? tab->c[k] : tab->c_inf;
   }
   const double invp_inf = tab->invp_inf, c_inf = tab->c_inf;
-  if (A0 == 0 && tid == 0) Us[spad(0)] = make_double2(0.0, 0.0);  // u_0 = 0 (auxi.c:62)
   __syncthreads();
-  // u_i = (-u_{i-1}/2 + 3 (y_{i-1} - 2 y_i + y_{i+1})) invp_i  (auxi.c:64-72)
-  // thread t: u on [A0 + t K, A0 + (t + 1) K); the thread of the last output chunk continues to U1
-  const int nch = (A1 - A0 + K - 1) / K;
-  if (tid < nch) {
-    const int ua = A0 + tid * K, ue = (tid == nch - 1) ? U1 + 1 : min(ua + K, U1 + 1);
-    int i = max(1, ua - SPL_W);
-    double ur = 0.0, ui = 0.0;
-    double ym_r = (double)Ys[spad(i - 1 - Ylo)].x, ym_i = (double)Ys[spad(i - 1 - Ylo)].y;
-    double y0_r = (double)Ys[spad(i - Ylo)].x, y0_i = (double)Ys[spad(i - Ylo)].y;
-    for (; i < ue; i++) {
-      const cplx<T> yp = Ys[spad(i + 1 - Ylo)];
-      const double yp_r = (double)yp.x, yp_i = (double)yp.y;
-      const double invp = i < SPL_NTAB ? tinv[i] : invp_inf;
-      const double dr = ym_r - 2. * y0_r + yp_r, di = ym_i - 2. * y0_i + yp_i;
-      ur = (-.5 * ur + 3. * dr) * invp;
-      ui = (-.5 * ui + 3. * di) * invp;
-      if (i >= ua) Us[spad(i - A0)] = make_double2(ur, ui);
-      ym_r = y0_r; ym_i = y0_i; y0_r = yp_r; y0_i = yp_i;
+  // u_i = (-u_{i-1}/2 + 3 (y_{i-1} - 2 y_i + y_{i+1})) invp_i  (auxi.c:64-72); thread t: u on
+  // [A0 + t (K+1), A0 + (t+1) (K+1)) within [A0, U1] (NT (K+1) covers the SPL_W - 1 values above
+  // the outputs that the backward warm-ups read)
+  const bool w0 = head && tid < 64;  // uniform across each wave
+  const bool tail = A0 + SPF_OUT + SPL_W - 1 > n - 2;
+  {
+    const int uf = A0 + tid * KF;
+    if (uf <= U1) {
+      const cplx<T>* Yw = Ys + tid * KF;  // y at uf - SPL_W - 1 = Ylo + tid KF
+      if (w0) spline_fwd<true, T, K>(Yw, Us, tsh + tid * KF, invp_inf, uf, A0, U1);
+      else spline_fwd<false, T, K>(Yw, Us, tsh, invp_inf, uf, A0, U1);
     }
   }
   __syncthreads();
@@ -302,34 +359,19 @@ __global__ __launch_bounds__(NT) void k_spline(const cplx<T>* __restrict__ y, si
   const int a = A0 + tid * K, e = min(a + K, A1);
   double2 out[K];
   if (a < A1) {
-    int k = (e + SPL_W >= n - 1) ? n - 2 : e + SPL_W - 1;
-    double zr = 0.0, zi = 0.0;
-    for (; k >= a + K; k--) {  // warm-up above the chunk
-      const double c = k < SPL_NTAB ? tcf[k] : c_inf;
-      const double2 uk = Us[spad(k - A0)];
-      zr = c * zr + uk.x;
-      zi = c * zi + uk.y;
-    }
-#pragma unroll
-    for (int j = K - 1; j >= 0; j--) {
-      const int kk = a + j;
-      if (kk <= k) {
-        const double c = kk < SPL_NTAB ? tcf[kk] : c_inf;
-        const double2 uk = Us[spad(kk - A0)];
-        zr = c * zr + uk.x;
-        zi = c * zi + uk.y;
-      }
-      out[j] = make_double2(zr, zi);
-    }
+    const double2* Uw = Us + spk<K>(a - A0);
+    if (w0) spline_bwd<true, true, K>(Uw, tsh + TI + tid * K, c_inf, a, n, out);
+    else if (tail) spline_bwd<false, true, K>(Uw, tsh, c_inf, a, n, out);
+    else spline_bwd<false, false, K>(Uw, tsh, c_inf, a, n, out);
   }
   __syncthreads();  // every u read is done: Us now stages y2
   if (a < A1) {
 #pragma unroll
     for (int j = 0; j < K; j++)
-      if (a + j < e) Us[spad(a + j - A0)] = out[j];
+      if (a + j < e) Us[spk<K>(a + j - A0)] = out[j];
   }
   __syncthreads();
-  for (int j = tid; j < A1 - A0; j += NT) Y2[A0 + j] = Us[spad(j)];
+  for (int j = tid; j < A1 - A0; j += NT) Y2[A0 + j] = Us[spk<K>(j)];
   if (A1 == n && tid == 0) {  // y2_{n-1} = 0 and the guard entry y2_n read when x == n-1
     Y2[n - 1] = make_double2(0.0, 0.0);
     Y2[n] = make_double2(0.0, 0.0);
