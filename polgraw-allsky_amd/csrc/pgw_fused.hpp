@@ -293,7 +293,7 @@ template <int P2, int P1> struct FusedShape {
   static constexpr int RJ = 256 / RowShape<P1>::NT;  // rows per row job
   static constexpr int NCOL = P2 * ColShape<P2, ColE<float, P2>::E>::C;
   static constexpr int NROW = RJ * RowShape<P1>::PADN;
-  static constexpr int NTHR = 4 * THR_MAXP;  // doubles: one mean table per wave
+  static constexpr int NTHR = 4 * THR_LDS;  // doubles: one mean table and bitmap per wave
   static constexpr int N = NCOL > NROW ? (NCOL > NTHR ? NCOL : NTHR) : (NROW > NTHR ? NROW : NTHR);
   // waves per SIMD: the job loop with the row pass inside needs 200-236 VGPRs (at 3 or 4 waves
   // per SIMD it spilled 30-130 registers)
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(256, (FusedShape<P2, P1>::W)) void k_spin_fused(con
       __syncthreads();
       const int w = threadIdx.x >> 6, part = 4 * i + w;
       if (part < a.tp.nparts)
-        thr_wave<float, 32, 16, true>(a.tp, part, t, threadIdx.x & 63, reinterpret_cast<double*>(lds) + w * THR_MAXP, a.Ft,
+        thr_wave<float, 32, 16, true>(a.tp, part, t, threadIdx.x & 63, reinterpret_cast<double*>(lds) + w * THR_LDS, a.Ft,
                         a.psum, a.pmax, a.mu, a.cand, a.ncand);
     }
     __syncthreads();  // lds and job_s are reused by the next job

@@ -561,8 +561,9 @@ __device__ __forceinline__ double part_mean(const double sum, int part, const Th
 }
 
 constexpr int THR_MAXP = 256;  // FStat blocks per template at most
+constexpr int THR_LDS = THR_MAXP + 64;  // doubles of LDS per threshold wave: block means, pre-selection bitmap
 
-// One wave (lane = 0..63) thresholds FStat block `part` of template t; mu: THR_MAXP doubles of LDS
+// One wave (lane = 0..63) thresholds FStat block `part` of template t; mu: THR_LDS doubles of LDS
 // owned by this wave.  TJ, QB: loads in flight per batch in the flagged path (bins, block means).
 // SC: F and the partials read at device scope (written by other workgroups of the same launch).
 template <typename T, int TJ = (sizeof(T) == 4 ? 32 : 16), int QB = 16, bool SC = false>
@@ -588,31 +589,51 @@ __device__ __forceinline__ void thr_wave(const ThrParams& p, int part, int t, in
   const double m = part_mean(sum, part, p);
   if (lane == 0) mu_out[(size_t)t * p.nparts + part] = m;
   if (!(mx / m > p.trl)) return;  // uniform across the wave
-  // flagged: this part's bins k = lok + lane + 64 j, all loads in flight at once (addresses
-  // clamped, no branches), pre-selected with F > trl m (1 - 1e-14), which every bin with
-  // F / m > trl passes; the exact test follows per selected bin
-  constexpr int NJ = PGW_NAV_DEV / 64;
+  // flagged: pre-select this part's bins with F > trl m (1 - 1e-14), which every bin with
+  // F / m > trl passes; the exact test follows per selected bin.  Bin k lies at row k mod L2,
+  // column k / L2 of F, so the part's bins are read row by row over the columns [cA, cA + ncol)
+  // that hold them, each load instruction covering contiguous row segments (element e = r ncol + c
+  // of that block for lane + 64 j, TJ loads in flight per batch).  A pre-selected bin at offset
+  // o = k - lok sets bit o / 64 of bitmap word o mod 64: lane l's word then holds the bins
+  // lok + l + 64 j that the exact tests below take lane by lane.
   const int lok = p.nmin + PGW_NAV_DEV * part;
   const int hik = (part < p.nblk) ? lok + PGW_NAV_DEV : p.nmax;
   const int mL2 = p.L2 - 1;
   auto at = [&](int k) { return (double)ld(&F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)]); };
-  // (TJ loads per batch keep the kernel's register count, and so the occupancy of the far more
-  // common unflagged waves, moderate)
-  static_assert(NJ % TJ == 0, "whole batches");
+  unsigned long long* bm = reinterpret_cast<unsigned long long*>(mu + THR_MAXP);
+  bm[lane] = 0;
+  wave_lds_sync();
   const double pre = p.trl * m * (1. - 1e-14);
-  uint64_t hits = 0;
+  {
+    const int cA = lok >> p.lgL2, ncol = ((hik - 1) >> p.lgL2) - cA + 1;
+    const int nel = ncol * p.L2;
+    const int dr = 64 / ncol, dc = 64 % ncol;  // the step of 64 elements in rows and columns
+    int r = lane / ncol, c = lane % ncol;
+    // (TJ loads per batch keep the kernel's register count, and so the occupancy of the far
+    // more common unflagged waves, moderate)
 #pragma unroll 1
-  for (int j0 = 0; j0 < NJ; j0 += TJ) {
-    T f[TJ];
+    for (int e0 = 0; e0 < nel; e0 += 64 * TJ) {
+      T f[TJ];
+      int o[TJ];
 #pragma unroll
-    for (int j = 0; j < TJ; j++) {
-      const int k = min(lok + lane + 64 * (j0 + j), hik - 1);
-      f[j] = ld(&F[(size_t)(k & mL2) * p.L1 + (k >> p.lgL2)]);
+      for (int j = 0; j < TJ; j++) {
+        const int k = (cA + c) * p.L2 + r;
+        o[j] = (r < p.L2 && k >= lok && k < hik) ? k - lok : -1;
+        f[j] = ld(&F[(size_t)min(r, mL2) * p.L1 + cA + c]);
+        c += dc;
+        r += dr;
+        if (c >= ncol) {
+          c -= ncol;
+          r++;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < TJ; j++)
+        if (o[j] >= 0 && (double)f[j] > pre) atomicOr(&bm[o[j] & 63], 1ull << (o[j] >> 6));
     }
-#pragma unroll
-    for (int j = 0; j < TJ; j++)
-      hits |= (uint64_t)(lok + lane + 64 * (j0 + j) < hik && (double)f[j] > pre) << (j0 + j);
   }
+  wave_lds_sync();
+  uint64_t hits = bm[lane];
   // means of all parts with the unflagged waves' arithmetic, QB parts' loads in flight at a time
 #pragma unroll 1
   for (int q0 = 0; q0 < p.nparts; q0 += QB) {
@@ -693,7 +714,7 @@ template <typename T>
 __global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restrict__ Ft, const T* __restrict__ psum,
                                                   const T* __restrict__ pmax, double* __restrict__ mu_out,
                                                   Cand* __restrict__ cand, int* __restrict__ ncand) {
-  __shared__ double mu[THR_MAXP];
+  __shared__ double mu[THR_LDS];
   thr_wave<T>(p, blockIdx.x, blockIdx.y, threadIdx.x, mu, Ft, psum, pmax, mu_out, cand, ncand);
 }
 

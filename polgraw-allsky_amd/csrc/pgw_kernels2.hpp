@@ -56,14 +56,14 @@ __global__ __launch_bounds__(NTH, (ColOcc<T, P2>::W)) void k_spin_col2(SpinParam
   constexpr int E = ColE<T, P2>::E;
   using S = ColShape<P2, E, NTH>;
   __shared__ T2 lds[(SEQ ? 1 : 2) * P2 * S::C];
-  static_assert(sizeof(lds) >= NTH / 64 * THR_MAXP * sizeof(double), "threshold waves' mean tables fit the LDS");
+  static_assert(sizeof(lds) >= NTH / 64 * THR_LDS * sizeof(double), "threshold waves' mean tables fit the LDS");
   if ((int)blockIdx.y < tf.rows) {  // whole workgroups: no barrier is split
     const int w = threadIdx.x >> 6;
     const int item = ((int)blockIdx.y * (int)gridDim.x + (int)blockIdx.x) * (NTH / 64) + w;
     // few loads in flight in the (rare) flagged path, so the column pass keeps its registers
     if (item < tf.p.nparts * tf.nt)
       thr_wave<T, 4, 2>(tf.p, item % tf.p.nparts, item / tf.p.nparts, threadIdx.x & 63,
-                  reinterpret_cast<double*>(lds) + w * THR_MAXP, tf.Ft, tf.psum, tf.pmax, tf.mu, tf.cand, tf.ncand);
+                  reinterpret_cast<double*>(lds) + w * THR_LDS, tf.Ft, tf.psum, tf.pmax, tf.mu, tf.cand, tf.ncand);
     return;
   }
   const int t = blockIdx.y - tf.rows;
