@@ -271,11 +271,12 @@ __device__ __forceinline__ void spline_fwd(const cplx<T>* Yw, double2* Us, const
 }
 
 // Backward recursion of one thread: y2 on [a, a + K) into out, from a zero state at
-// a + K + SPL_W - 1.  Uw = Us + spk<K>(a - A0).  HEAD: c_k from Tc (Tc[m] for k = a + m).  TAIL
-// (last workgroups of a series): steps above n - 2 keep the state at 0 (y2_{n-1} = 0).
+// a + K + SPL_W - 1 (and, if above is given, y2 at a + K).  Uw = Us + spk<K>(a - A0).  HEAD: c_k from
+// Tc (Tc[m] for k = a + m).  TAIL (last workgroups of a series): steps above n - 2 keep the state at
+// 0 (y2_{n-1} = 0).
 template <bool HEAD, bool TAIL, int K>
 __device__ __forceinline__ void spline_bwd(const double2* Uw, const double* Tc, double c_inf, int a, int n,
-                                           double2 (&out)[K]) {
+                                           double2 (&out)[K], double2* above = nullptr) {
   double zr = 0.0, zi = 0.0;
 #pragma unroll
   for (int s = 0; s < SPL_W + K; s++) {
@@ -287,6 +288,7 @@ __device__ __forceinline__ void spline_bwd(const double2* Uw, const double* Tc, 
     zr = nr;
     zi = ni;
     if (s >= SPL_W) out[K - 1 - (s - SPL_W)] = make_double2(zr, zi);
+    if (above && s == SPL_W - 1) *above = make_double2(zr, zi);  // y2 at a + K (SPL_W - 1 warm-up steps)
     if (s % SPL_SB == SPL_SB - 1) __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -408,6 +410,161 @@ __global__ __launch_bounds__(NT) void k_spline_eval(const cplx<T>* __restrict__ 
   }
   const double sig2 = it.p.det[d].sig2;
   static_cast<cplx<TO>*>(it.xm)[(size_t)bs * N + i] = mk<cplx<TO>>((TO)(vr / sig2), (TO)(vi / sig2));
+}
+
+// The resampled series' last three steps in one kernel: the inverse FFT's T-layout output Zi (L2
+// rows of L1, series b at b * L1 * L2) read as the spline's y window with the scale ft applied
+// (k_untranspose), both spline recursions (k_spline), and the evaluation at interpftpad (i -
+// shftf[i]) divided by sig2 for the outputs i whose klo falls in this workgroup's knots
+// (k_spline_eval): y and y2 stay in LDS.  x_i = interpftpad (i - shftf_i) increases with i, so a
+// workgroup's outputs are the i in [min{i : x_i >= A0}, min{i : x_i >= A1}) (from 0 in the first
+// workgroup, to N in the last); each bound is found by one wave from a guess at A / interpftpad +
+// shftf and a 64-wide window around it.  A bound outside its window (the shift would have to change
+// by more than 30 samples over the distance of the shift) sets *err and leaves the outputs
+// unwritten.  y2 at A1 (the knot above the last output knot) comes from the last chunk's warm-up.
+template <typename T, int K>
+__global__ __launch_bounds__(NT) void k_spline_fused(const cplx<T>* __restrict__ Zi, int L1, int lgL2, T ft,
+                                                     size_t zstride, const SplineTab* __restrict__ tab,
+                                                     const double* __restrict__ shftf, size_t Nstride, int interpftpad,
+                                                     const SkyItem* __restrict__ items, int nb,
+                                                     const double* __restrict__ paa, const double* __restrict__ pbb,
+                                                     int npro, int* __restrict__ err) {
+  constexpr int KF = K + 1;
+  static_assert(NT * KF >= NT * K + SPL_W, "forward chunks cover the warm-up values");
+  static_assert(64 * KF >= SPL_NTAB + SPL_W && 64 * K >= SPL_NTAB, "tabulated coefficients stay in the first wave");
+  constexpr int SPF_OUT = spf_out(K), SPF_UW = spf_uw(K), SPF_YW = spf_yw(K);
+  constexpr int TI = spl_ti(K), TC = spl_tc(K);
+  __shared__ cplx<T> Ys[SPF_YW];
+  __shared__ double2 Us[spk<K>(SPF_UW) + 1];
+  __shared__ double tsh[2 * NT > TI + TC ? 2 * NT : TI + TC];
+  __shared__ int ibound[2];
+  const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int j = b / nb, bs = b % nb, d = bs >> 1;
+  const SkyItem& it = items[j];
+  if (blockIdx.x == 0 && bs == 0) {
+    const size_t o = (size_t)j * it.p.nifo * npro;
+    aabb_reduce(it.p, paa + o, pbb + o, npro, it.aabb, reinterpret_cast<double (*)[NT]>(tsh));
+  }
+  const int n = it.nspl, N = it.p.N;
+  const int A0 = blockIdx.x * SPF_OUT;
+  if (A0 >= n) return;  // beyond a shorter sky point's knots (uniform across the workgroup)
+  const int A1 = min(A0 + SPF_OUT, n);
+  const int U1 = min(A1 + SPL_W - 1, n - 2);
+  const int Ylo = A0 - SPL_W - 1, Yhi = U1 + 1;
+  const double* sh = shftf + ((size_t)j * it.p.nifo + d) * Nstride;
+  // output bounds: wave 0 the lower, wave 1 the upper (guess and window loads in flight early)
+  if (w < 2) {
+    const int A = w == 0 ? A0 : A1;
+    const bool fixed = w == 0 ? A0 == 0 : A1 == n;
+    if (fixed) {
+      if (lane == 0) ibound[w] = w == 0 ? 0 : N;
+    } else {
+      const int i0 = min(max(A / interpftpad, 0), N - 1);
+      const int g = min(max((int)floor((double)A / interpftpad + sh[i0]) - 31, 0), N - 1);
+      const int i = g + lane;
+      const double x = i < 0 ? -1.0 : i >= N ? 1e300 : interpftpad * (i - sh[i]);
+      const unsigned long long bal = __ballot(x >= (double)A);
+      if (lane == 0) {
+        if ((bal & 1ull) || bal == 0) {  // the bound is not inside the window
+          if (err) *err = 2;
+          ibound[w] = -1;
+        } else {
+          ibound[w] = g + __builtin_ctzll(bal);
+        }
+      }
+    }
+  }
+  // y window from the T-layout: element (r, c) of the block of rows 0..L2-1 and columns
+  // [cA, cA + ncol) holding y on [max(Ylo, 0), Yhi], row by row (contiguous row segments)
+  {
+    const int L2 = 1 << lgL2, jlo = max(Ylo, 0);
+    const int cA = jlo >> lgL2, ncol = (Yhi >> lgL2) - cA + 1, nel = ncol * L2;
+    const cplx<T>* Z = Zi + (size_t)b * zstride;
+    const int dr = NT / ncol, dc = NT % ncol;
+    int r = tid / ncol, c = tid % ncol;
+    for (int e0 = 0; e0 < nel; e0 += NT) {
+      const int jj = (cA + c) * L2 + r;
+      if (e0 + tid < nel && jj >= jlo && jj <= Yhi) {
+        const cplx<T> v = Z[(size_t)r * L1 + cA + c];
+        Ys[jj - Ylo] = mk<cplx<T>>(v.x * ft, v.y * ft);
+      }
+      c += dc;
+      r += dr;
+      if (c >= ncol) {
+        c -= ncol;
+        r++;
+      }
+    }
+    // window entries below 0 (first workgroup) and above Yhi: finite values for discarded steps
+    for (int m = tid; m < SPF_YW; m += NT)
+      if (m < jlo - Ylo || m > Yhi - Ylo) Ys[m] = mk<cplx<T>>(0, 0);
+  }
+  const bool head = A0 == 0;
+  if (head) {
+    for (int q = tid; q < TI; q += NT) {
+      const int i = q - SPL_W;
+      tsh[q] = i < 1 ? 0.0 : i < SPL_NTAB ? tab->invp[i] : tab->invp_inf;
+    }
+    for (int k = tid; k < TC; k += NT) tsh[TI + k] = k < SPL_NTAB ? tab->c[k] : tab->c_inf;
+  }
+  const double invp_inf = tab->invp_inf, c_inf = tab->c_inf;
+  __syncthreads();
+  const bool w0 = head && tid < 64;
+  const bool tail = A0 + SPF_OUT + SPL_W - 1 > n - 2;
+  {
+    const int uf = A0 + tid * KF;
+    if (uf <= U1) {
+      const cplx<T>* Yw = Ys + tid * KF;
+      if (w0) spline_fwd<true, T, K>(Yw, Us, tsh + tid * KF, invp_inf, uf, A0, U1);
+      else spline_fwd<false, T, K>(Yw, Us, tsh, invp_inf, uf, A0, U1);
+    }
+  }
+  __syncthreads();
+  const int a = A0 + tid * K, e = min(a + K, A1);
+  double2 out[K + 1];  // out[K]: y2 at a + K, from the warm-up
+  if (a < A1) {
+    const double2* Uw = Us + spk<K>(a - A0);
+    double2 o2[K];
+    if (w0) spline_bwd<true, true, K>(Uw, tsh + TI + tid * K, c_inf, a, n, o2, &out[K]);
+    else if (tail) spline_bwd<false, true, K>(Uw, tsh, c_inf, a, n, o2, &out[K]);
+    else spline_bwd<false, false, K>(Uw, tsh, c_inf, a, n, o2, &out[K]);
+#pragma unroll
+    for (int q = 0; q < K; q++) out[q] = o2[q];
+  }
+  __syncthreads();  // every u read is done: Us now holds y2 on [A0, A1]
+  if (a < A1) {
+#pragma unroll
+    for (int q = 0; q < K; q++)
+      if (a + q < e) Us[spk<K>(a + q - A0)] = out[q];
+    if (a + K == A1 && A1 < n) Us[spk<K>(A1 - A0)] = out[K];
+  }
+  __syncthreads();
+  // splint at interpftpad (i - shftf[i]) (auxi.c:83-120), / sig2 (jobcore.c:386-389)
+  const int ilo = ibound[0], ihi = ibound[1];
+  if (ilo < 0 || ihi < 0) return;
+  const double sig2 = it.p.det[d].sig2;
+  cplx<T>* xo = static_cast<cplx<T>*>(it.xm) + (size_t)bs * N;
+  for (int i = ilo + tid; i < ihi; i += NT) {
+    const double x = interpftpad * (i - sh[i]);
+    double vr = 0.0, vi = 0.0;
+    if (!(x < 0 || x > n - 1)) {
+      const int klo = (int)floor(x), khi = klo + 1;
+      const double Aw = khi - x, Bw = x - klo;
+      const double ca = Aw * Aw * Aw - Aw, cb = Bw * Bw * Bw - Bw;
+      const cplx<T> yl = Ys[klo - Ylo];
+      const double2 ylo = make_double2((double)yl.x, (double)yl.y);
+      double2 yhi = make_double2(0.0, 0.0), zhi = make_double2(0.0, 0.0);
+      if (khi < n) {  // khi == n only at x == n-1 exactly, where its weight Bw is 0
+        const cplx<T> yh = Ys[khi - Ylo];
+        yhi = make_double2((double)yh.x, (double)yh.y);
+        zhi = Us[spk<K>(khi - A0)];
+      }
+      const double2 zlo = Us[spk<K>(klo - A0)];
+      vr = Aw * ylo.x + Bw * yhi.x + (ca * zlo.x + cb * zhi.x) / 6.0;
+      vi = Aw * ylo.y + Bw * yhi.y + (ca * zlo.y + cb * zhi.y) / 6.0;
+    }
+    xo[i] = mk<cplx<T>>((T)(vr / sig2), (T)(vi / sig2));
+  }
 }
 
 // ---------------------------------------------------------- template stage --

@@ -485,6 +485,7 @@ template <typename T> struct Engine final : pgw_handle {
   int row_na = 1;
   bool col_seq = true;  // column pass: the two spectra share one LDS buffer in turn
   int spl_k = 4;  // spline outputs per thread (4, 8 or 16)
+  bool spl_fuse = true;  // untranspose, spline and its evaluation in one kernel (k_spline_fused)
   bool row_half = true;  // row pass forms only the outputs that can fall in [nmin, nmax)
   // row pass with 8 elements per thread (1 with the second row prefetched, 2 without; 3 and 4
   // likewise with the transforms' later exchanges inside each wave): the FP64 default, 3 (two
@@ -669,6 +670,7 @@ template <typename T> struct Engine final : pgw_handle {
       if (const char* q = strstr(tune, "tb=")) if (sscanf(q, "tb=%d", &v) == 1 && v >= 1 && v <= kTBMax) TB = v;
       if (const char* q = strstr(tune, "group=")) if (sscanf(q, "group=%d", &v) == 1 && v >= 1 && v <= kSlots) group = v;
       if (const char* q = strstr(tune, "spl_k=")) if (sscanf(q, "spl_k=%d", &v) == 1 && (v == 4 || v == 8 || v == 16)) spl_k = v;
+      if (const char* q = strstr(tune, "spl_fuse=")) if (sscanf(q, "spl_fuse=%d", &v) == 1) spl_fuse = v != 0;
       if (const char* q = strstr(tune, "half=")) if (sscanf(q, "half=%d", &v) == 1) row_half = v != 0;
       if (const char* q = strstr(tune, "row8=")) if (sscanf(q, "row8=%d", &v) == 1 && v >= 0 && v <= 4) row8 = v;
       if (const char* q = strstr(tune, "thr_fuse=")) if (sscanf(q, "thr_fuse=%d", &v) == 1) thr_fuse = v != 0;
@@ -822,6 +824,22 @@ template <typename T> struct Engine final : pgw_handle {
     LoadZeroInsert<T2> lz{Zf.p, S.nfft, S.Ninterp, S.nfft / 2 + 1, g0.pf->L1, g0.pf->L2};
     launch_col<T, +1>(*g0.pi, lz, Zi.p, nb * G, hi.p, lo.p, stream);
     launch_row<T, +1>(*g0.pi, Zi.p, nb * G, stream);
+    if (spl_fuse) {
+      // untranspose, spline and evaluation in one kernel (y and y2 stay in LDS)
+      int lg = 0;
+      while ((1 << lg) < g0.pi->L2) lg++;
+      const T ft = (T)((double)S.interpftpad / S.Ninterp);
+      int* err = zero ? zero + 1 : nullptr;
+#define PGW_SPLINE_F(K)                                                                                           \
+  hipLaunchKernelGGL((k_spline_fused<T, K>), dim3((nsplg + spf_out(K) - 1) / spf_out(K), nb * G), dim3(NT), 0,    \
+                     stream, Zi.p, g0.pi->L1, lg, ft, (size_t)S.Ninterp, spl.p, shftf.p, Nmax, S.interpftpad, dp, \
+                     nb, paa.p, pbb.p, g0.npro, err)
+      if (spl_k == 4) PGW_SPLINE_F(4);
+      else if (spl_k == 16) PGW_SPLINE_F(16);
+      else PGW_SPLINE_F(8);
+#undef PGW_SPLINE_F
+      return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
+    }
     const int count = std::min(S.Ninterp, nsplg);
     const int rows_k1 = std::min(g0.pi->L1, (count + g0.pi->L2 - 1) / g0.pi->L2);
     dim3 gt((rows_k1 + 31) / 32, (g0.pi->L2 + 31) / 32, nb * G);
@@ -1433,11 +1451,12 @@ template <typename T> struct Engine final : pgw_handle {
     if (!buf || len <= 0) return PGW_ERR_ARG;
     const int n = snprintf(buf, (size_t)len,
                            "tb=%d,row_na=%d,row8=%d,col_seq=%d,col_lf=%d,col_nt=%d,row_nt=%d,f_nt=%d,rev=%d,"
-                           "half=%d,spl_k=%d,group=%d,thr_fuse=%d,l1max=%d,candcap=%d,graph=%d,fused=%d,flag=%d,col_wide=%d",
+                           "half=%d,spl_k=%d,group=%d,thr_fuse=%d,l1max=%d,candcap=%d,graph=%d,fused=%d,flag=%d,col_wide=%d,"
+                           "spl_fuse=%d",
                            TB, row_na, row8, col_seq ? 1 : 0, col_lf ? 1 : 0, col_nt ? 1 : 0, row_nt ? 1 : 0,
                            f_nt ? 1 : 0, phase_rev ? 1 : 0,
                            row_half ? 1 : 0, spl_k, group, thr_fuse ? 1 : 0, l1max, cand_cap, graph ? 1 : 0, fused ? 1 : 0,
-                           fused_lag, col_wide ? 1 : 0);
+                           fused_lag, col_wide ? 1 : 0, spl_fuse ? 1 : 0);
     return n >= 0 && n < len ? PGW_OK : PGW_ERR_ARG;
   }
 

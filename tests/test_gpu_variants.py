@@ -1,7 +1,8 @@
 """Kernel variants (chosen per process with PGW_TUNE, or by the autotuner) give the oracle's
 records: row pass with 16 or 8 elements per thread (the latter also with the transforms' later
 exchanges inside each wave, row8=3/4), with and without the half-spectrum pruning,
-both spectra together, column pass with shared LDS, spline chunk lengths, launch and group
+both spectra together, column pass with shared LDS, spline chunk lengths, the spline with its
+untranspose and evaluation fused or as three kernels (spl_fuse), launch and group
 sizes, thresholds folded into the next column launch or launched on their own, a device
 candidate buffer smaller than a group's candidates (the group runs again into a buffer of the
 needed size), FFT rows capped at 2048 or 1024 points (longer columns), groups launched as captured HIP
@@ -24,7 +25,8 @@ VARIANTS = ["", "row8=0", "row8=1", "row8=2", "half=0", "row8=0,half=0", "row8=2
             "candcap=16,group=1", "l1max=2048", "l1max=2048,row8=0", "l1max=1024", "row8=3", "row8=4",
             "row8=3,half=0", "row8=4,half=0", "row8=3,l1max=2048", "row8=3,l1max=1024", "row8=4,tb=8",
             "col_nt=1", "col_nt=1,tb=8", "col_nt=1,col_seq=0", "col_nt=1,l1max=1024", "row_nt=0",
-            "row_nt=0,row8=0", "row_nt=0,row8=1", "f_nt=0", "f_nt=0,row8=0", "graph=1", "graph=1,group=1,tb=8"]
+            "row_nt=0,row8=0", "row_nt=0,row8=1", "f_nt=0", "f_nt=0,row8=0", "graph=1", "graph=1,group=1,tb=8",
+            "spl_fuse=0", "spl_fuse=0,spl_k=16"]
 CASES = ["plain-1det-32k", "net-2det-32k", "net-2det-86k"]
 
 
