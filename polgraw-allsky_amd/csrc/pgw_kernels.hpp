@@ -187,6 +187,24 @@ template <typename T2> struct LoadZeroInsert {
   }
 };
 
+// Odd samples of the two-fold interpolation (jobcore.c:334-352) without the 2 nf-point transform:
+// at 2m+1 the inverse transform of the zero-inserted spectrum is the nf-point inverse transform of
+// s_k e^{i pi k / nf} X[k] (s_k = -1 for k >= nyqst: those bins sit above the inserted zeros, at
+// frequency k - nf), and at 2m it is nf times the input series.  X: the forward spectrum (length
+// nf, T-layout with row length L1f and L2f rows); e^{i pi k / nf} = conj(W_{2^20}^{k << shift}) from
+// the shared twiddle tables (shift = 20 - log2(2 nf)).
+template <typename T2> struct LoadHalfShift {
+  const T2* src;
+  const T2* hi;
+  const T2* lo;
+  int nf, nyqst, L1f, L2f, shift;
+  __device__ T2 operator()(int b, int k) const {
+    const T2 x = src[(size_t)b * nf + (size_t)(k % L2f) * L1f + k / L2f];
+    const T2 v = cmul(x, conjg(tw_big(hi, lo, (uint32_t)k << shift)));
+    return k < nyqst ? v : mk<T2>(-v.x, -v.y);
+  }
+};
+
 // T-layout (rows of L1, L2 rows) -> natural order y[b][j] = scale * X[j], j < count.
 // grid (ceil(count/ (32*L2)) ... ) handled as tiles of 32 x 32 over the [L2][L1] matrix.
 template <typename T>
@@ -413,8 +431,9 @@ __global__ __launch_bounds__(NT) void k_spline_eval(const cplx<T>* __restrict__ 
 }
 
 // The resampled series' last three steps in one kernel: the inverse FFT's T-layout output Zi (L2
-// rows of L1, series b at b * L1 * L2) read as the spline's y window with the scale ft applied
-// (k_untranspose), both spline recursions (k_spline), and the evaluation at interpftpad (i -
+// rows of L1, series b at b * zstride) read as the spline's y window with the scale ft applied
+// (k_untranspose; HALF: Zi holds the odd samples only (LoadHalfShift) and the even ones are the
+// input series R), both spline recursions (k_spline), and the evaluation at interpftpad (i -
 // shftf[i]) divided by sig2 for the outputs i whose klo falls in this workgroup's knots
 // (k_spline_eval): y and y2 stay in LDS.  x_i = interpftpad (i - shftf_i) increases with i, so a
 // workgroup's outputs are the i in [min{i : x_i >= A0}, min{i : x_i >= A1}) (from 0 in the first
@@ -422,9 +441,10 @@ __global__ __launch_bounds__(NT) void k_spline_eval(const cplx<T>* __restrict__ 
 // shftf and a 64-wide window around it.  A bound outside its window (the shift would have to change
 // by more than 30 samples over the distance of the shift) sets *err and leaves the outputs
 // unwritten.  y2 at A1 (the knot above the last output knot) comes from the last chunk's warm-up.
-template <typename T, int K>
+template <typename T, int K, bool HALF>
 __global__ __launch_bounds__(NT) void k_spline_fused(const cplx<T>* __restrict__ Zi, int L1, int lgL2, T ft,
-                                                     size_t zstride, const SplineTab* __restrict__ tab,
+                                                     size_t zstride, const cplx<T>* __restrict__ R, size_t rstride,
+                                                     const SplineTab* __restrict__ tab,
                                                      const double* __restrict__ shftf, size_t Nstride, int interpftpad,
                                                      const SkyItem* __restrict__ items, int nb,
                                                      const double* __restrict__ paa, const double* __restrict__ pbb,
@@ -474,19 +494,23 @@ __global__ __launch_bounds__(NT) void k_spline_fused(const cplx<T>* __restrict__
       }
     }
   }
-  // y window from the T-layout: element (r, c) of the block of rows 0..L2-1 and columns
-  // [cA, cA + ncol) holding y on [max(Ylo, 0), Yhi], row by row (contiguous row segments)
+  // y window on [jlo, Yhi]: from the T-layout, element (r, c) of the block of rows 0..L2-1 and
+  // columns [cA, cA + ncol) holding the samples m in [mlo, mhi], read row by row (contiguous row
+  // segments); sample m is y at j = 2m + 1 (HALF: the odd samples' transform) or at j = m.  HALF:
+  // the even samples y at 2m are the input series R itself.
+  const int jlo = max(Ylo, 0);
   {
-    const int L2 = 1 << lgL2, jlo = max(Ylo, 0);
-    const int cA = jlo >> lgL2, ncol = (Yhi >> lgL2) - cA + 1, nel = ncol * L2;
+    const int L2 = 1 << lgL2;
+    const int mlo = HALF ? jlo / 2 : jlo, mhi = HALF ? (Yhi - 1) / 2 : Yhi;
+    const int cA = mlo >> lgL2, ncol = (mhi >> lgL2) - cA + 1, nel = ncol * L2;
     const cplx<T>* Z = Zi + (size_t)b * zstride;
     const int dr = NT / ncol, dc = NT % ncol;
     int r = tid / ncol, c = tid % ncol;
     for (int e0 = 0; e0 < nel; e0 += NT) {
-      const int jj = (cA + c) * L2 + r;
-      if (e0 + tid < nel && jj >= jlo && jj <= Yhi) {
+      const int m = (cA + c) * L2 + r;
+      if (e0 + tid < nel && m >= mlo && m <= mhi) {
         const cplx<T> v = Z[(size_t)r * L1 + cA + c];
-        Ys[jj - Ylo] = mk<cplx<T>>(v.x * ft, v.y * ft);
+        Ys[(HALF ? 2 * m + 1 : m) - Ylo] = mk<cplx<T>>(v.x * ft, v.y * ft);
       }
       c += dc;
       r += dr;
@@ -495,10 +519,14 @@ __global__ __launch_bounds__(NT) void k_spline_fused(const cplx<T>* __restrict__
         r++;
       }
     }
-    // window entries below 0 (first workgroup) and above Yhi: finite values for discarded steps
-    for (int m = tid; m < SPF_YW; m += NT)
-      if (m < jlo - Ylo || m > Yhi - Ylo) Ys[m] = mk<cplx<T>>(0, 0);
+    if (HALF) {
+      const cplx<T>* Rb = R + (size_t)b * rstride;
+      for (int m = (jlo + 1) / 2 + tid; m <= Yhi / 2; m += NT) Ys[2 * m - Ylo] = Rb[m];
+    }
   }
+  // window entries below 0 (first workgroup) and above Yhi: finite values for discarded steps
+  for (int m = tid; m < SPF_YW; m += NT)
+    if (m < jlo - Ylo || m > Yhi - Ylo) Ys[m] = mk<cplx<T>>(0, 0);
   const bool head = A0 == 0;
   if (head) {
     for (int q = tid; q < TI; q += NT) {

@@ -821,29 +821,32 @@ template <typename T> struct Engine final : pgw_handle {
     LoadPlain<T2> lp{R.p, S.nfft, S.nfft};
     launch_col<T, -1>(*g0.pf, lp, Zf.p, nb * G, hi.p, lo.p, stream);
     launch_row<T, -1>(*g0.pf, Zf.p, nb * G, stream);
-    LoadZeroInsert<T2> lz{Zf.p, S.nfft, S.Ninterp, S.nfft / 2 + 1, g0.pf->L1, g0.pf->L2};
-    launch_col<T, +1>(*g0.pi, lz, Zi.p, nb * G, hi.p, lo.p, stream);
-    launch_row<T, +1>(*g0.pi, Zi.p, nb * G, stream);
+    const T ft = (T)((double)S.interpftpad / S.Ninterp);
     if (spl_fuse) {
-      // untranspose, spline and evaluation in one kernel (y and y2 stay in LDS)
+      // odd samples by an nfft-point inverse transform (even ones: R itself); then untranspose,
+      // spline and evaluation in one kernel (y and y2 stay in LDS)
+      LoadHalfShift<T2> lh{Zf.p, hi.p, lo.p, S.nfft, S.nfft / 2 + 1, g0.pf->L1, g0.pf->L2, g0.pi->twshift};
+      launch_col<T, +1>(*g0.pf, lh, Zi.p, nb * G, hi.p, lo.p, stream);
+      launch_row<T, +1>(*g0.pf, Zi.p, nb * G, stream);
       int lg = 0;
-      while ((1 << lg) < g0.pi->L2) lg++;
-      const T ft = (T)((double)S.interpftpad / S.Ninterp);
+      while ((1 << lg) < g0.pf->L2) lg++;
       int* err = zero ? zero + 1 : nullptr;
-#define PGW_SPLINE_F(K)                                                                                           \
-  hipLaunchKernelGGL((k_spline_fused<T, K>), dim3((nsplg + spf_out(K) - 1) / spf_out(K), nb * G), dim3(NT), 0,    \
-                     stream, Zi.p, g0.pi->L1, lg, ft, (size_t)S.Ninterp, spl.p, shftf.p, Nmax, S.interpftpad, dp, \
-                     nb, paa.p, pbb.p, g0.npro, err)
+#define PGW_SPLINE_F(K)                                                                                          \
+  hipLaunchKernelGGL((k_spline_fused<T, K, true>), dim3((nsplg + spf_out(K) - 1) / spf_out(K), nb * G), dim3(NT), \
+                     0, stream, Zi.p, g0.pf->L1, lg, ft, (size_t)S.nfft, R.p, (size_t)S.nfft, spl.p, shftf.p,    \
+                     Nmax, S.interpftpad, dp, nb, paa.p, pbb.p, g0.npro, err)
       if (spl_k == 4) PGW_SPLINE_F(4);
       else if (spl_k == 16) PGW_SPLINE_F(16);
       else PGW_SPLINE_F(8);
 #undef PGW_SPLINE_F
       return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
     }
+    LoadZeroInsert<T2> lz{Zf.p, S.nfft, S.Ninterp, S.nfft / 2 + 1, g0.pf->L1, g0.pf->L2};
+    launch_col<T, +1>(*g0.pi, lz, Zi.p, nb * G, hi.p, lo.p, stream);
+    launch_row<T, +1>(*g0.pi, Zi.p, nb * G, stream);
     const int count = std::min(S.Ninterp, nsplg);
     const int rows_k1 = std::min(g0.pi->L1, (count + g0.pi->L2 - 1) / g0.pi->L2);
     dim3 gt((rows_k1 + 31) / 32, (g0.pi->L2 + 31) / 32, nb * G);
-    const T ft = (T)((double)S.interpftpad / S.Ninterp);
     hipLaunchKernelGGL((k_untranspose<T>), gt, dim3(NT), 0, stream, Zi.p, Y.p, g0.pi->L1, g0.pi->L2, count, ft);
     // spline on the first interpftpad*N samples, evaluated at interpftpad*(i - shftf[i]) (auxi.c:98-122)
 #define PGW_SPLINE(K)                                                                                           \
