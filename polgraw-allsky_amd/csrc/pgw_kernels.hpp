@@ -190,18 +190,21 @@ template <typename T2> struct LoadZeroInsert {
 // Odd samples of the two-fold interpolation (jobcore.c:334-352) without the 2 nf-point transform:
 // at 2m+1 the inverse transform of the zero-inserted spectrum is the nf-point inverse transform of
 // s_k e^{i pi k / nf} X[k] (s_k = -1 for k >= nyqst: those bins sit above the inserted zeros, at
-// frequency k - nf), and at 2m it is nf times the input series.  X: the forward spectrum (length
-// nf, T-layout with row length L1f and L2f rows); e^{i pi k / nf} = conj(W_{2^20}^{k << shift}) from
-// the shared twiddle tables (shift = 20 - log2(2 nf)).
-template <typename T2> struct LoadHalfShift {
+// frequency k - nf), and at 2m it is nf times the input series.  That inverse transform runs as
+// row transforms over k1 of the spectrum's T-layout rows k2 (k = L2 k1 + k2; k_fft_row_shift),
+// then twiddles and column transforms over k2 whose outputs are in natural order.  This loader
+// feeds the column pass: element n = n1 + L1 k2 of the row-transformed array times
+// W_L^{-n1 k2} e^{i pi k2 / L} = conj(W_{2L}^{(2 n1 + 1) k2}) from the shared twiddle tables
+// (shift = 20 - log2(2 L)).
+template <typename T2> struct LoadOddTw {
   const T2* src;
   const T2* hi;
   const T2* lo;
-  int nf, nyqst, L1f, L2f, shift;
-  __device__ T2 operator()(int b, int k) const {
-    const T2 x = src[(size_t)b * nf + (size_t)(k % L2f) * L1f + k / L2f];
-    const T2 v = cmul(x, conjg(tw_big(hi, lo, (uint32_t)k << shift)));
-    return k < nyqst ? v : mk<T2>(-v.x, -v.y);
+  int L, lgL1, shift;
+  __device__ T2 operator()(int b, int n) const {
+    const uint32_t n1 = (uint32_t)n & ((1u << lgL1) - 1u), k2 = (uint32_t)n >> lgL1;
+    const uint32_t m = ((2u * n1 + 1u) * k2) & (2u * (uint32_t)L - 1u);
+    return cmul(src[(size_t)b * L + n], conjg(tw_big(hi, lo, m << shift)));
   }
 };
 
@@ -432,8 +435,8 @@ __global__ __launch_bounds__(NT) void k_spline_eval(const cplx<T>* __restrict__ 
 
 // The resampled series' last three steps in one kernel: the inverse FFT's T-layout output Zi (L2
 // rows of L1, series b at b * zstride) read as the spline's y window with the scale ft applied
-// (k_untranspose; HALF: Zi holds the odd samples only (LoadHalfShift) and the even ones are the
-// input series R), both spline recursions (k_spline), and the evaluation at interpftpad (i -
+// (k_untranspose; HALF: Zi holds the odd samples only, in natural order (LoadOddTw), and the even
+// ones are the input series R), both spline recursions (k_spline), and the evaluation at interpftpad (i -
 // shftf[i]) divided by sig2 for the outputs i whose klo falls in this workgroup's knots
 // (k_spline_eval): y and y2 stay in LDS.  x_i = interpftpad (i - shftf_i) increases with i, so a
 // workgroup's outputs are the i in [min{i : x_i >= A0}, min{i : x_i >= A1}) (from 0 in the first
@@ -494,23 +497,30 @@ __global__ __launch_bounds__(NT) void k_spline_fused(const cplx<T>* __restrict__
       }
     }
   }
-  // y window on [jlo, Yhi]: from the T-layout, element (r, c) of the block of rows 0..L2-1 and
-  // columns [cA, cA + ncol) holding the samples m in [mlo, mhi], read row by row (contiguous row
-  // segments); sample m is y at j = 2m + 1 (HALF: the odd samples' transform) or at j = m.  HALF:
-  // the even samples y at 2m are the input series R itself.
+  // y window on [jlo, Yhi].  HALF: Zi holds the odd samples y at 2m + 1 in natural order (m) and
+  // the even samples y at 2m are the input series R itself.  Else from the T-layout: element (r, c)
+  // of the block of rows 0..L2-1 and columns [cA, cA + ncol) holding y on [jlo, Yhi], read row by
+  // row (contiguous row segments).
   const int jlo = max(Ylo, 0);
-  {
+  if (HALF) {
+    const cplx<T>* Z = Zi + (size_t)b * zstride;
+    const cplx<T>* Rb = R + (size_t)b * rstride;
+    for (int m = jlo / 2 + tid; m <= (Yhi - 1) / 2; m += NT) {
+      const cplx<T> v = Z[m];
+      Ys[2 * m + 1 - Ylo] = mk<cplx<T>>(v.x * ft, v.y * ft);
+    }
+    for (int m = (jlo + 1) / 2 + tid; m <= Yhi / 2; m += NT) Ys[2 * m - Ylo] = Rb[m];
+  } else {
     const int L2 = 1 << lgL2;
-    const int mlo = HALF ? jlo / 2 : jlo, mhi = HALF ? (Yhi - 1) / 2 : Yhi;
-    const int cA = mlo >> lgL2, ncol = (mhi >> lgL2) - cA + 1, nel = ncol * L2;
+    const int cA = jlo >> lgL2, ncol = (Yhi >> lgL2) - cA + 1, nel = ncol * L2;
     const cplx<T>* Z = Zi + (size_t)b * zstride;
     const int dr = NT / ncol, dc = NT % ncol;
     int r = tid / ncol, c = tid % ncol;
     for (int e0 = 0; e0 < nel; e0 += NT) {
-      const int m = (cA + c) * L2 + r;
-      if (e0 + tid < nel && m >= mlo && m <= mhi) {
+      const int jj = (cA + c) * L2 + r;
+      if (e0 + tid < nel && jj >= jlo && jj <= Yhi) {
         const cplx<T> v = Z[(size_t)r * L1 + cA + c];
-        Ys[(HALF ? 2 * m + 1 : m) - Ylo] = mk<cplx<T>>(v.x * ft, v.y * ft);
+        Ys[jj - Ylo] = mk<cplx<T>>(v.x * ft, v.y * ft);
       }
       c += dc;
       r += dr;
@@ -518,10 +528,6 @@ __global__ __launch_bounds__(NT) void k_spline_fused(const cplx<T>* __restrict__
         c -= ncol;
         r++;
       }
-    }
-    if (HALF) {
-      const cplx<T>* Rb = R + (size_t)b * rstride;
-      for (int m = (jlo + 1) / 2 + tid; m <= Yhi / 2; m += NT) Ys[2 * m - Ylo] = Rb[m];
     }
   }
   // window entries below 0 (first workgroup) and above Yhi: finite values for discarded steps

@@ -518,8 +518,10 @@ __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const doub
   }
 }
 
-// Generic column pass (one array per block): grid (L1/C, nbatch).
-template <typename T, int P2, int DIR, typename Loader>
+// Generic column pass (one array per block): grid (L1/C, nbatch).  OTW: the four-step's output
+// twiddle W_L^{n1 k2} (false: the column transform alone, for a transform whose twiddles the
+// loader applies to its input)
+template <typename T, int P2, int DIR, typename Loader, bool OTW = true>
 __global__ __launch_bounds__(256) void k_fft_col2(Loader ld, cplx<T>* __restrict__ Z, int L1,
                                                   const cplx<T>* __restrict__ twc, const cplx<T>* __restrict__ hi,
                                                   const cplx<T>* __restrict__ lo, int twshift) {
@@ -535,12 +537,12 @@ __global__ __launch_bounds__(256) void k_fft_col2(Loader ld, cplx<T>* __restrict
 #pragma unroll
   for (int r = 0; r < E; r++) x[0][r] = ld(b, n1 + L1 * (g + S::G * r));
   T2 w[E];
-  col_out_twiddles<P2, E>(tw_dir<DIR>(tw_big(hi, lo, (uint32_t)n1 << twshift)), w);
+  if constexpr (OTW) col_out_twiddles<P2, E>(tw_dir<DIR>(tw_big(hi, lo, (uint32_t)n1 << twshift)), w);
   col_fft<T2, P2, E, 1, DIR>(x, lds, twc);
 #pragma unroll
   for (int s = 0; s < E; s++) {
     const int k2 = col_out_k2<P2, E>(s);
-    Z[(size_t)b * L + (size_t)k2 * L1 + n1] = cmul(x[0][s], w[s]);
+    Z[(size_t)b * L + (size_t)k2 * L1 + n1] = OTW ? cmul(x[0][s], w[s]) : x[0][s];
   }
 }
 
@@ -557,6 +559,37 @@ __global__ __launch_bounds__(P1 / 16) void k_fft_row2(cplx<T>* __restrict__ Z, i
   row_fft<T2, P1, 1, DIR>(x, lds, twr);
 #pragma unroll
   for (int m = 0; m < 16; m++) row[threadIdx.x + S::NT * m] = x[0][m];
+}
+
+// Resampling's odd samples, first half (jobcore.c:334-352): on each row k2 of the forward
+// spectrum in T-layout (X[L2 k1 + k2] at k1 of row k2), the forward row transform, then X[k] s_k
+// e^{i pi k / L} with k = L2 k1 + k2, of which the factor e^{i pi k1 / L1} (and the sign s_k = -1 for
+// k >= nyqst) is applied here and e^{i pi k2 / L}, constant along the row, by the column pass's
+// loader (LoadOddTw), then the inverse row transform over k1: the inverse transform's first step,
+// in place.  grid (L2, nbatch), P1/16 threads; shift1 = 20 - log2(2 P1).
+template <typename T, int P1>
+__global__ __launch_bounds__(P1 / 16) void k_fft_row_shift(cplx<T>* __restrict__ Z, int L2, const cplx<T>* __restrict__ twr,
+                                                           const cplx<T>* __restrict__ hi, const cplx<T>* __restrict__ lo,
+                                                           int shift1, int nyqst) {
+  using T2 = cplx<T>;
+  using S = RowShape<P1>;
+  __shared__ T2 lds[S::PADN];
+  const int k2 = blockIdx.x, t = threadIdx.x;
+  T2* row = Z + (size_t)blockIdx.y * P1 * L2 + (size_t)k2 * P1;
+  T2 x[1][16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) x[0][m] = row[t + S::NT * m];
+  row_fft<T2, P1, 1, -1, 2>(x, lds, twr);
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k1 = t + S::NT * m;
+    const T2 v = cmul(x[0][m], conjg(tw_big(hi, lo, (uint32_t)k1 << shift1)));
+    x[0][m] = L2 * k1 + k2 < nyqst ? v : mk<T2>(-v.x, -v.y);
+  }
+  __syncthreads();  // the forward transform's last LDS reads are done
+  row_fft<T2, P1, 1, 1, 2>(x, lds, twr);
+#pragma unroll
+  for (int m = 0; m < 16; m++) row[t + S::NT * m] = x[0][m];
 }
 
 }  // namespace pgw

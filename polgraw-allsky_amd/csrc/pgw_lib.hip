@@ -141,12 +141,12 @@ template <typename T> struct Plan {
 };
 
 // ---------------------------------------------------------- launch helpers
-template <typename T, int DIR, typename Loader>
+template <typename T, int DIR, bool OTW = true, typename Loader>
 void launch_col(const Plan<T>& pl, Loader ld, cplx<T>* Z, int nbatch, const cplx<T>* hi, const cplx<T>* lo,
                 hipStream_t st) {
 #define PGW_COL_CASE(P2)                                                                       \
   case P2:                                                                                     \
-    hipLaunchKernelGGL((k_fft_col2<T, P2, DIR, Loader>), dim3(pl.L1 / col_tile<T, P2>(), nbatch), dim3(256), 0, \
+    hipLaunchKernelGGL((k_fft_col2<T, P2, DIR, Loader, OTW>), dim3(pl.L1 / col_tile<T, P2>(), nbatch), dim3(256), 0, \
                        st, ld, Z, pl.L1, pl.twc.p, hi, lo, pl.twshift);                        \
     break;
   switch (pl.L2) {
@@ -164,6 +164,24 @@ void launch_row(const Plan<T>& pl, cplx<T>* Z, int nbatch, hipStream_t st) {
 #define PGW_ROW_CASE(P1)                                                                                    \
   case P1:                                                                                                  \
     hipLaunchKernelGGL((k_fft_row2<T, P1, DIR>), dim3(pl.L2, nbatch), dim3(P1 / 16), 0, st, Z, pl.L2, pl.twr.p); \
+    break;
+  switch (pl.L1) {
+    PGW_ROW_CASE(1024)
+    PGW_ROW_CASE(2048)
+    PGW_ROW_CASE(4096)
+  }
+#undef PGW_ROW_CASE
+}
+
+// forward row transform, phase shift and sign, inverse row transform on each row (k_fft_row_shift)
+template <typename T>
+void launch_row_shift(const Plan<T>& pl, cplx<T>* Z, int nbatch, const cplx<T>* hi, const cplx<T>* lo, int nyqst,
+                      hipStream_t st) {
+  const int shift1 = 19 - ilog2(pl.L1);  // e^{i pi k1 / L1} = conj(W_{2^20}^{k1 << (20 - log2(2 L1))})
+#define PGW_ROW_CASE(P1)                                                                                    \
+  case P1:                                                                                                  \
+    hipLaunchKernelGGL((k_fft_row_shift<T, P1>), dim3(pl.L2, nbatch), dim3(P1 / 16), 0, st, Z, pl.L2, pl.twr.p, hi, \
+                       lo, shift1, nyqst);                                                                  \
     break;
   switch (pl.L1) {
     PGW_ROW_CASE(1024)
@@ -820,14 +838,15 @@ template <typename T> struct Engine final : pgw_handle {
     // resampling: forward FFT (nfft), zero-insert, backward FFT (Ninterp), scale (jobcore.c:319-362)
     LoadPlain<T2> lp{R.p, S.nfft, S.nfft};
     launch_col<T, -1>(*g0.pf, lp, Zf.p, nb * G, hi.p, lo.p, stream);
-    launch_row<T, -1>(*g0.pf, Zf.p, nb * G, stream);
     const T ft = (T)((double)S.interpftpad / S.Ninterp);
     if (spl_fuse) {
-      // odd samples by an nfft-point inverse transform (even ones: R itself); then untranspose,
-      // spline and evaluation in one kernel (y and y2 stay in LDS)
-      LoadHalfShift<T2> lh{Zf.p, hi.p, lo.p, S.nfft, S.nfft / 2 + 1, g0.pf->L1, g0.pf->L2, g0.pi->twshift};
-      launch_col<T, +1>(*g0.pf, lh, Zi.p, nb * G, hi.p, lo.p, stream);
-      launch_row<T, +1>(*g0.pf, Zi.p, nb * G, stream);
+      // odd samples by an nfft-point inverse transform (even ones: R itself): forward row
+      // transforms, phase shift and inverse row transforms in one pass, then the column pass with
+      // input twiddles (natural-order output); then untranspose, spline and evaluation in one kernel
+      // (y and y2 stay in LDS)
+      launch_row_shift<T>(*g0.pf, Zf.p, nb * G, hi.p, lo.p, S.nfft / 2 + 1, stream);
+      LoadOddTw<T2> lw{Zf.p, hi.p, lo.p, S.nfft, ilog2(g0.pf->L1), g0.pi->twshift};
+      launch_col<T, +1, false>(*g0.pf, lw, Zi.p, nb * G, hi.p, lo.p, stream);
       int lg = 0;
       while ((1 << lg) < g0.pf->L2) lg++;
       int* err = zero ? zero + 1 : nullptr;
@@ -841,6 +860,7 @@ template <typename T> struct Engine final : pgw_handle {
 #undef PGW_SPLINE_F
       return hipGetLastError() == hipSuccess ? PGW_OK : PGW_ERR_DEVICE;
     }
+    launch_row<T, -1>(*g0.pf, Zf.p, nb * G, stream);
     LoadZeroInsert<T2> lz{Zf.p, S.nfft, S.Ninterp, S.nfft / 2 + 1, g0.pf->L1, g0.pf->L2};
     launch_col<T, +1>(*g0.pi, lz, Zi.p, nb * G, hi.p, lo.p, stream);
     launch_row<T, +1>(*g0.pi, Zi.p, nb * G, stream);
