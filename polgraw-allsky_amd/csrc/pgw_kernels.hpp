@@ -757,11 +757,15 @@ constexpr int THR_LDS = THR_MAXP + 64;  // doubles of LDS per threshold wave: bl
 // One wave (lane = 0..63) thresholds FStat block `part` of template t; mu: THR_LDS doubles of LDS
 // owned by this wave.  TJ, QB: loads in flight per batch in the flagged path (bins, block means).
 // SC: F and the partials read at device scope (written by other workgroups of the same launch).
-template <typename T, int TJ = (sizeof(T) == 4 ? 32 : 16), int QB = 16, bool SC = false>
+// HITF: Fh (PGW_NAV_DEV elements of LDS owned by this wave) keeps the F of the pre-selected bins,
+// so that the exact tests and the run walks inside the block read LDS (a bin that is not
+// pre-selected is not above trl: the run ends there) and only a run continuing into the next
+// block loads from F.
+template <typename T, int TJ = (sizeof(T) == 4 ? 32 : 16), int QB = 16, bool SC = false, bool HITF = false>
 __device__ __forceinline__ void thr_wave(const ThrParams& p, int part, int t, int lane, double* mu,
                                          const T* __restrict__ Ft, const T* __restrict__ psum,
                                          const T* __restrict__ pmax, double* __restrict__ mu_out,
-                                         Cand* __restrict__ cand, int* __restrict__ ncand) {
+                                         Cand* __restrict__ cand, int* __restrict__ ncand, T* Fh = nullptr) {
   const size_t L = (size_t)p.L1 * p.L2;
   const T* F = Ft + (size_t)t * L;
   const T* ps0 = psum + (size_t)t * p.nparts * p.L2;
@@ -820,7 +824,10 @@ __device__ __forceinline__ void thr_wave(const ThrParams& p, int part, int t, in
       }
 #pragma unroll
       for (int j = 0; j < TJ; j++)
-        if (o[j] >= 0 && (double)f[j] > pre) atomicOr(&bm[o[j] & 63], 1ull << (o[j] >> 6));
+        if (o[j] >= 0 && (double)f[j] > pre) {
+          atomicOr(&bm[o[j] & 63], 1ull << (o[j] >> 6));
+          if constexpr (HITF) Fh[o[j]] = f[j];
+        }
     }
   }
   wave_lds_sync();
@@ -856,24 +863,33 @@ __device__ __forceinline__ void thr_wave(const ThrParams& p, int part, int t, in
     const int j = __builtin_ctzll(hits);
     hits &= hits - 1;
     const int k = lok + lane + 64 * j;
-    const double fn = at(k) / mu[part];
+    const double fn = (HITF ? (double)Fh[k - lok] : at(k)) / mu[part];
     if (!(fn > p.trl)) continue;
     bool start = (k == p.nmin);
     if (!start && (k == lok || (prevm >> j) & 1)) {  // previous bin in another part, or pre-selected
       const int kp = k - 1;
       const int pp = min((kp - p.nmin) / PGW_NAV_DEV, p.nparts - 1);
-      start = !(at(kp) / mu[pp] > p.trl);
+      start = !(((HITF && k != lok) ? (double)Fh[kp - lok] : at(kp)) / mu[pp] > p.trl);
     } else if (!start) {
       start = true;
     }
     if (!start) continue;
     int ii = k;
     double Fc = fn;
-    const bool ends = k + 1 < hik && !((nextm >> j) & 1);  // next bin of this part not pre-selected
+    bool ends = k + 1 < hik && !((nextm >> j) & 1);  // next bin of this part not pre-selected
+    int kw = k + 1;  // first bin of the run walk below
+    if constexpr (HITF) {  // the run inside this part, from LDS
+      for (; !ends && kw < hik; kw++) {
+        const int o = kw - lok;
+        const double v = (double)Fh[o] / mu[part];
+        if (!((bm[o & 63] >> (o >> 6)) & 1ull) || !(v > p.trl)) ends = true;
+        else if (v >= Fc) { ii = kw; Fc = v; }
+      }
+    }
     // the run's later bins RB at a time, their loads in flight together (addresses clamped below
     // nmax; bins past the run's end are loaded and ignored)
     constexpr int RB = TJ < 8 ? TJ : 8;
-    for (int kk0 = k + 1; !ends && kk0 < p.nmax; kk0 += RB) {
+    for (int kk0 = kw; !ends && kk0 < p.nmax; kk0 += RB) {
       double v[RB];
 #pragma unroll
       for (int q = 0; q < RB; q++) {
@@ -906,7 +922,13 @@ __global__ __launch_bounds__(64) void k_threshold(ThrParams p, const T* __restri
                                                   const T* __restrict__ pmax, double* __restrict__ mu_out,
                                                   Cand* __restrict__ cand, int* __restrict__ ncand) {
   __shared__ double mu[THR_LDS];
-  thr_wave<T>(p, blockIdx.x, blockIdx.y, threadIdx.x, mu, Ft, psum, pmax, mu_out, cand, ncand);
+  if constexpr (sizeof(T) == 4) {  // (FP64: 32 KB of LDS per wave cost more occupancy than it saved)
+    __shared__ T Fh[PGW_NAV_DEV];
+    thr_wave<T, 32, 16, false, true>(p, blockIdx.x, blockIdx.y, threadIdx.x, mu, Ft, psum, pmax, mu_out, cand, ncand,
+                                     Fh);
+  } else {
+    thr_wave<T>(p, blockIdx.x, blockIdx.y, threadIdx.x, mu, Ft, psum, pmax, mu_out, cand, ncand);
+  }
 }
 
 // A threshold launch folded into the next column-pass launch: the first `rows` grid rows of that
