@@ -45,7 +45,6 @@ from polgraw_allsky_amd import abi, pgw, synth  # noqa: E402
 
 METRIC = "F-stat templates/s on 1 MI355X (FP32-batched vs FP64 plain); max rel ΔF vs FP64"
 HBM_PEAK_GBS = 8000.0
-HBM_MEASURED = None  # {"copy", "read", "write"}: GB/s of streaming kernels in this run (pgw_hbm_probe)
 # the template kernels of every KTIME_EVERY-th launch of the timed steps are bracketed by HIP
 # events (each event record costs the following kernels a little; sampling keeps that < 1 %)
 KTIME_EVERY = 4
@@ -158,6 +157,23 @@ def alg_bytes(seg, p):
     return nifo * N * (4 * p + 8), 4 * L * p, 4 * L * p + M * p
 
 
+def annotate_stream(obj, hbm):
+    """Adds to every template-kernel roofline in obj the streaming bandwidth measured in this run
+    (pgw_hbm_probe) and the kernel's fraction of it: the column pass mostly writes, the row pass
+    mostly reads."""
+    if isinstance(obj, dict):
+        if obj.get("bound") == "hbm" and "achieved" in obj and "hbm_measured_GBps" in obj:
+            against = "write" if obj["kernel"] == "k_spin_col2" else "read"
+            obj["hbm_measured_GBps"] = hbm
+            obj["frac_of_measured_stream"] = {"against": against, "GBps": hbm[against],
+                                              "frac": round(obj["achieved"] / hbm[against], 4)}
+        for v in obj.values():
+            annotate_stream(v, hbm)
+    elif isinstance(obj, list):
+        for v in obj:
+            annotate_stream(v, hbm)
+
+
 def roofline(kt, segw, prec_bytes, workload, tuned=None, spins=512):
     """The dominant template kernel against the HBM roofline.  kt: the library's HIP-event kernel
     timing over the timed steps (pgw_kernel_times); segw: [(segment, templates)] of those steps
@@ -189,16 +205,9 @@ def roofline(kt, segw, prec_bytes, workload, tuned=None, spins=512):
     ntl = n / nl
     other = "row" if k == "col" else "col"
     ach_other = (bytes_row if k == "col" else bytes_col) / (kt[other + "_ms"] * 1e-3) / 1e9
-    # against what simple streaming kernels reach on this device in the same run (the column pass
-    # mostly writes, the row pass mostly reads)
-    stream = None
-    if HBM_MEASURED:
-        against = "write" if k == "col" else "read"
-        stream = {"against": against, "GBps": HBM_MEASURED[against],
-                  "frac": round(achieved / HBM_MEASURED[against], 4)}
     return dict(bound="hbm", kernel=kname, achieved=round(achieved, 1),
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
-                hbm_measured_GBps=HBM_MEASURED, frac_of_measured_stream=stream,
+                hbm_measured_GBps=None, frac_of_measured_stream=None,  # set by annotate_stream
                 traffic=(int(tr * ntl) if tr else None), traffic_unit="bytes per launch",
                 # counter-measured HBM bytes over this run's kernel duration
                 traffic_GBps=(round(tr * n / (kt[k + "_ms"] * 1e-3) / 1e9, 1) if tr else None),
@@ -260,13 +269,14 @@ def cpu_baseline(seg, rng, opts, sky=None, budget_s=10.0):
                 threads_env=threads_env(), host_cpus=len(os.sched_getaffinity(0)))
 
 
-def tune_handle(args, h, segi, seg, opts):
-    """The library's autotuner on the injected sky point (setup, before any timed step): kernel
-    variants and templates per launch for this handle.  Returns the choice (None with --no-tune)."""
+def tune_handle(args, h, segi, seg, opts, rng):
+    """The library's autotuner on the injected sky point (setup, before any timed step), over the
+    templates the workload's spindown range forms there: kernel variants and templates per launch
+    for this handle.  Returns the choice (None with --no-tune)."""
     if args.no_tune:
         return None
     pm, mm, nn = seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2])
-    choice = h.autotune(segi, pm, mm, nn, opts)
+    choice = h.autotune(segi, pm, mm, nn, opts, rng)
     choice["variants"] = h.tune_string()
     print(f"autotune: {choice}", file=sys.stderr, flush=True)
     return choice
@@ -298,7 +308,7 @@ def bench_templates(args, name, steps=None, warmup=None, per_step=None, cpu=True
     print(f"workload {name}: N={s['N']} nfft={s['nfft']} fftpad={s['fftpad']} L={s['nfftf']} "
           f"sky={len(sky)} spindowns={nspin}", file=sys.stderr, flush=True)
     h32 = pgw.Handle(seg, abi.PREC_FP32)
-    tune = {"fp32": tune_handle(args, h32, 0, seg, opts)}
+    tune = {"fp32": tune_handle(args, h32, 0, seg, opts, rng)}
     n32, t32, g32, r32, kt32 = run_path(h32, sky, rng, opts, steps, warmup, per_step)
     v32 = n32 / t32
     print(f"fp32: {n32} templates in {t32:.3f} s -> {v32:.0f} templates/s (gpu {g32:.1f} ms)",
@@ -306,7 +316,7 @@ def bench_templates(args, name, steps=None, warmup=None, per_step=None, cpu=True
     out_fp64, acc = None, None
     if not args.no_fp64:
         h64 = pgw.Handle(seg, abi.PREC_FP64)
-        tune["fp64"] = tune_handle(args, h64, 0, seg, opts)
+        tune["fp64"] = tune_handle(args, h64, 0, seg, opts, rng)
         n64, t64, g64, r64, kt64 = run_path(h64, sky, rng, opts, steps, warmup, per_step)
         v64 = n64 / t64
         print(f"fp64: {n64} templates in {t64:.3f} s -> {v64:.0f} templates/s (gpu {g64:.1f} ms)",
@@ -359,7 +369,8 @@ def bench_batching(args):
 
     def run(prec):
         h = pgw.Handle(segs, prec)
-        tune["fp32" if prec == abi.PREC_FP32 else "fp64"] = tune_handle(args, h, ktune, segs[ktune], opts)
+        tune["fp32" if prec == abi.PREC_FP32 else "fp64"] = tune_handle(args, h, ktune, segs[ktune], opts,
+                                                                       abi.make_range(segs[ktune]))
         chunks = [work[(i * per_step) % len(work):(i * per_step) % len(work) + per_step]
                   for i in range(args.warmup + args.steps)]
         # each segment's spindowns come from its own range; the call takes one range, so the
@@ -562,7 +573,7 @@ def bench_dropin(args, nsky=64):
     out = {}
     for name, prec in (("fp64", abi.PREC_FP64), ("fp32", abi.PREC_FP32)):
         h = pgw.Handle(seg, prec)
-        tune_handle(args, h, 0, seg, opts)
+        tune_handle(args, h, 0, seg, opts, rng)
         for pm, mm, nn in sky[:2]:
             h.job_core(pm, mm, nn, rng, opts)
         t0 = time.perf_counter()
@@ -633,11 +644,6 @@ def main():
               "(sky points would shard across GPUs, one process and handle each, with no collectives)",
               file=sys.stderr)
         sys.exit(2)
-    global HBM_MEASURED
-    # streaming copy / read / write kernels over 1 GiB: the bandwidth this device reaches (the
-    # roofline's peak stays the published 8 TB/s)
-    HBM_MEASURED = pgw.hbm_probe(1 << 30, reps=5)
-    print(f"hbm probe (GB/s): {HBM_MEASURED}", file=sys.stderr, flush=True)
     if args.copy_calib:
         ms = pgw.copy_bench(256 << 20, reps=3)
         print(f"copy 256 MiB: {ms:.3f} ms ({2 * 256 * 1.048576 / ms:.0f} GB/s)", file=sys.stderr, flush=True)
@@ -649,6 +655,12 @@ def main():
         line = bench_differentiation(args)
     if args.workload == "batching" and not args.no_secondary:
         line["other_workloads"] = secondary(args)
+    # streaming copy / read / write kernels over 1 GiB: the bandwidth this device reaches (the
+    # roofline's peak stays the published 8 TB/s).  Run after the timed work: the small plain
+    # workload measured up to 20 % slower in a process that had run them first.
+    hbm = pgw.hbm_probe(1 << 30, reps=5)
+    print(f"hbm probe (GB/s): {hbm}", file=sys.stderr, flush=True)
+    annotate_stream(line, hbm)
     print(json.dumps(line), flush=True)
 
 

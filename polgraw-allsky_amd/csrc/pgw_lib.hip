@@ -410,7 +410,8 @@ struct pgw_handle {
   virtual int debug_resampled(int pm, int mm, int nn, int det, double* a, double* b) = 0;
   virtual int time_stages(int seg, int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps,
                           double* ms) = 0;
-  virtual int autotune(int seg, int pm, int mm, int nn, const pgw_opts* o, int* tb, int* row_na, int* col_seq) = 0;
+  virtual int autotune(int seg, int pm, int mm, int nn, const pgw_opts* o, const pgw_range* r, int* tb, int* row_na,
+                       int* col_seq) = 0;
   virtual int tune_string(char* buf, int len) const = 0;
   virtual int kernel_times(int enable, double* out) = 0;
   virtual int fstatnet(int seg, const double* pts, int n, double* out, double* grad) = 0;
@@ -1483,7 +1484,10 @@ template <typename T> struct Engine final : pgw_handle {
     return n >= 0 && n < len ? PGW_OK : PGW_ERR_ARG;
   }
 
-  int autotune(int sgi, int pm, int mm, int nn, const pgw_opts* o, int* tb_out, int* na_out, int* seq_out) override {
+  // r: the caller's spindown range (each work item's templates as a search would form them), or
+  // nullptr (512 spindowns per work item)
+  int autotune(int sgi, int pm, int mm, int nn, const pgw_opts* o, const pgw_range* r, int* tb_out, int* na_out,
+               int* seq_out) override {
     if (sgi < 0 || sgi >= (int)segs.size()) return PGW_ERR_ARG;
     const int tbs[3] = {64, 128, 256};
     // row pass variants: 16 elements per thread with the spectra in turn (na 1), 8 elements per
@@ -1508,11 +1512,14 @@ template <typename T> struct Engine final : pgw_handle {
     oq.trl = 1e300;
     pgw_range rq;
     memset(&rq, 0, sizeof(rq));
+    if (r) rq = *r;
     const int kt_saved = ktime;
     ktime = 0;
     auto per_template = [&](double& ms) {
       int sgnlc = 0, fn = 0;
-      const int st = run(work.data(), NI, &rq, &oq, &sgnlc, nullptr, 0, nullptr, &fn, ssoff.data(), sslist.data(), false);
+      pgw_range rr = rq;  // run may update the range it is given
+      const int st = r ? run(work.data(), NI, &rr, &oq, &sgnlc, nullptr, 0, nullptr, &fn, nullptr, nullptr, false)
+                       : run(work.data(), NI, &rr, &oq, &sgnlc, nullptr, 0, nullptr, &fn, ssoff.data(), sslist.data(), false);
       if (st) return st;
       if (fn == 0) return PGW_ERR_ARG;  // outside the sky disk
       ms = last_ms / fn;
@@ -1708,7 +1715,13 @@ int pgw_job_batch_multi(pgw_handle* h, const int* work, int nwork, pgw_range* s_
 int pgw_autotune(pgw_handle* h, int seg, int pm, int mm, int nn, const pgw_opts* opts, int* tb, int* row_na,
                  int* col_seq) {
   if (!h || !opts) return PGW_ERR_ARG;
-  return h->autotune(seg, pm, mm, nn, opts, tb, row_na, col_seq);
+  return h->autotune(seg, pm, mm, nn, opts, nullptr, tb, row_na, col_seq);
+}
+
+int pgw_autotune_range(pgw_handle* h, int seg, int pm, int mm, int nn, const pgw_opts* opts,
+                       const pgw_range* s_range, int* tb, int* row_na, int* col_seq) {
+  if (!h || !opts || !s_range) return PGW_ERR_ARG;
+  return h->autotune(seg, pm, mm, nn, opts, s_range, tb, row_na, col_seq);
 }
 
 int pgw_tune_string(const pgw_handle* h, char* buf, int len) {
@@ -1799,13 +1812,15 @@ int pgw_hbm_probe(int kind, size_t bytes, int reps, double* ms) {
   void *a = nullptr, *b = nullptr;
   float* o = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipStream_t sq = nullptr;  // a stream of its own (the null stream stays unused)
   int st = PGW_OK;
   auto fail = [&](int s) {
     if (!st) st = s;
   };
   if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess || hipMalloc(&o, 64) != hipSuccess)
     fail(PGW_ERR_NOMEM);
-  if (!st && (hipMemset(a, 0, n * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+  if (!st && (hipStreamCreateWithFlags(&sq, hipStreamNonBlocking) != hipSuccess ||
+              hipMemsetAsync(a, 0, n * 16, sq) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
               hipEventCreate(&e1) != hipSuccess))
     fail(PGW_ERR_DEVICE);
   double best = 1e30;
@@ -1814,15 +1829,15 @@ int pgw_hbm_probe(int kind, size_t bytes, int reps, double* ms) {
   for (int grid : {2048, 8192, 32768}) {
     if (st) break;
     auto go = [&]() {
-      if (kind == 0) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(NT), 0, 0, (const double4*)a, (double4*)b, n / 2);
-      else if (kind == 1) hipLaunchKernelGGL(k_read_probe, dim3(grid), dim3(NT), 0, 0, (const float4*)a, n, o);
-      else hipLaunchKernelGGL(k_write_probe, dim3(grid), dim3(NT), 0, 0, (float4*)b, n);
+      if (kind == 0) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(NT), 0, sq, (const double4*)a, (double4*)b, n / 2);
+      else if (kind == 1) hipLaunchKernelGGL(k_read_probe, dim3(grid), dim3(NT), 0, sq, (const float4*)a, n, o);
+      else hipLaunchKernelGGL(k_write_probe, dim3(grid), dim3(NT), 0, sq, (float4*)b, n);
     };
     go();  // warm-up
-    if (hipEventRecord(e0, 0) != hipSuccess) { fail(PGW_ERR_DEVICE); break; }
+    if (hipEventRecord(e0, sq) != hipSuccess) { fail(PGW_ERR_DEVICE); break; }
     for (int r = 0; r < reps; r++) go();
     float t = 0.f;
-    if (hipEventRecord(e1, 0) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+    if (hipEventRecord(e1, sq) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
         hipEventElapsedTime(&t, e0, e1) != hipSuccess)
       fail(PGW_ERR_DEVICE);
     else
@@ -1830,6 +1845,10 @@ int pgw_hbm_probe(int kind, size_t bytes, int reps, double* ms) {
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
+  if (sq) {
+    (void)hipStreamSynchronize(sq);
+    (void)hipStreamDestroy(sq);
+  }
   (void)hipFree(a);
   (void)hipFree(b);
   (void)hipFree(o);

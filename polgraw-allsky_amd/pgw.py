@@ -53,6 +53,8 @@ def lib():
                                           _ip, _dp, C.c_int, _ip, _ip]
         L.pgw_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts),
                                    _ip, _ip, _ip]
+        L.pgw_autotune_range.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts),
+                                         C.POINTER(abi.Range), _ip, _ip, _ip]
         L.pgw_tune_string.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
         L.pgw_fstatnet.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp]
         L.pgw_fstatnet_multi.argtypes = [C.c_void_p, _ip, _dp, C.c_int, _dp, _dp]
@@ -80,7 +82,7 @@ def lib():
 
 EXPORTED = ["pgw_search_settings", "pgw_rogcvir", "pgw_sig2", "pgw_init", "pgw_job_core", "pgw_job_batch",
             "pgw_job_core_f", "pgw_job_batch_f",
-            "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_tune_string", "pgw_fstatnet", "pgw_fstatnet_multi", "pgw_fstatnet_noalloc",
+            "pgw_init_multi", "pgw_job_batch_multi", "pgw_autotune", "pgw_autotune_range", "pgw_tune_string", "pgw_fstatnet", "pgw_fstatnet_multi", "pgw_fstatnet_noalloc",
             "pgw_get_settings",
             "pgw_search", "pgw_read_checkpoint", "pgw_job_spotlight",
             "pgw_debug_fstat", "pgw_debug_resampled", "pgw_time_stages", "pgw_copy_bench", "pgw_hbm_probe",
@@ -282,9 +284,15 @@ class Handle:
             raise PGWError(st, "pgw_search")
         return fn.value
 
-    def autotune(self, seg, pm, mm, nn, opts):
+    def autotune(self, seg, pm, mm, nn, opts, rng=None):
+        """pgw_autotune, or with a spindown range rng pgw_autotune_range (the templates a search
+        with that range forms at the sky point)."""
         tb, na, seq = C.c_int(), C.c_int(), C.c_int()
-        st = lib().pgw_autotune(self.h, seg, pm, mm, nn, C.byref(opts), C.byref(tb), C.byref(na), C.byref(seq))
+        if rng is None:
+            st = lib().pgw_autotune(self.h, seg, pm, mm, nn, C.byref(opts), C.byref(tb), C.byref(na), C.byref(seq))
+        else:
+            st = lib().pgw_autotune_range(self.h, seg, pm, mm, nn, C.byref(opts), C.byref(rng), C.byref(tb),
+                                          C.byref(na), C.byref(seq))
         if st != 0:
             raise PGWError(st, "pgw_autotune")
         return dict(tb=tb.value, row_na=na.value, col_seq=seq.value)
