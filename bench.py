@@ -269,14 +269,14 @@ def cpu_baseline(seg, rng, opts, sky=None, budget_s=10.0):
                 threads_env=threads_env(), host_cpus=len(os.sched_getaffinity(0)))
 
 
-def tune_handle(args, h, segi, seg, opts, rng):
-    """The library's autotuner on the injected sky point (setup, before any timed step), over the
-    templates the workload's spindown range forms there: kernel variants and templates per launch
-    for this handle.  Returns the choice (None with --no-tune)."""
+def tune_handle(args, h, segi, seg, opts, rng, nsky):
+    """The library's autotuner on the injected sky point (setup, before any timed step), over calls
+    of nsky copies of it with the templates the workload's spindown range forms there: kernel
+    variants and templates per launch for this handle.  Returns the choice (None with --no-tune)."""
     if args.no_tune:
         return None
     pm, mm, nn = seg.injection["pm"], int(seg.injection["grid"][3]), int(seg.injection["grid"][2])
-    choice = h.autotune(segi, pm, mm, nn, opts, rng)
+    choice = h.autotune(segi, pm, mm, nn, opts, rng, min(nsky, 32))
     choice["variants"] = h.tune_string()
     print(f"autotune: {choice}", file=sys.stderr, flush=True)
     return choice
@@ -308,7 +308,7 @@ def bench_templates(args, name, steps=None, warmup=None, per_step=None, cpu=True
     print(f"workload {name}: N={s['N']} nfft={s['nfft']} fftpad={s['fftpad']} L={s['nfftf']} "
           f"sky={len(sky)} spindowns={nspin}", file=sys.stderr, flush=True)
     h32 = pgw.Handle(seg, abi.PREC_FP32)
-    tune = {"fp32": tune_handle(args, h32, 0, seg, opts, rng)}
+    tune = {"fp32": tune_handle(args, h32, 0, seg, opts, rng, per_step)}
     n32, t32, g32, r32, kt32 = run_path(h32, sky, rng, opts, steps, warmup, per_step)
     v32 = n32 / t32
     print(f"fp32: {n32} templates in {t32:.3f} s -> {v32:.0f} templates/s (gpu {g32:.1f} ms)",
@@ -316,7 +316,7 @@ def bench_templates(args, name, steps=None, warmup=None, per_step=None, cpu=True
     out_fp64, acc = None, None
     if not args.no_fp64:
         h64 = pgw.Handle(seg, abi.PREC_FP64)
-        tune["fp64"] = tune_handle(args, h64, 0, seg, opts, rng)
+        tune["fp64"] = tune_handle(args, h64, 0, seg, opts, rng, per_step)
         n64, t64, g64, r64, kt64 = run_path(h64, sky, rng, opts, steps, warmup, per_step)
         v64 = n64 / t64
         print(f"fp64: {n64} templates in {t64:.3f} s -> {v64:.0f} templates/s (gpu {g64:.1f} ms)",
@@ -370,7 +370,7 @@ def bench_batching(args):
     def run(prec):
         h = pgw.Handle(segs, prec)
         tune["fp32" if prec == abi.PREC_FP32 else "fp64"] = tune_handle(args, h, ktune, segs[ktune], opts,
-                                                                       abi.make_range(segs[ktune]))
+                                                                       abi.make_range(segs[ktune]), per_step)
         chunks = [work[(i * per_step) % len(work):(i * per_step) % len(work) + per_step]
                   for i in range(args.warmup + args.steps)]
         # each segment's spindowns come from its own range; the call takes one range, so the
@@ -573,7 +573,7 @@ def bench_dropin(args, nsky=64):
     out = {}
     for name, prec in (("fp64", abi.PREC_FP64), ("fp32", abi.PREC_FP32)):
         h = pgw.Handle(seg, prec)
-        tune_handle(args, h, 0, seg, opts, rng)
+        tune_handle(args, h, 0, seg, opts, rng, 1)
         for pm, mm, nn in sky[:2]:
             h.job_core(pm, mm, nn, rng, opts)
         t0 = time.perf_counter()

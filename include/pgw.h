@@ -221,13 +221,14 @@ int pgw_fstatnet_noalloc(pgw_handle *h, int seg, const double *pts, int npts, do
 int pgw_autotune(pgw_handle *h, int seg, int pm, int mm, int nn, const pgw_opts *opts, int *tb,
                  int *row_na, int *col_seq);
 
-/* pgw_autotune over the templates a search with s_range forms at the sky point (no counterpart
- * in the reference): each of its four work items takes the spindowns that s_range (and
- * opts->use_range) give there, as pgw_job_batch would, instead of 512.  Small searches (few
- * spindowns per sky point) favour other variants than large ones: their spectra can stay in the
- * device's last-level cache between the passes. */
+/* pgw_autotune over calls shaped like the caller's (no counterpart in the reference): each timed
+ * call holds nsky work items (1..64, copies of the sky point), each with the spindowns that
+ * s_range (and opts->use_range) give there, as pgw_job_batch would, instead of four items of 512,
+ * and the searches use opts->trl (candidates and records formed as in the caller's search).
+ * Small calls favour other variants than large ones (their spectra can stay in the device's
+ * last-level cache between the passes; their launches have fewer neighbours to overlap with). */
 int pgw_autotune_range(pgw_handle *h, int seg, int pm, int mm, int nn, const pgw_opts *opts,
-                       const pgw_range *s_range, int *tb, int *row_na, int *col_seq);
+                       const pgw_range *s_range, int nsky, int *tb, int *row_na, int *col_seq);
 
 /* The handle's kernel variants (no counterpart in the reference) as a PGW_TUNE string, e.g.
  * "tb=128,row_na=1,row8=0,col_seq=1,col_lf=0,col_nt=0,row_nt=1,f_nt=1,rev=1,half=1,spl_k=4,

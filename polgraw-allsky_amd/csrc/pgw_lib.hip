@@ -410,8 +410,8 @@ struct pgw_handle {
   virtual int debug_resampled(int pm, int mm, int nn, int det, double* a, double* b) = 0;
   virtual int time_stages(int seg, int pm, int mm, int nn, int s0, int nt, const pgw_opts* o, int reps,
                           double* ms) = 0;
-  virtual int autotune(int seg, int pm, int mm, int nn, const pgw_opts* o, const pgw_range* r, int* tb, int* row_na,
-                       int* col_seq) = 0;
+  virtual int autotune(int seg, int pm, int mm, int nn, const pgw_opts* o, const pgw_range* r, int nsky, int* tb,
+                       int* row_na, int* col_seq) = 0;
   virtual int tune_string(char* buf, int len) const = 0;
   virtual int kernel_times(int enable, double* out) = 0;
   virtual int fstatnet(int seg, const double* pts, int n, double* out, double* grad) = 0;
@@ -1485,9 +1485,9 @@ template <typename T> struct Engine final : pgw_handle {
   }
 
   // r: the caller's spindown range (each work item's templates as a search would form them), or
-  // nullptr (512 spindowns per work item)
-  int autotune(int sgi, int pm, int mm, int nn, const pgw_opts* o, const pgw_range* r, int* tb_out, int* na_out,
-               int* seq_out) override {
+  // nullptr (512 spindowns per work item); nsky: work items (copies of the sky point) per timed call
+  int autotune(int sgi, int pm, int mm, int nn, const pgw_opts* o, const pgw_range* r, int nsky, int* tb_out,
+               int* na_out, int* seq_out) override {
     if (sgi < 0 || sgi >= (int)segs.size()) return PGW_ERR_ARG;
     const int tbs[3] = {64, 128, 256};
     // row pass variants: 16 elements per thread with the spectra in turn (na 1), 8 elements per
@@ -1497,7 +1497,8 @@ template <typename T> struct Engine final : pgw_handle {
     // block-wide, are slower than 3 and 4 at every measured shape; PGW_TUNE still selects them.)
     const int rv_na[4] = {1, 1, 1, 2}, rv_r8[4] = {0, 4, 3, 0};
     const int nrv = std::is_same<T, double>::value ? 4 : 1;
-    constexpr int NI = 4, NS = 512;
+    const int NI = nsky;
+    constexpr int NS = 512;
     std::vector<int> work(4 * NI), ssoff(NI + 1), sslist((size_t)NI * NS);
     for (int i = 0; i < NI; i++) {
       work[4 * i] = sgi;
@@ -1508,8 +1509,12 @@ template <typename T> struct Engine final : pgw_handle {
       for (int j = 0; j < NS; j++) sslist[(size_t)i * NS + j] = j;
     }
     ssoff[NI] = NI * NS;
+    // with the caller's range the searches keep the caller's threshold, so that the candidates'
+    // cost (the threshold's run walks, the records) is timed as well; else no candidates
     pgw_opts oq = *o;
-    oq.trl = 1e300;
+    if (!r) oq.trl = 1e300;
+    constexpr int kRecCap = 1 << 16;
+    std::vector<double> rec(r ? (size_t)kRecCap * PGW_NPAR : 0);
     pgw_range rq;
     memset(&rq, 0, sizeof(rq));
     if (r) rq = *r;
@@ -1518,8 +1523,9 @@ template <typename T> struct Engine final : pgw_handle {
     auto per_template = [&](double& ms) {
       int sgnlc = 0, fn = 0;
       pgw_range rr = rq;  // run may update the range it is given
-      const int st = r ? run(work.data(), NI, &rr, &oq, &sgnlc, nullptr, 0, nullptr, &fn, nullptr, nullptr, false)
-                       : run(work.data(), NI, &rr, &oq, &sgnlc, nullptr, 0, nullptr, &fn, ssoff.data(), sslist.data(), false);
+      int st = r ? run(work.data(), NI, &rr, &oq, &sgnlc, rec.data(), kRecCap, nullptr, &fn, nullptr, nullptr, false)
+                 : run(work.data(), NI, &rr, &oq, &sgnlc, nullptr, 0, nullptr, &fn, ssoff.data(), sslist.data(), false);
+      if (st == PGW_ERR_CAPACITY) st = PGW_OK;  // more records than the buffer holds: timed all the same
       if (st) return st;
       if (fn == 0) return PGW_ERR_ARG;  // outside the sky disk
       ms = last_ms / fn;
@@ -1715,13 +1721,13 @@ int pgw_job_batch_multi(pgw_handle* h, const int* work, int nwork, pgw_range* s_
 int pgw_autotune(pgw_handle* h, int seg, int pm, int mm, int nn, const pgw_opts* opts, int* tb, int* row_na,
                  int* col_seq) {
   if (!h || !opts) return PGW_ERR_ARG;
-  return h->autotune(seg, pm, mm, nn, opts, nullptr, tb, row_na, col_seq);
+  return h->autotune(seg, pm, mm, nn, opts, nullptr, 4, tb, row_na, col_seq);
 }
 
 int pgw_autotune_range(pgw_handle* h, int seg, int pm, int mm, int nn, const pgw_opts* opts,
-                       const pgw_range* s_range, int* tb, int* row_na, int* col_seq) {
-  if (!h || !opts || !s_range) return PGW_ERR_ARG;
-  return h->autotune(seg, pm, mm, nn, opts, s_range, tb, row_na, col_seq);
+                       const pgw_range* s_range, int nsky, int* tb, int* row_na, int* col_seq) {
+  if (!h || !opts || !s_range || nsky < 1 || nsky > 64) return PGW_ERR_ARG;
+  return h->autotune(seg, pm, mm, nn, opts, s_range, nsky, tb, row_na, col_seq);
 }
 
 int pgw_tune_string(const pgw_handle* h, char* buf, int len) {

@@ -54,7 +54,7 @@ def lib():
         L.pgw_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts),
                                    _ip, _ip, _ip]
         L.pgw_autotune_range.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.Opts),
-                                         C.POINTER(abi.Range), _ip, _ip, _ip]
+                                         C.POINTER(abi.Range), C.c_int, _ip, _ip, _ip]
         L.pgw_tune_string.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
         L.pgw_fstatnet.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp]
         L.pgw_fstatnet_multi.argtypes = [C.c_void_p, _ip, _dp, C.c_int, _dp, _dp]
@@ -284,14 +284,14 @@ class Handle:
             raise PGWError(st, "pgw_search")
         return fn.value
 
-    def autotune(self, seg, pm, mm, nn, opts, rng=None):
-        """pgw_autotune, or with a spindown range rng pgw_autotune_range (the templates a search
-        with that range forms at the sky point)."""
+    def autotune(self, seg, pm, mm, nn, opts, rng=None, nsky=4):
+        """pgw_autotune, or with a spindown range rng pgw_autotune_range (calls of nsky copies of
+        the sky point with the templates a search with that range forms there)."""
         tb, na, seq = C.c_int(), C.c_int(), C.c_int()
         if rng is None:
             st = lib().pgw_autotune(self.h, seg, pm, mm, nn, C.byref(opts), C.byref(tb), C.byref(na), C.byref(seq))
         else:
-            st = lib().pgw_autotune_range(self.h, seg, pm, mm, nn, C.byref(opts), C.byref(rng), C.byref(tb),
+            st = lib().pgw_autotune_range(self.h, seg, pm, mm, nn, C.byref(opts), C.byref(rng), nsky, C.byref(tb),
                                           C.byref(na), C.byref(seq))
         if st != 0:
             raise PGWError(st, "pgw_autotune")

@@ -264,7 +264,7 @@ def test_tune_string(pgw, abi):
     assert int(kv["tb"]) == choice["tb"] and int(kv["col_seq"]) == choice["col_seq"]
     # tuned over the templates of the workload's spindown range (pgw_autotune_range)
     h32 = pgw.Handle(seg, abi.PREC_FP32)
-    choice = h32.autotune(0, pm, mm, nn, abi.make_opts(trl=12.0), abi.make_range(seg))
+    choice = h32.autotune(0, pm, mm, nn, abi.make_opts(trl=12.0), abi.make_range(seg), 8)
     kv = dict(x.split("=") for x in h32.tune_string().split(","))
     print(choice, kv, flush=True)
     assert int(kv["tb"]) == choice["tb"] and int(kv["col_seq"]) == choice["col_seq"]
