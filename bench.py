@@ -300,10 +300,13 @@ def bench_templates(args, name, steps=None, warmup=None, per_step=None, cpu=True
     per_step = per_step or min(len(sky), {"network": 16, "plain": 4, "precision": 64}[name])
     warmup = args.warmup if warmup is None else warmup
     if steps is None:
-        # plain: 4 sky points are one step of 0.3 ms; 256 repetitions time ~70 ms, so one-time
+        # plain: 4 sky points are one step of ~0.2 ms; 1024 repetitions time ~0.2 s, after at least
+        # 32 untimed ones: the device's clocks ramp up over the first ~100 steps, and one-time
         # host-side costs of the runtime (a stall of a few ms after ~1000 queued operations was
         # seen) do not dominate the timed region
-        steps = max(1, len(sky) // per_step) if name != "plain" else 256
+        steps = max(1, len(sky) // per_step) if name != "plain" else 1024
+        if name == "plain":
+            warmup = max(warmup, 32)
     s = seg.sett
     print(f"workload {name}: N={s['N']} nfft={s['nfft']} fftpad={s['fftpad']} L={s['nfftf']} "
           f"sky={len(sky)} spindowns={nspin}", file=sys.stderr, flush=True)
