@@ -643,8 +643,14 @@ struct SlotBases {
 // het1 and sgnlt[1] of the templates of a launch of at most kSpinTab templates, passed by value as
 // a kernel argument: a block reads its entry with one scalar load from the argument segment.
 constexpr int kSpinTab = 128;
+// Launches spanning several sky points (slot of template t in slot[t], data length of a slot in
+// slotN) read their templates' sky slots from the table as well, so a group whose launches all
+// fit the table needs no descriptors on the device.
+constexpr int kSpinSlots = 16;
 struct SpinTab {
   double het1[kSpinTab], sgnlt1[kSpinTab];
+  unsigned char slot[kSpinTab];
+  int slotN[kSpinSlots];
 };
 
 struct SpinParams {
@@ -729,7 +735,14 @@ struct RowParams {
   int L2, nmin, nmax, nblk, nparts, lgL2;  // L2 = 1 << lgL2
   int nt;   // spectra read with non-temporal loads
   int fnt;  // F written with non-temporal stores
+  // sky slot of template t: uslot (>= 0: the launch's one slot), else slot[t] (tab = 1) or the
+  // descriptor's
+  int uslot, tab;
+  unsigned char slot[kSpinTab];
 };
+__device__ __forceinline__ int row_slot(const RowParams& p, const TmplDesc* __restrict__ desc, int t) {
+  return p.uslot >= 0 ? p.uslot : p.tab ? (int)p.slot[t] : desc[t].slot;
+}
 
 struct Cand {
   int t, ii;   // template index within the group, frequency bin

@@ -79,6 +79,11 @@ __global__ __launch_bounds__(NTH, (ColOcc<T, P2>::W)) void k_spin_col2(SpinParam
       het1 = ld_global(&desc[t].het1, 0);
       sgnlt1 = ld_global(&desc[t].sgnlt1, 0);
     }
+  } else if (sb.tab) {
+    slot = tab.slot[t];
+    N = tab.slotN[slot];
+    het1 = tab.het1[t];
+    sgnlt1 = tab.sgnlt1[t];
   } else {
     const TmplDesc& dsc = desc[t];
     slot = dsc.slot;
@@ -266,7 +271,7 @@ __global__ __launch_bounds__(P1 / 16, (RowOcc<T, P1, NA>::W)) void k_spin_row2(R
   const uint32_t vr = (uint32_t)tid * (uint32_t)sizeof(T2), mb = NT * (uint32_t)sizeof(T2);
   const uint32_t ra = (uint32_t)(k2 * P1) * (uint32_t)sizeof(T2), rb = ra + (uint32_t)(L * sizeof(T2));
   auto ldz = [&](uint32_t off) { return p.nt ? Zr.load<T2, 2>(vr, off) : Zr.load<T2>(vr, off); };
-  const double* aabb = aabb_slots + 2 * desc[t].slot;
+  const double* aabb = aabb_slots + 2 * row_slot(p, desc, t);
   // F = |xa|^2/aa + |xb|^2/bb (jobcore.c:635-638) with the two reciprocals taken once: each
   // product differs from the quotient by at most one rounding
   const T raa = (T)(1.0 / ld_global(aabb, 0)), rbb = (T)(1.0 / ld_global(aabb, 1));
@@ -408,7 +413,7 @@ __global__ __launch_bounds__(P1 / 8, 4) void k_spin_row8(RowParams p, const doub
   const uint32_t vr = (uint32_t)tid * (uint32_t)sizeof(T2), mb = NT * (uint32_t)sizeof(T2);
   const uint32_t ra = (uint32_t)(k2 * P1) * (uint32_t)sizeof(T2), rb = ra + (uint32_t)(L * sizeof(T2));
   auto ldz = [&](uint32_t off) { return p.nt ? Zr.load<T2, 2>(vr, off) : Zr.load<T2>(vr, off); };
-  const double* aabb = aabb_slots + 2 * desc[t].slot;
+  const double* aabb = aabb_slots + 2 * row_slot(p, desc, t);
   const T raa = (T)(1.0 / ld_global(aabb, 0)), rbb = (T)(1.0 / ld_global(aabb, 1));
   constexpr int MO = HALF ? 4 : 8;  // output slots that can hold a bin of [nmin, nmax)
   T F[MO];

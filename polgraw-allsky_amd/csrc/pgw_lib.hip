@@ -49,6 +49,7 @@ constexpr int kCandChunk = 4096;    // candidates copied back with the count, wi
 // keeps counting past the capacity; the error word is set by a fused launch whose dependency wait
 // ran out), the candidates from [1].  One copy brings the header and the first kCandChunk back.
 constexpr int kSlots = 16;          // sky points per group
+static_assert(kSlots <= kSpinSlots, "a launch's sky slots fit the kernel-argument table");
 constexpr int kDescCap = 1 << 16;   // templates per group
 constexpr int kTBMax = 256;         // templates per launch, upper bound
 constexpr size_t kScratchDoubles = PGW_FSN_SCRATCH_BYTES / sizeof(double);
@@ -956,8 +957,22 @@ template <typename T> struct Engine final : pgw_handle {
         const SpinGrid q = spin_grid(segs[uni->seg]->sett.M, uni->mm, uni->nn, 0, o->s0_flag);
         for (int i = 0; i < nt; i++) spin_coeffs(q, uni->ss_at(j0 + i), tab.het1[i], tab.sgnlt1[i]);
       }
+    } else if (nt <= kSpinTab) {
+      // templates of several sky points: their values from the host copy of the descriptors
+      const TmplDesc* hd = desc_pin.p + (size_t)buf * kDescCap + t0;
+      sb.tab = 1;
+      for (int i = 0; i < kSpinSlots; i++) tab.slotN[i] = 0;
+      for (int i = 0; i < nt; i++) {
+        tab.het1[i] = hd[i].het1;
+        tab.sgnlt1[i] = hd[i].sgnlt1;
+        tab.slot[i] = (unsigned char)hd[i].slot;
+        tab.slotN[hd[i].slot] = hd[i].N;
+      }
     }
-    RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2), row_nt ? 1 : 0, f_nt ? 1 : 0};
+    RowParams rp{g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, ilog2(g.ps->L2), row_nt ? 1 : 0, f_nt ? 1 : 0,
+                 sb.uslot, sb.tab, {}};
+    if (sb.uslot < 0 && sb.tab)
+      for (int i = 0; i < nt; i++) rp.slot[i] = tab.slot[i];
     ThrParams tp{g.ps->L1, g.ps->L2, g.sett.nmin, g.sett.nmax, g.nblk, g.nparts, o->white_flag, t0, ccap, ilog2(g.ps->L2), o->trl};
     if constexpr (std::is_same<T, float>::value) {
       // one persistent kernel for the whole template stage (its threshold included)
@@ -1037,8 +1052,11 @@ template <typename T> struct Engine final : pgw_handle {
     if ((st = sky_batch(*segs[gr.items[0].seg], si, (int)gr.items.size(), gr.buf, &gr.cbuf->t))) return st;
     TmplDesc* hd = desc_pin.p + (size_t)gr.buf * kDescCap;
     for (const Item& it : gr.items) fill_desc(it, o, hd);
-    HIPCHK(hipMemcpyAsync(desc.p + (size_t)gr.buf * kDescCap, hd, sizeof(TmplDesc) * gr.ntempl, hipMemcpyHostToDevice,
-                          stream));
+    // launches of at most kSpinTab templates take their templates' values from kernel arguments;
+    // only larger launches and the persistent template-stage kernel read descriptors on the device
+    if (TB > kSpinTab || fused)
+      HIPCHK(hipMemcpyAsync(desc.p + (size_t)gr.buf * kDescCap, hd, sizeof(TmplDesc) * gr.ntempl,
+                            hipMemcpyHostToDevice, stream));
     const Seg& g0 = *segs[gr.items[0].seg];
     int Ng = 0;  // largest N of the group
     for (const Item& it : gr.items) Ng = std::max(Ng, segs[it.seg]->sett.N);
