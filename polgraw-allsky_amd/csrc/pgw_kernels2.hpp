@@ -36,12 +36,17 @@ template <> struct ColE<double, 256> { static constexpr int E = 16; static const
 // of them is used (indexes clamped into the series, rows past N add nothing), so a wave waits on
 // its loads once per detector instead of once per row.  Built for FP32 with E = 16 only: at
 // E = 8 the extra registers cost waves per SIMD and the pass is 12-15 % slower.
-// minimum waves per SIMD of the column pass (FP32 with 8 register rows: PGW_COL32_W)
+// minimum waves per SIMD of the column pass (FP32 with 8 register rows: PGW_COL32_W; FP64 with 8
+// register rows: PGW_COL64_W = 4, i.e. at most 128 VGPRs: the 64-point pass needed 129 and ran at 3
+// waves per SIMD; at 4 (12 bytes spilled) it is 3.5 % faster)
 #ifndef PGW_COL32_W
 #define PGW_COL32_W 1
 #endif
+#ifndef PGW_COL64_W
+#define PGW_COL64_W 4
+#endif
 template <typename T, int P2> struct ColOcc {
-  static constexpr int W = (std::is_same<T, float>::value && ColE<T, P2>::E == 8) ? PGW_COL32_W : 1;
+  static constexpr int W = ColE<T, P2>::E != 8 ? 1 : std::is_same<T, float>::value ? PGW_COL32_W : PGW_COL64_W;
 };
 
 // NTH: threads per workgroup (256, or 512 for tiles of twice the columns: each store instruction
