@@ -442,8 +442,11 @@ __global__ __launch_bounds__(NT) void k_spline_eval(const cplx<T>* __restrict__ 
 // workgroup's outputs are the i in [min{i : x_i >= A0}, min{i : x_i >= A1}) (from 0 in the first
 // workgroup, to N in the last); each bound is found by one wave from a guess at A / interpftpad +
 // shftf and a 64-wide window around it.  A bound outside its window (the shift would have to change
-// by more than 30 samples over the distance of the shift) sets *err and leaves the outputs
-// unwritten.  y2 at A1 (the knot above the last output knot) comes from the last chunk's warm-up.
+// by more than 30 samples over the distance of the shift), or an output whose knots are not this
+// workgroup's (abscissae that do not increase with i), sets *err and leaves the outputs
+// unwritten: the group's search returns PGW_ERR_DEVICE (the three-kernel path, spl_fuse=0, takes
+// any abscissae).  y2 at A1 (the knot above the last output knot) comes from the last chunk's
+// warm-up.
 template <typename T, int K, bool HALF>
 __global__ __launch_bounds__(NT) void k_spline_fused(const cplx<T>* __restrict__ Zi, int L1, int lgL2, T ft,
                                                      size_t zstride, const cplx<T>* __restrict__ R, size_t rstride,
@@ -583,6 +586,10 @@ __global__ __launch_bounds__(NT) void k_spline_fused(const cplx<T>* __restrict__
     double vr = 0.0, vi = 0.0;
     if (!(x < 0 || x > n - 1)) {
       const int klo = (int)floor(x), khi = klo + 1;
+      if (klo < A0 || klo >= A1) {  // the abscissae do not increase with i: not this workgroup's knots
+        if (err) *err = 2;
+        continue;
+      }
       const double Aw = khi - x, Bw = x - klo;
       const double ca = Aw * Aw * Aw - Aw, cb = Bw * Bw * Bw - Bw;
       const cplx<T> yl = Ys[klo - Ylo];

@@ -3,6 +3,7 @@
 jobcore.c:424-444 spindown range without a range file; --nospindown (jobcore.c:444,456); the
 known-line veto (jobcore.c:676-681); the sky disk check (jobcore.c:248); the record buffer
 capacity (an error code instead of the reference's silent overflow); job_batch == job_core loop;
+an ephemeris outside the fused spline kernel's assumption (an error, and the three-kernel path);
 and the FP32 form against the FP64 oracle.
 """
 import numpy as np
@@ -99,6 +100,47 @@ def test_capacity_error(pgw, abi):
     with pytest.raises(pgw.PGWError) as e:
         h.job_core(pm, mm, nn, rng, opts, cap=2)
     assert e.value.status == abi.PGW_ERR_CAPACITY
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_nonmonotonic_shift_is_an_error(pgw, abi, prec):
+    """The fused spline kernel evaluates each output in the workgroup that holds its knots, which
+    needs the abscissae interpftpad (i - shftf_i) to increase with i (physical ephemerides change
+    the shift by far less than a sample per sample).  An ephemeris with a jump of thousands of
+    samples breaks that: the search must fail with PGW_ERR_DEVICE, and the three-kernel path
+    (spl_fuse=0, which evaluates every output on its own) must run.  The jump is 3000 samples of
+    shift along the sky point's direction (from a record of the unmodified segment), so the
+    abscissae fall back by 6000 knots, more than one workgroup's knots."""
+    import dataclasses
+    import os
+    base = segment("plain-1det-32k")
+    N = base.sett["N"]
+    rng = abi.make_range(base)
+    opts = abi.make_opts(trl=20.0)
+    pm, mm, nn = injected_sky(base)
+    p = abi.PREC_FP64 if prec == "fp64" else abi.PREC_FP32
+    recs, fn = pgw.Handle(base, p).job_core(pm, mm, nn, rng, opts)
+    assert len(recs) > 0
+    delta, alpha = float(recs[0][2]), float(recs[0][3])
+    nsrc = np.array([np.cos(alpha) * np.cos(delta), np.sin(alpha) * np.cos(delta), np.sin(delta)])
+    ssb = base.DetSSB[0].copy().reshape(N, 3)
+    ssb[N // 2:] += 3000.0 * nsrc
+    seg = dataclasses.replace(base, DetSSB=[ssb.reshape(-1)])
+    rng = abi.make_range(seg)
+    with pytest.raises(pgw.PGWError) as e:
+        pgw.Handle(seg, p).job_core(pm, mm, nn, rng, opts)
+    assert e.value.status == abi.PGW_ERR_DEVICE
+    old = os.environ.get("PGW_TUNE")
+    os.environ["PGW_TUNE"] = "spl_fuse=0"
+    try:
+        h = pgw.Handle(seg, p)
+    finally:
+        if old is None:
+            os.environ.pop("PGW_TUNE", None)
+        else:
+            os.environ["PGW_TUNE"] = old
+    recs, fn = h.job_core(pm, mm, nn, rng, opts)
+    assert fn > 0
 
 
 @pytest.mark.parametrize("name", ["net-2det-86k", "plain-1det-86k", "batch-3det-60k", "prec-2det-259k"])
