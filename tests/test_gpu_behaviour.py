@@ -143,6 +143,48 @@ def test_nonmonotonic_shift_is_an_error(pgw, abi, prec):
     assert fn > 0
 
 
+def test_small_shift_jump_is_exact_or_an_error(pgw, abi):
+    """A jump of 10 samples of shift drops the abscissae by 18 knots: the outputs after it belong
+    to the workgroup of the knots before it unless the drop crosses a workgroup's first knot.  The
+    fused spline kernel must then give the three-kernel path's records (fields 0-3 equal, F within
+    1e-9) or fail with PGW_ERR_DEVICE, never other records."""
+    import dataclasses
+    import os
+    base = segment("plain-1det-32k")
+    N = base.sett["N"]
+    rng = abi.make_range(base)
+    opts = abi.make_opts(trl=20.0)
+    pm, mm, nn = injected_sky(base)
+    recs, _ = pgw.Handle(base, abi.PREC_FP64).job_core(pm, mm, nn, rng, opts)
+    delta, alpha = float(recs[0][2]), float(recs[0][3])
+    nsrc = np.array([np.cos(alpha) * np.cos(delta), np.sin(alpha) * np.cos(delta), np.sin(delta)])
+    ssb = base.DetSSB[0].copy().reshape(N, 3)
+    ssb[N // 2 + 777:] += 10.0 * nsrc
+    seg = dataclasses.replace(base, DetSSB=[ssb.reshape(-1)])
+    rng = abi.make_range(seg)
+    old = os.environ.get("PGW_TUNE")
+    os.environ["PGW_TUNE"] = "spl_fuse=0"
+    try:
+        h = pgw.Handle(seg, abi.PREC_FP64)
+    finally:
+        if old is None:
+            os.environ.pop("PGW_TUNE", None)
+        else:
+            os.environ["PGW_TUNE"] = old
+    ref, fr = h.job_core(pm, mm, nn, rng, opts)
+    assert fr > 0 and len(ref) > 0
+    try:
+        got, fg = pgw.Handle(seg, abi.PREC_FP64).job_core(pm, mm, nn, rng, opts)
+    except pgw.PGWError as e:
+        assert e.status == abi.PGW_ERR_DEVICE
+        print("fused spline: PGW_ERR_DEVICE (the drop crosses a workgroup's first knot)")
+        return
+    assert fg == fr and got.shape == ref.shape
+    assert np.array_equal(got[:, :4], ref[:, :4])
+    assert np.max(np.abs(got[:, 4] - ref[:, 4]) / ref[:, 4]) <= 1e-9
+    print("fused spline: records equal to the three-kernel path's")
+
+
 @pytest.mark.parametrize("name", ["net-2det-86k", "plain-1det-86k", "batch-3det-60k", "prec-2det-259k"])
 def test_fp32_form_vs_fp64_oracle(name, pgw, abi, orc):
     """FP32 series/FFTs (FP64 phases and spline coefficients) vs the FP64 oracle: max relative dF
