@@ -8,7 +8,7 @@ fftpad 1), 32 sky points x 512 spindowns per segment = 2048 work items x 512 = 1
 in one handle, threshold F = 20.
 
 A step is one pgw_job_batch_multi call over `--sky-per-step` work items (default 256 items =
-131072 templates); 8 steps cover the workload once, and the default 16 steps cover it twice.
+131072 templates); 8 steps cover the workload once, and the default 24 steps cover it three times.
 Inputs are resident in HBM before the timed region.  The headline value is the FP32 form (FP64
 phases and spline coefficients, FP32 series and FFTs); the FP64 path is timed the same way over
 the same work items, and the FP32 candidates and F values are compared with the FP64 ones.
@@ -358,8 +358,8 @@ def bench_batching(args):
     work = [(k, pm, mm, nn) for k, sg in enumerate(segs) for pm, mm, nn in synth.sky_list(sg)]
     per_step = args.sky_per_step or 256
     if args.steps is None:
-        # two passes over the 2048 work items: >= 2 s of timed work per precision
-        args.steps = 2 * max(1, len(work) // per_step)
+        # three passes over the 2048 work items: >= 2 s of timed work per precision
+        args.steps = 3 * max(1, len(work) // per_step)
     n16 = sum(1 for sg in segs if sg.sett["nfft"] == 1 << 16)
     print(f"workload batching: {len(segs)} segments ({n16} with nfft 2^16), {len(work)} work items",
           file=sys.stderr, flush=True)
